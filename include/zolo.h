@@ -1,0 +1,114 @@
+/*
+ * zolo.h -- C ABI of the MI355X (gfx950) Zolo-PD / QDWH-PD library (libzolo.so).
+ *
+ * All matrices are column-major FP64 with an explicit leading dimension, as on
+ * a 1x1 ScaLAPACK grid of the reference.  Every entry point returns an int
+ * status: 0 on success, a negative value -i when argument i is illegal
+ * (ScaLAPACK convention), or one of the positive ZOLO_ERR_* codes below.  The
+ * library never aborts; zolo_last_error() describes the last failure.
+ *
+ * Threading: calls are serialised on one device context (a grow-only device
+ * workspace owned by the library); they are not re-entrant.  `stream` is a
+ * hipStream_t (NULL = the default stream).
+ */
+#ifndef ZOLO_H_
+#define ZOLO_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZOLO_OK 0
+#define ZOLO_ERR_HIP 1001            /* HIP runtime error */
+#define ZOLO_ERR_NOT_SPD 1002        /* a shifted Gram matrix was not positive definite */
+#define ZOLO_ERR_NO_CONVERGENCE 1003 /* iteration cap reached above tolerance */
+#define ZOLO_ERR_ALLOC 1004          /* device allocation failed */
+#define ZOLO_ERR_INTERNAL 1005
+
+/* Arithmetic of the Cholesky-based iterations. */
+#define ZOLO_PREC_F64 0     /* FP64 throughout (the reference's arithmetic)               */
+#define ZOLO_PREC_F32 1     /* FP32 Cholesky iterations, FP64 QR step + polish            */
+#define ZOLO_PREC_BF16X3 2  /* BF16x3 MFMA Cholesky iterations, FP64 QR step + polish     */
+
+/* Per-run record: iteration trajectory and phase times. */
+typedef struct zolo_info {
+  int iters;              /* iterations executed                                       */
+  int r;                  /* Zolotarev degree (number of shifts)                       */
+  int info;               /* first non-positive pivot column + 1 of a shifted Cholesky */
+  int polish_passes;      /* FP64 polish passes executed (mixed precision only)         */
+  char schedule[16];      /* per iteration: 'Q' QR step, 'C' FP64 Cholesky, 'c' low prec*/
+  double err[16];         /* ERR = ||X+ - X||_F after each FormX2                      */
+  double con[16];         /* CON at the start of each iteration                        */
+  double maxc[16];        /* max c_i of each iteration                                 */
+  double alpha, beta;     /* bounds used                                               */
+  double time_ms;         /* device time of the whole call (hipEvent)                  */
+  double phase_ms[8];     /* 0 bounds, 1 QR steps, 2 Cholesky steps, 3 FormX2,
+                             4 polish, 5 H, 6 other                                    */
+} zolo_info_t;
+
+/* ---- library lifetime ---------------------------------------------------- */
+int zolo_init(int device);
+int zolo_finalize(void);
+const char* zolo_last_error(void);
+int zolo_version(void);
+
+/* ---- scalar helpers: Auxil_Zolo.f90:36-200, hfunc.f90:2-42 ---------------- */
+/* coeff[0:2r] = c_1..c_2r, coeff[2r:3r] = a_1..a_r (Compute_Coeff, Auxil_Zolo.f90:165) */
+int zolo_compute_coeff(double con, double tol, int rk, double* coeff);
+int zolo_choosem(double con);                                          /* Auxil_Zolo.f90:130 */
+double zolo_hfunc(double x);                                           /* hfunc.f90:2        */
+void zolo_mellipke(double alpha, double tol, double* k, double* e);    /* Auxil_Zolo.f90:36  */
+void zolo_mellipj(double u, double alpha, double tol, double* sn, double* cn, double* dn); /* :73 */
+
+/* ---- Zolo-PD: replaces PZOLOPD1 (pzolopd1.f90:2) on a 1x1 grid ----------------
+ * U = polar factor of A (n x n), H = (U^T A + A^T U)/2 if H != NULL.
+ * sym = 'S' or 'N' (zolochol.f90:99), rk = number of shifts (1..8),
+ * alpha, beta as in pzolopd1: X0 = A/(alpha*beta), CON = 1/beta.
+ * Device pointers; A is read-only. */
+int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
+                   int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream);
+/* Same with host arrays (copied to and from the device). */
+int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
+                 int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info);
+
+/* ---- Bounds: alpha = ||A||_F, beta from LU + 1-norm estimate ---------------
+ * test_zolodrv1.f90:179-189 (convention 0: beta = rcond/sqrt(n)) and
+ * pqdwhfacs.f90:318-344 (convention 1: beta = ||A/alpha||_1 rcond/sqrt(n)). */
+int zolo_bounds_device(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta,
+                       void* stream);
+
+/* ---- QDWH-PD: replaces PQDWHFACS (pqdwhfacs.f90:2) on a 1x1 grid ---------- */
+int zolo_qdwh_device(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu,
+                     double* H, int64_t ldh, zolo_info_t* info, void* stream);
+int zolo_qdwh_host(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu, double* H,
+                   int64_t ldh, zolo_info_t* info);
+
+/* ---- Single-shift steps: replace ZOLOCHOL (zolochol.f90:2) / ZOLOQR (zoloqr.f90:2)
+ * Z = -a_ip X (X^T X + c_{2ip-1} I)^{-1}  and  Z = -a_ip/sqrt(c) Q1 Q2^T, ip 1-based.
+ * coeff holds 3*rk values as from zolo_compute_coeff. Host arrays. */
+int zolo_chol_step_host(char sym, int64_t n, const double* X, int64_t ldx, double* Z, int64_t ldz, const double* coeff,
+                        int rk, int ip);
+int zolo_qr_step_host(int64_t n, const double* X, int64_t ldx, double* Z, int64_t ldz, const double* coeff, int rk,
+                      int ip);
+
+/* ---- 2-norm estimate: replaces PDNORMEST (pdnormest.f90:2) ----------------- */
+int zolo_normest_device(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt,
+                        void* stream);
+
+/* ---- building blocks (device pointers; used by tests and the benchmark) --- */
+/* C = alpha op(A) op(B) + beta C, precision 0 = F64, 1 = F32 (float data), 2 = BF16x3 (float data) */
+int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                     const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc,
+                     void* stream);
+/* Structured QR of the 2n x n stack S = [X; s I] in place (R + reflectors), Q (2n x n) out, tau (n) out.
+ * Restates MPDGEQRF + MPDORGQR (mpdgeqrf.f:5, mpdorgqr.f:5). */
+int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t ldq, double* tau, void* stream);
+/* Lower Cholesky with explicit inverse: A (lower) -> L, Li = L^{-1}. */
+int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* Li, int64_t ldl, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZOLO_H_ */

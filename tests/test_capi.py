@@ -1,0 +1,67 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol that
+include/zolo.h declares, and its host-side scalar math (the Zolotarev
+coefficients, the degree rule and the QDWH weight) equals the oracle's."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import zolo_oracle as zo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "zolo.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(zolo_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(zl):
+    names = _declared()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(zl.LIB, name), name
+
+
+def test_version(zl):
+    assert zl.LIB.zolo_version() >= 100
+
+
+@pytest.mark.parametrize("con", [1.0005, 1.3, 1.9, 3.0, 150.0, 1e4, 1e6, 1e8, 1e12, 1e15])
+@pytest.mark.parametrize("rk", [1, 2, 5, 8])
+def test_coeff_matches_oracle(zl, con, rk):
+    got = zl.compute_coeff(con, zo.DLAMCH_P, rk)
+    ref = zo.compute_coeff(con, zo.DLAMCH_P, rk)
+    assert np.allclose(got, ref, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("con", [1.0, 1.001, 1.01, 1.1, 1.2, 1.5, 2.0, 6.49, 6.5, 179.9, 1.5e4, 2e6, 1e9, 3e12, 1e20])
+def test_choosem_matches_oracle(zl, con):
+    assert zl.choosem(con) == zo.choosem(con)
+
+
+@pytest.mark.parametrize("x", [1e-12, 1e-6, 0.01, 0.5, 0.999, 1.0])
+def test_hfunc_matches_oracle(zl, x):
+    assert math.isclose(zl.hfunc(x), zo.hfunc(x), rel_tol=1e-14)
+
+
+def test_elliptic_matches_oracle(zl):
+    for alpha in [0.1, 1.0, 1.5, math.acos(1e-8)]:
+        k, e = zl.mellipke(alpha, zo.DLAMCH_P)
+        ko, eo = zo.mellipke(alpha, zo.DLAMCH_P)
+        assert math.isclose(k, ko, rel_tol=1e-14) and math.isclose(e, eo, rel_tol=1e-12, abs_tol=1e-14)
+        for u in [0.1, 0.7 * k]:
+            got = zl.mellipj(u, alpha, zo.DLAMCH_P)
+            ref = zo.mellipj(u, alpha, zo.DLAMCH_P)
+            assert np.allclose(got, ref, rtol=1e-13, atol=1e-15)
+
+
+def test_argument_validation_without_gpu(zl):
+    with pytest.raises(zl.ZoloError) as e:
+        zl.compute_coeff(0.5, 1e-16, 3)
+    assert e.value.status == -1
+    with pytest.raises(zl.ZoloError):
+        zl.compute_coeff(2.0, 1e-16, 9)

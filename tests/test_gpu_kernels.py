@@ -1,0 +1,114 @@
+"""GPU parity of the building blocks: MFMA GEMM (F64 / F32 / BF16x3), the
+recursive Cholesky + inverse, the structured Householder QR, and the single
+shift steps ZOLOCHOL / ZOLOQR against the CPU oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import zolo_oracle as zo
+
+pytestmark = pytest.mark.gpu
+
+
+def _gemm_case(torch, zl, prec, ta, tb, m, n, k, beta, seed):
+    rng = np.random.default_rng(seed)
+    dt = np.float64 if prec == "f64" else np.float32
+    Ah = rng.standard_normal((k, m) if ta else (m, k)).astype(dt)
+    Bh = rng.standard_normal((n, k) if tb else (k, n)).astype(dt)
+    Ch = rng.standard_normal((m, n)).astype(dt)
+    dA, dB, dC = zl.to_device(Ah), zl.to_device(Bh), zl.to_device(Ch)
+    zl.gemm_device(ta, tb, m, n, k, 1.5, dA, Ah.shape[0], dB, Bh.shape[0], beta, dC, m, precision=prec)
+    torch.cuda.synchronize()
+    got = zl.from_device(dC)
+    opA = Ah.T.astype(np.float64) if ta else Ah.astype(np.float64)
+    opB = Bh.T.astype(np.float64) if tb else Bh.astype(np.float64)
+    ref = 1.5 * opA @ opB + beta * Ch.astype(np.float64)
+    scale = 1.5 * np.abs(opA) @ np.abs(opB) + abs(beta) * np.abs(Ch)
+    return float(np.max(np.abs(got - ref) / scale))
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6), ("bf16x3", 2e-5)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(200, 130, 70), (64, 257, 300), (300, 40, 129), (1, 1, 1)])
+def test_gemm(torch_gpu, zl, prec, tol, ta, tb, m, n, k):
+    # tolerance is relative to |A||B| elementwise: a fixed multiple of the unit roundoff
+    err = _gemm_case(torch_gpu, zl, prec, ta, tb, m, n, k, 0.5, m * 7 + n + k)
+    assert err < tol * max(1, math.sqrt(k)), err
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("n", [37, 64, 200, 513])
+def test_chol_inv(torch_gpu, zl, prec, n):
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    W = G.T @ G / n + 0.5 * np.eye(n)
+    dt = np.float64 if prec == "f64" else np.float32
+    dA = zl.to_device(W.astype(dt))
+    dLi = zl.to_device(np.zeros((n, n), dtype=dt))
+    zl.chol_inv_device(dA, dLi, precision=prec)
+    torch_gpu.cuda.synchronize()
+    Li = zl.from_device(dLi).astype(np.float64)
+    L = np.linalg.cholesky(W)
+    tol = 1e-12 if prec == "f64" else 1e-4
+    assert np.allclose(np.triu(Li, 1), 0.0)
+    assert np.linalg.norm(Li @ L - np.eye(n)) / math.sqrt(n) < tol
+    assert np.linalg.norm(Li - np.linalg.inv(L)) / np.linalg.norm(np.linalg.inv(L)) < tol
+
+
+@pytest.mark.parametrize("n,s", [(5, 1.0), (64, 1e-3), (100, 0.5), (130, 1e4), (257, 2.0)])
+def test_stacked_qr(torch_gpu, zl, n, s):
+    """MPDGEQRF+MPDORGQR restatement vs the oracle's structured Householder QR."""
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    S = np.zeros((2 * n, n))
+    S[:n] = X
+    S[n:] = s * np.eye(n)
+    dS = zl.to_device(S)
+    dQ = zl.to_device(np.zeros((2 * n, n)))
+    dtau = torch_gpu.zeros(n, dtype=torch_gpu.float64, device="cuda")
+    zl.stacked_qr_device(dS, n, dQ, dtau)
+    torch_gpu.cuda.synchronize()
+    F = zl.from_device(dS)
+    Q = zl.from_device(dQ)
+    tau = dtau.cpu().numpy()
+    Fo, tauo = zo.structured_geqrf(S, n)
+    Qo = zo.structured_orgqr(Fo, tauo, n)
+    R = np.triu(F[:n])
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13 * n
+    assert np.linalg.norm(Q @ R - S) / np.linalg.norm(S) < 1e-14 * n
+    # same reflectors as the oracle (Householder QR is unique given the sign convention)
+    assert np.allclose(tau, tauo, rtol=1e-9, atol=1e-12)
+    assert np.linalg.norm(Q - Qo) < 1e-11 * n
+    assert np.linalg.norm(np.triu(F[:n]) - np.triu(Fo[:n])) / np.linalg.norm(np.triu(Fo[:n])) < 1e-12 * n
+
+
+@pytest.mark.parametrize("sym", ["N", "S"])
+@pytest.mark.parametrize("n,con,rk", [(96, 1.5, 3), (130, 40.0, 4), (64, 1e3, 2)])
+def test_zolochol(zl, sym, n, con, rk):
+    rng = np.random.default_rng(n + rk)
+    U = zo.haar(n, rng)
+    V = U if sym == "S" else zo.haar(n, rng)
+    s = np.geomspace(1.0, con, n)
+    X = (U * s) @ V.T
+    coeff = zo.compute_coeff(con, zo.DLAMCH_P, rk)
+    for ip in range(1, rk + 1):
+        Zg = zl.zolochol(X, coeff, rk, ip, sym)
+        Zo = zo.zolochol_step(X, coeff, rk, ip, sym)
+        assert np.linalg.norm(Zg - Zo) / np.linalg.norm(Zo) < 1e-11, ip
+
+
+@pytest.mark.parametrize("n,con,rk", [(96, 1e6, 5), (130, 1e10, 7), (64, 30.0, 2)])
+def test_zoloqr(zl, n, con, rk):
+    rng = np.random.default_rng(n + 17)
+    U, V = zo.haar(n, rng), zo.haar(n, rng)
+    s = np.geomspace(1.0, con, n)
+    X = (U * s) @ V.T
+    coeff = zo.compute_coeff(con, zo.DLAMCH_P, rk)
+    for ip in range(1, rk + 1):
+        Zg = zl.zoloqr(X, coeff, rk, ip)
+        Zo = zo.zoloqr_step(X, coeff, rk, ip)
+        # Z = -a X (X^T X + c I)^{-1} is as sensitive as X^{-T} when c << sigma^2:
+        # two backward-stable QRs agree to about eps * kappa(X)
+        tol = max(1e-11, 1e-15 * con)
+        assert np.linalg.norm(Zg - Zo) / max(np.linalg.norm(Zo), 1e-300) < tol, ip

@@ -1,0 +1,117 @@
+"""GPU parity of the polar-decomposition drivers against the CPU oracle and
+the closed form of the generator (A = U diag(s) V^T  =>  U_p = U V^T,
+H = V diag(s) V^T)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import zolo_oracle as zo
+
+pytestmark = pytest.mark.gpu
+
+
+def _errs(A, U, H):
+    return zo.backward_error(A, U, H), zo.orthogonality(U)[0]
+
+
+@pytest.mark.parametrize("n,kind,kappa,seed", [(64, "geometric", 1e6, 1), (200, "loguniform", 1e12, 2),
+                                               (150, "cluster", 1e8, 3), (128, "sine", 0, 4)])
+def test_pzolopd1_f64_matches_oracle(zl, n, kind, kappa, seed):
+    A, Ug, s, Vg = zo.make_matrix(n, kind, kappa, seed)
+    alpha, beta = zo.bounds_estimate(A)
+    rk = zo.choosem(1.0 / beta)
+    Uo, lo = zo.pzolopd1(A, alpha, beta, rk)
+    U, H, info = zl.pzolopd1(A, alpha, beta, rk)
+    # same schedule and iteration count as the reference restatement
+    sched = "".join("Q" if x == "QR" else "C" for x in lo.schedule)
+    assert info.schedule == sched and info.iters == lo.iters
+    assert np.allclose(info.con, lo.con, rtol=1e-9)
+    Ho = zo.polar_h(Uo, A)
+    be, orth = _errs(A, U, H)
+    beo, ortho = _errs(A, Uo, Ho)
+    assert be <= 10 * max(beo, 1e-15) and orth <= 10 * max(ortho, 1e-15), (be, beo, orth, ortho)
+    # distance to the oracle's polar factor and to the closed form
+    d_o = np.linalg.norm(U - Uo) / math.sqrt(n)
+    d_cf = np.linalg.norm(U - Ug @ Vg.T) / math.sqrt(n)
+    d_cf_o = np.linalg.norm(Uo - Ug @ Vg.T) / math.sqrt(n)
+    assert d_cf <= 10 * max(d_cf_o, 1e-14), (d_cf, d_cf_o)
+    assert d_o <= 10 * max(d_cf_o, 1e-14)
+    assert np.allclose(H, H.T)
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16x3"])
+@pytest.mark.parametrize("n,kind,kappa,seed", [(256, "cluster", 1e8, 5), (200, "loguniform", 1e12, 6)])
+def test_pzolopd1_mixed(zl, prec, n, kind, kappa, seed):
+    A, Ug, s, Vg = zo.make_matrix(n, kind, kappa, seed)
+    alpha, beta = zo.bounds_estimate(A)
+    rk = zo.choosem(1.0 / beta)
+    U, H, info = zl.pzolopd1(A, alpha, beta, rk, precision=prec)
+    assert info.schedule[0] == "Q" and set(info.schedule[1:]) <= {"c"}
+    assert info.polish_passes >= 1
+    be, orth = _errs(A, U, H)
+    assert be <= 1e-13 and orth <= 1e-13, (be, orth)
+    Uo, _ = zo.pzolopd1(A, alpha, beta, rk)
+    d_cf = np.linalg.norm(U - Ug @ Vg.T) / math.sqrt(n)
+    d_cf_o = np.linalg.norm(Uo - Ug @ Vg.T) / math.sqrt(n)
+    assert d_cf <= 10 * max(d_cf_o, 1e-14), (d_cf, d_cf_o)
+
+
+@pytest.mark.parametrize("n,kappa,seed", [(64, 1e6, 7), (160, 1e10, 8), (100, 10.0, 9)])
+def test_pqdwhfacs_matches_oracle(zl, n, kappa, seed):
+    A, Ug, s, Vg = zo.make_matrix(n, "geometric", kappa, seed)
+    Uo, Ho, lo = zo.pqdwhfacs(A)
+    U, H, info = zl.pqdwhfacs(A)
+    sched = "".join("Q" if x == "QR" else "C" for x in lo.schedule)
+    assert info.schedule == sched
+    assert abs(info.alpha - lo.alpha) <= 1e-13 * lo.alpha
+    assert abs(info.beta - lo.beta) <= 1e-8 * lo.beta
+    be, orth = _errs(A, U, H)
+    beo, ortho = _errs(A, Uo, Ho)
+    assert be <= 10 * max(beo, 1e-15) and orth <= 10 * max(ortho, 1e-15)
+    assert np.linalg.norm(U - Uo) <= 10 * max(np.linalg.norm(Uo - Ug @ Vg.T), 1e-13)
+
+
+@pytest.mark.parametrize("n,conv", [(50, "zolo"), (300, "zolo"), (257, "qdwh")])
+def test_bounds(torch_gpu, zl, n, conv):
+    A, *_ = zo.make_matrix(n, "geometric", 1e7, n)
+    dA = zl.to_device(A)
+    a, b = zl.bounds_device(dA, conv)
+    ao, bo = zo.bounds_estimate(A, conv)
+    assert abs(a - ao) <= 1e-13 * ao
+    assert abs(b - bo) <= 1e-8 * bo, (b, bo)
+
+
+@pytest.mark.parametrize("m,n", [(100, 100), (300, 120), (77, 200)])
+def test_normest(torch_gpu, zl, m, n):
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n))
+    dA = zl.to_device(A)
+    est, cnt = zl.pdnormest_device(dA, 1e-8)
+    eo, co = zo.normest(A, 1e-8)
+    assert abs(est - eo) <= 1e-10 * eo and cnt == co
+    # power iteration approaches ||A||_2 from below
+    assert est <= np.linalg.norm(A, 2) * (1 + 1e-12) and est >= 0.99 * np.linalg.norm(A, 2)
+
+
+def test_pzolopd1_device_entry(torch_gpu, zl):
+    n = 96
+    A, Ug, s, Vg = zo.make_matrix(n, "geometric", 1e5, 11)
+    alpha, beta = zo.bounds_estimate(A)
+    dA = zl.to_device(A)
+    dU, dH, info = zl.pzolopd1_device(dA, alpha, beta, 5)
+    torch_gpu.cuda.synchronize()
+    U = zl.from_device(dU)
+    Uh, _, _ = zl.pzolopd1(A, alpha, beta, 5)
+    assert np.linalg.norm(U - Uh) < 1e-13 * n
+    assert info.time_ms > 0
+
+
+def test_argument_errors(zl):
+    A = np.eye(4)
+    with pytest.raises(zl.ZoloError) as e:
+        zl.pzolopd1(A, 1.0, 0.5, 9)
+    assert e.value.status == -3
+    with pytest.raises(zl.ZoloError) as e:
+        zl.pzolopd1(A, 1.0, 2.0, 2)
+    assert e.value.status == -6

@@ -1,0 +1,293 @@
+// extern "C" entry points of libzolo.so (declared in include/zolo.h).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "coeff.hpp"
+#include "driver.hpp"
+
+namespace zolo {
+namespace {
+thread_local std::string g_err;
+}
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+}  // namespace zolo
+
+using namespace zolo;
+
+namespace {
+
+// Copies host column-major matrices to the device and back for the host-pointer entry points.
+struct DevMat {
+  double* p = nullptr;
+  ~DevMat() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(int64_t n) {
+    if (hipMalloc(&p, sizeof(double) * (size_t)n * n) != hipSuccess) {
+      p = nullptr;
+      set_error("host entry: hipMalloc failed");
+      return kErrAlloc;
+    }
+    return kOk;
+  }
+  int upload(const double* h, int64_t ld, int64_t n) {
+    ZOLO_HIP(hipMemcpy2D(p, n * sizeof(double), h, ld * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice));
+    return kOk;
+  }
+  int download(double* h, int64_t ld, int64_t n) {
+    ZOLO_HIP(hipMemcpy2D(h, ld * sizeof(double), p, n * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost));
+    return kOk;
+  }
+};
+
+int g_device = -1;
+
+int ensure_device() {
+  if (g_device < 0) {
+    ZOLO_HIP(hipGetDevice(&g_device));
+  }
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+int zolo_init(int device) {
+  if (device >= 0) ZOLO_HIP(hipSetDevice(device));
+  ZOLO_HIP(hipGetDevice(&g_device));
+  return kOk;
+}
+
+int zolo_finalize(void) {
+  workspace_release_all();
+  return kOk;
+}
+
+const char* zolo_last_error(void) { return last_error(); }
+
+int zolo_version(void) { return 100; }
+
+int zolo_compute_coeff(double con, double tol, int rk, double* coeff) {
+  if (!(con >= 1.0)) return -1;
+  if (rk < 1 || rk > 8) return -3;
+  compute_coeff(con, tol, rk, coeff);
+  return kOk;
+}
+
+int zolo_choosem(double con) { return choosem(con); }
+double zolo_hfunc(double x) { return hfunc(x); }
+void zolo_mellipke(double alpha, double tol, double* k, double* e) { mellipke(alpha, tol, k, e); }
+void zolo_mellipj(double u, double alpha, double tol, double* sn, double* cn, double* dn) {
+  mellipj(u, alpha, tol, sn, cn, dn);
+}
+
+int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
+                   int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream) {
+  ZOLO_CHECK(ensure_device());
+  return zolo_pd_run(precision, sym, rk, n, alpha, beta, A, lda, U, ldu, H, ldh, info, (hipStream_t)stream);
+}
+
+int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
+                 int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) {
+  if (n < 1) return -4;
+  if (lda < n) return -8;
+  if (ldu < n) return -10;
+  if (H && ldh < n) return -12;
+  ZOLO_CHECK(ensure_device());
+  DevMat dA, dU, dH;
+  ZOLO_CHECK(dA.alloc(n));
+  ZOLO_CHECK(dU.alloc(n));
+  if (H) ZOLO_CHECK(dH.alloc(n));
+  ZOLO_CHECK(dA.upload(A, lda, n));
+  const int st = zolo_pd_run(precision, sym, rk, n, alpha, beta, dA.p, n, dU.p, n, H ? dH.p : nullptr, n, info, nullptr);
+  if (st < 0) return st;
+  ZOLO_HIP(hipDeviceSynchronize());
+  ZOLO_CHECK(dU.download(U, ldu, n));
+  if (H) ZOLO_CHECK(dH.download(H, ldh, n));
+  return st;
+}
+
+int zolo_bounds_device(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta,
+                       void* stream) {
+  if (convention != 0 && convention != 1) return -1;
+  if (n < 1) return -2;
+  if (lda < n) return -4;
+  ZOLO_CHECK(ensure_device());
+  const size_t bytes = bounds_bytes(n);
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  return bounds_run(convention, n, A, lda, alpha, beta, ar, (hipStream_t)stream);
+}
+
+int zolo_qdwh_device(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu,
+                     double* H, int64_t ldh, zolo_info_t* info, void* stream) {
+  ZOLO_CHECK(ensure_device());
+  return qdwh_run(sym, n, sigma, A, lda, U, ldu, H, ldh, info, (hipStream_t)stream);
+}
+
+int zolo_qdwh_host(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu, double* H,
+                   int64_t ldh, zolo_info_t* info) {
+  if (n < 1) return -2;
+  if (lda < n) return -5;
+  if (ldu < n) return -7;
+  if (H && ldh < n) return -9;
+  ZOLO_CHECK(ensure_device());
+  DevMat dA, dU, dH;
+  ZOLO_CHECK(dA.alloc(n));
+  ZOLO_CHECK(dU.alloc(n));
+  if (H) ZOLO_CHECK(dH.alloc(n));
+  ZOLO_CHECK(dA.upload(A, lda, n));
+  const int st = qdwh_run(sym, n, sigma, dA.p, n, dU.p, n, H ? dH.p : nullptr, n, info, nullptr);
+  if (st < 0) return st;
+  ZOLO_HIP(hipDeviceSynchronize());
+  ZOLO_CHECK(dU.download(U, ldu, n));
+  if (H) ZOLO_CHECK(dH.download(H, ldh, n));
+  return st;
+}
+
+int zolo_chol_step_host(char sym, int64_t n, const double* X, int64_t ldx, double* Z, int64_t ldz, const double* coeff,
+                        int rk, int ip) {
+  if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -1;
+  if (n < 1) return -2;
+  if (ldx < n) return -4;
+  if (ldz < n) return -6;
+  if (rk < 1 || rk > 8) return -8;
+  if (ip < 1 || ip > rk) return -9;
+  ZOLO_CHECK(ensure_device());
+  DevMat dX, dZ;
+  ZOLO_CHECK(dX.alloc(n));
+  ZOLO_CHECK(dZ.alloc(n));
+  ZOLO_CHECK(dX.upload(X, ldx, n));
+  const size_t bytes = chol_step_bytes<double>(n, 1) + 8 * 256;
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  double* dsc = ar.take<double>(4);
+  int* dinfo = ar.take<int>(4);
+  const double hs[2] = {coeff[2 * ip - 2], -coeff[2 * rk + ip - 1]};
+  ZOLO_HIP(hipMemcpy(dsc, hs, sizeof(hs), hipMemcpyHostToDevice));
+  ZOLO_HIP(hipMemset(dinfo, 0, sizeof(int)));
+  Ctx c;
+  const bool left = (sym == 'S' || sym == 's');
+  ZOLO_CHECK(chol_step<double>(c, dX.p, n, n, 1, dsc, 1.0, 0.0, dsc + 1, dZ.p, n, n * n, ar, dinfo, left));
+  int hinfo = 0;
+  ZOLO_HIP(hipMemcpy(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost));
+  ZOLO_CHECK(dZ.download(Z, ldz, n));
+  if (hinfo) {
+    set_error("zolo_chol_step: not positive definite");
+    return kErrNotSpd;
+  }
+  return kOk;
+}
+
+int zolo_qr_step_host(int64_t n, const double* X, int64_t ldx, double* Z, int64_t ldz, const double* coeff, int rk,
+                      int ip) {
+  if (n < 1) return -1;
+  if (ldx < n) return -3;
+  if (ldz < n) return -5;
+  if (rk < 1 || rk > 8) return -7;
+  if (ip < 1 || ip > rk) return -8;
+  ZOLO_CHECK(ensure_device());
+  DevMat dX, dZ;
+  ZOLO_CHECK(dX.alloc(n));
+  ZOLO_CHECK(dZ.alloc(n));
+  ZOLO_CHECK(dX.upload(X, ldx, n));
+  const size_t bytes = qr_step_bytes<double>(n, 1) + 8 * 256;
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  double* dsc = ar.take<double>(4);
+  const double c = coeff[2 * ip - 2];
+  const double hs[2] = {std::sqrt(c), -coeff[2 * rk + ip - 1] / std::sqrt(c)};
+  ZOLO_HIP(hipMemcpy(dsc, hs, sizeof(hs), hipMemcpyHostToDevice));
+  Ctx cx;
+  ZOLO_CHECK(qr_step<double>(cx, dX.p, n, n, 1, dsc, dsc + 1, dZ.p, n, n * n, ar));
+  ZOLO_HIP(hipDeviceSynchronize());
+  ZOLO_CHECK(dZ.download(Z, ldz, n));
+  return kOk;
+}
+
+int zolo_normest_device(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt,
+                        void* stream) {
+  if (m < 1) return -1;
+  if (n < 1) return -2;
+  if (lda < m) return -4;
+  ZOLO_CHECK(ensure_device());
+  const size_t bytes = normest_bytes(m, n);
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  return normest_run(m, n, A, lda, tol, est, cnt, ar, (hipStream_t)stream);
+}
+
+int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                     const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc,
+                     void* stream) {
+  if (precision < 0 || precision > 2) return -1;
+  if (m < 0) return -4;
+  if (n < 0) return -5;
+  if (k < 0) return -6;
+  if (lda < std::max<int64_t>(1, transa ? k : m)) return -9;
+  if (ldb < std::max<int64_t>(1, transb ? n : k)) return -11;
+  if (ldc < std::max<int64_t>(1, m)) return -14;
+  GemmDesc d;
+  d.transA = transa; d.transB = transb; d.M = m; d.N = n; d.K = k;
+  d.A = A; d.lda = lda; d.B = B; d.ldb = ldb; d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
+  return gemm((Prec)precision, d, (hipStream_t)stream);
+}
+
+int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t ldq, double* tau, void* stream) {
+  if (n < 1) return -1;
+  if (lds < 2 * n) return -3;
+  if (Q && ldq < 2 * n) return -5;
+  ZOLO_CHECK(ensure_device());
+  const size_t bytes = qr_step_bytes<double>(n, 1);
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  Ctx c;
+  c.s = (hipStream_t)stream;
+  return stacked_qr<double>(c, S, lds, 0, Q, ldq, 0, n, 1, tau, ar);
+}
+
+int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* Li, int64_t ldl, void* stream) {
+  if (precision < 0 || precision > 2) return -1;
+  if (n < 1) return -2;
+  if (lda < n) return -4;
+  if (ldl < n) return -6;
+  ZOLO_CHECK(ensure_device());
+  const int64_t h = (n + 1) / 2 + kCholLeaf;
+  const size_t bytes = 8 * (size_t)h * h + 8 * 256;
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  int* dinfo = ar.take<int>(4);
+  Ctx c;
+  c.s = (hipStream_t)stream;
+  c.low = precision == 2 ? Prec::BF16X3 : Prec::F32;
+  ZOLO_HIP(hipMemsetAsync(dinfo, 0, sizeof(int), c.s));
+  int st;
+  if (precision == 0) {
+    double* tmp = ar.take<double>((size_t)h * h);
+    st = chol_inv<double>(c, (double*)A, lda, 0, (double*)Li, ldl, 0, n, tmp, h * h, dinfo, 1);
+  } else {
+    float* tmp = ar.take<float>((size_t)h * h);
+    st = chol_inv<float>(c, (float*)A, lda, 0, (float*)Li, ldl, 0, n, tmp, h * h, dinfo, 1);
+  }
+  if (st) return st;
+  int hinfo = 0;
+  ZOLO_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, c.s));
+  ZOLO_HIP(hipStreamSynchronize(c.s));
+  if (hinfo) {
+    set_error("zolo_chol_inv: not positive definite at column " + std::to_string(hinfo));
+    return kErrNotSpd;
+  }
+  return kOk;
+}
+
+}  // extern "C"

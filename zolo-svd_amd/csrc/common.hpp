@@ -1,0 +1,71 @@
+// Shared definitions for the Zolo-PD HIP library (gfx950 / CDNA4).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace zolo {
+
+// Status codes returned through the C ABI (mirrors include/zolo.h).
+enum Status : int {
+  kOk = 0,
+  kErrArg = -1,
+  kErrHip = 1001,
+  kErrNotSpd = 1002,
+  kErrNoConvergence = 1003,
+  kErrAlloc = 1004,
+  kErrInternal = 1005,
+};
+
+// Thread-local last error message, readable through zolo_last_error().
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define ZOLO_HIP(expr)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::zolo::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));   \
+      return ::zolo::kErrHip;                                                 \
+    }                                                                         \
+  } while (0)
+
+#define ZOLO_CHECK(...)                                                       \
+  do {                                                                        \
+    int s_ = (__VA_ARGS__);                                                   \
+    if (s_ != 0) return s_;                                                   \
+  } while (0)
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// GEMM descriptor.  Column-major operands, C = alpha * op(A) op(B) + beta * C,
+// op(A) is M x K, op(B) is K x N.  Batched over blockIdx.z with element strides.
+// K-range trimming: the caller may declare that op(A) / op(B) are exactly zero
+// outside a band, so that tiles skip the zero part of the K loop:
+//   aLo >= 0 : op(A)[i][k] == 0 for k < i - aLo     (upper-type)
+//   aHi >= 0 : op(A)[i][k] == 0 for k > i + aHi     (lower-type)
+//   bLo >= 0 : op(B)[k][j] == 0 for k < j - bLo     (lower-type in (k,j))
+//   bHi >= 0 : op(B)[k][j] == 0 for k > j + bHi     (upper-type in (k,j))
+// The zeros must be present in memory; trimming is an optimisation only.
+// lowerC: only tiles intersecting the lower triangle (i >= j) are computed.
+// ---------------------------------------------------------------------------
+struct GemmDesc {
+  int transA = 0, transB = 0;
+  int64_t M = 0, N = 0, K = 0;
+  const void* A = nullptr; int64_t lda = 0, strideA = 0;
+  const void* B = nullptr; int64_t ldb = 0, strideB = 0;
+  void* C = nullptr; int64_t ldc = 0, strideC = 0;
+  double alpha = 1.0, beta = 0.0;
+  const double* alphaVec = nullptr;   // optional per-batch alpha (device), multiplies alpha
+  int batch = 1;
+  int aLo = -1, aHi = -1, bLo = -1, bHi = -1;
+  int lowerC = 0;
+};
+
+enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
+
+int gemm(Prec p, const GemmDesc& d, hipStream_t s);
+
+}  // namespace zolo

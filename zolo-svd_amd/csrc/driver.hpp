@@ -1,0 +1,29 @@
+// Internal driver entry points behind the C ABI.
+#pragma once
+#include "../../include/zolo.h"
+#include "linalg.hpp"
+
+namespace zolo {
+
+int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A, int64_t lda,
+                double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s);
+int qdwh_run(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu, double* H,
+             int64_t ldh, zolo_info_t* info, hipStream_t s);
+int form_h(const Ctx& c, const double* U, int64_t ldu, const double* A, int64_t lda, int64_t n, double* H,
+           int64_t ldh);
+
+// Bounds (test_zolodrv1.f90:179-189 / pqdwhfacs.f90:318-344): alpha = ||A||_F,
+// beta from a GPU LU with partial pivoting and the Hager-Higham 1-norm estimator.
+size_t bounds_bytes(int64_t n);
+int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta, Arena& ar,
+               hipStream_t s);
+int normest_run(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt, Arena& ar,
+                hipStream_t s);
+size_t normest_bytes(int64_t m, int64_t n);
+
+// FP64 polish of a low-precision polar factor against an FP64 iterate Xref
+// that has the same polar factor (first-order Lyapunov correction passes).
+size_t polish_bytes(int64_t n);
+int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx, int64_t n, char* ws, int* passes);
+
+}  // namespace zolo

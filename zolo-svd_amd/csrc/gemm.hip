@@ -1,0 +1,435 @@
+// LDS-tiled MFMA GEMM engine for gfx950 (CDNA4), column-major operands.
+//
+// One engine serves every dense contraction of the polar-decomposition path:
+// the Gram matrix X^T X (lower triangle only), the Cholesky trailing updates,
+// the triangular multiplies by explicit inverse factors, the compact-WY
+// updates of the structured Householder QR and the Q1 Q2^T product.
+//
+//  * F64    : v_mfma_f64_16x16x4f64, 128x128x16 tile, 4 waves of 64x64.
+//  * F32    : v_mfma_f32_16x16x4f32, same tiling.
+//  * BF16X3 : fp32 operands split at LDS staging into hi + lo bfloat16 halves,
+//             three v_mfma_f32_16x16x32_bf16 products (hi*hi + hi*lo + lo*hi)
+//             accumulated in fp32; 128x128x32 tile.
+//
+// Work distribution: tiles are numbered so that the 8 XCDs (round-robin
+// workgroup dispatch) each own a contiguous block of tiles, and inside a block
+// tiles walk down groups of 8 tile-rows so neighbouring workgroups share the
+// same A and B panels in their XCD's L2.
+#include "common.hpp"
+
+namespace zolo {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int BM = 128, BN = 128;   // bf16x3 tile
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T> struct Acc;
+template <> struct Acc<double> { typedef d4 type; };
+template <> struct Acc<float> { typedef f4 type; };
+
+__device__ inline d4 mfma16(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline f4 mfma16(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+struct TileCoord { int64_t i0, j0; bool skip; };
+
+// Map the linear workgroup id to a (row, col) tile, XCD-aware and grouped.
+template <int BM, int BN>
+__device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC) {
+  const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+  const int64_t nt = tm * tn;
+  int64_t pid = blockIdx.x;
+  const int64_t full = (nt / 8) * 8;
+  if (pid < full) {  // XCD x (= pid % 8) owns logical tiles [x*nt/8, (x+1)*nt/8)
+    pid = (pid % 8) * (full / 8) + pid / 8;
+  }
+  const int64_t G = 8;  // tile-rows per group
+  const int64_t group = pid / (G * tn);
+  const int64_t first = group * G;
+  const int64_t gsz = (tm - first) < G ? (tm - first) : G;
+  const int64_t r = pid - group * G * tn;
+  TileCoord t;
+  t.i0 = (first + r % gsz) * BM;
+  t.j0 = (r / gsz) * BN;
+  t.skip = lowerC && (t.j0 > t.i0 + BM - 1);
+  return t;
+}
+
+template <int BM, int BN>
+__device__ inline void k_range(const GemmDesc& d, int64_t i0, int64_t j0, int64_t& kb, int64_t& ke) {
+  kb = 0;
+  ke = d.K;
+  if (d.aLo >= 0) kb = max(kb, i0 - d.aLo);
+  if (d.bLo >= 0) kb = max(kb, j0 - d.bLo);
+  if (d.aHi >= 0) ke = min(ke, i0 + BM + d.aHi);
+  if (d.bHi >= 0) ke = min(ke, j0 + BN + d.bHi);
+}
+
+// ---------------------------------------------------------------------------
+// F64 / F32 kernel.  Tile TBM x TBN x 16; the four waves form a WM x WN grid,
+// each wave owning FM x FN blocks of 16x16 MFMA outputs.
+// ---------------------------------------------------------------------------
+template <typename T, int TA, int TB, int TBM, int TBN>
+__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
+  constexpr int BK = 16;
+  constexpr int PAD = (sizeof(T) == 8) ? 4 : 8;
+  constexpr int WM = (TBM == 128 && TBN == 128) ? 2 : (TBM == 64 ? 1 : 4);
+  constexpr int WN = 4 / WM;
+  constexpr int FM = TBM / WM / 16, FN = TBN / WN / 16;
+  constexpr int EA = TBM * BK / kThreads;   // A elements per thread
+  constexpr int EB = TBN * BK / kThreads;   // B elements per thread
+  __shared__ T As[BK][TBM + PAD];
+  __shared__ T Bs[BK][TBN + PAD];
+
+  const TileCoord tc = tile_of<TBM, TBN>(d.M, d.N, d.lowerC);
+  if (tc.skip) return;
+  const int64_t i0 = tc.i0, j0 = tc.j0;
+  const int64_t z = blockIdx.z;
+  const T* __restrict__ A = reinterpret_cast<const T*>(d.A) + z * d.strideA;
+  const T* __restrict__ B = reinterpret_cast<const T*>(d.B) + z * d.strideB;
+  T* __restrict__ C = reinterpret_cast<T*>(d.C) + z * d.strideC;
+
+  int64_t kb, ke;
+  k_range<TBM, TBN>(d, i0, j0, kb, ke);
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+
+  typedef typename Acc<T>::type AccT;
+  AccT acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) acc[a][b] = AccT{0, 0, 0, 0};
+
+  T ra[EA], rb[EB];
+  // A: TA==0 -> EA consecutive i at fixed k; TA==1 -> EA consecutive k at fixed i.
+  const int a_r = (TA == 0) ? t / (TBM / EA) : t / (BK / EA);
+  const int a_c = (TA == 0) ? (t % (TBM / EA)) * EA : (t % (BK / EA)) * EA;
+  // B: TB==0 -> EB consecutive k at fixed j; TB==1 -> EB consecutive j at fixed k.
+  const int b_r = (TB == 0) ? t / (BK / EB) : t / (TBN / EB);
+  const int b_c = (TB == 0) ? (t % (BK / EB)) * EB : (t % (TBN / EB)) * EB;
+
+  auto load_tiles = [&](int64_t k0) {
+    if (TA == 0) {
+      const int64_t k = k0 + a_r;
+      const bool kin = k < ke;
+#pragma unroll
+      for (int e = 0; e < EA; ++e) {
+        const int64_t i = i0 + a_c + e;
+        ra[e] = (kin && i < d.M) ? A[i + k * d.lda] : T(0);
+      }
+    } else {
+      const int64_t i = i0 + a_r;
+#pragma unroll
+      for (int e = 0; e < EA; ++e) {
+        const int64_t k = k0 + a_c + e;
+        ra[e] = (i < d.M && k < ke) ? A[k + i * d.lda] : T(0);
+      }
+    }
+    if (TB == 0) {
+      const int64_t j = j0 + b_r;
+#pragma unroll
+      for (int e = 0; e < EB; ++e) {
+        const int64_t k = k0 + b_c + e;
+        rb[e] = (j < d.N && k < ke) ? B[k + j * d.ldb] : T(0);
+      }
+    } else {
+      const int64_t k = k0 + b_r;
+      const bool kin = k < ke;
+#pragma unroll
+      for (int e = 0; e < EB; ++e) {
+        const int64_t j = j0 + b_c + e;
+        rb[e] = (kin && j < d.N) ? B[j + k * d.ldb] : T(0);
+      }
+    }
+  };
+  auto store_tiles = [&]() {
+    if (TA == 0) {
+#pragma unroll
+      for (int e = 0; e < EA; ++e) As[a_r][a_c + e] = ra[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < EA; ++e) As[a_c + e][a_r] = ra[e];
+    }
+    if (TB == 0) {
+#pragma unroll
+      for (int e = 0; e < EB; ++e) Bs[b_c + e][b_r] = rb[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < EB; ++e) Bs[b_r][b_c + e] = rb[e];
+    }
+  };
+
+  if (kb < ke) {
+    load_tiles(kb);
+    store_tiles();
+    __syncthreads();
+    for (int64_t k0 = kb; k0 < ke; k0 += BK) {
+      const bool more = (k0 + BK) < ke;
+      if (more) load_tiles(k0 + BK);
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        T av[FM], bv[FN];
+#pragma unroll
+        for (int m = 0; m < FM; ++m) av[m] = As[kr][wm * (FM * 16) + m * 16 + (lane & 15)];
+#pragma unroll
+        for (int n = 0; n < FN; ++n) bv[n] = Bs[kr][wn * (FN * 16) + n * 16 + (lane & 15)];
+#pragma unroll
+        for (int m = 0; m < FM; ++m)
+#pragma unroll
+          for (int n = 0; n < FN; ++n) acc[m][n] = mfma16(av[m], bv[n], acc[m][n]);
+      }
+      __syncthreads();
+      if (more) {
+        store_tiles();
+        __syncthreads();
+      }
+    }
+  }
+
+  const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
+  const T al = T(alpha), be = T(d.beta);
+#pragma unroll
+  for (int m = 0; m < FM; ++m) {
+#pragma unroll
+    for (int n = 0; n < FN; ++n) {
+      const int64_t col = j0 + wn * (FN * 16) + n * 16 + (lane & 15);
+      if (col >= d.N) continue;
+      const int64_t rowb = i0 + wm * (FM * 16) + m * 16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // Output lane layout: f32 16x16x4 gives lane rows 4*(lane/16)+v,
+        // f64 16x16x4 interleaves them as 4*v + lane/16 (column = lane%16).
+        const int64_t row = rowb + ((sizeof(T) == 8) ? (4 * v + (lane >> 4)) : ((lane >> 4) * 4 + v));
+        if (row < d.M) {
+          T* p = C + row + col * d.ldc;
+          const T val = al * acc[m][n][v];
+          *p = (d.beta == 0.0) ? val : val + be * (*p);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 kernel: fp32 in / fp32 out, three bf16 MFMA products per k-step.
+// ---------------------------------------------------------------------------
+__device__ inline unsigned short bf16_rn(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+template <int TA, int TB>
+__global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmDesc d) {
+  constexpr int BK = 32;
+  constexpr int LDK = BK + 8;  // row pitch in bf16 elements (80 bytes)
+  __shared__ unsigned short Ah[BM][LDK], Al[BM][LDK], Bh[BN][LDK], Bl[BN][LDK];
+
+  const TileCoord tc = tile_of<BM, BN>(d.M, d.N, d.lowerC);
+  if (tc.skip) return;
+  const int64_t i0 = tc.i0, j0 = tc.j0;
+  const int64_t z = blockIdx.z;
+  const float* __restrict__ A = reinterpret_cast<const float*>(d.A) + z * d.strideA;
+  const float* __restrict__ B = reinterpret_cast<const float*>(d.B) + z * d.strideB;
+  float* __restrict__ C = reinterpret_cast<float*>(d.C) + z * d.strideC;
+
+  int64_t kb, ke;
+  k_range<BM, BN>(d, i0, j0, kb, ke);
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
+
+  float ra[16], rb[16];
+  auto load_tiles = [&](int64_t k0) {
+    if (TA == 0) {  // thread: k = t>>3, 16 consecutive i
+      const int kk = t >> 3, ii = (t & 7) * 16;
+      const int64_t k = k0 + kk;
+      const bool kin = k < ke;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t i = i0 + ii + e;
+        ra[e] = (kin && i < d.M) ? A[i + k * d.lda] : 0.f;
+      }
+    } else {  // thread: i = t>>1, 16 consecutive k
+      const int ii = t >> 1, kk = (t & 1) * 16;
+      const int64_t i = i0 + ii;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t k = k0 + kk + e;
+        ra[e] = (i < d.M && k < ke) ? A[k + i * d.lda] : 0.f;
+      }
+    }
+    if (TB == 0) {  // thread: j = t>>1, 16 consecutive k
+      const int jj = t >> 1, kk = (t & 1) * 16;
+      const int64_t j = j0 + jj;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t k = k0 + kk + e;
+        rb[e] = (j < d.N && k < ke) ? B[k + j * d.ldb] : 0.f;
+      }
+    } else {  // thread: k = t>>3, 16 consecutive j
+      const int kk = t >> 3, jj = (t & 7) * 16;
+      const int64_t k = k0 + kk;
+      const bool kin = k < ke;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t j = j0 + jj + e;
+        rb[e] = (kin && j < d.N) ? B[j + k * d.ldb] : 0.f;
+      }
+    }
+  };
+  auto split_store = [&](unsigned short (*H)[LDK], unsigned short (*L)[LDK], int r, int c, float x) {
+    const unsigned short h = bf16_rn(x);
+    H[r][c] = h;
+    L[r][c] = bf16_rn(x - bf16_to_f(h));
+  };
+  auto store_tiles = [&]() {
+    if (TA == 0) {
+      const int kk = t >> 3, ii = (t & 7) * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) split_store(Ah, Al, ii + e, kk, ra[e]);
+    } else {
+      const int ii = t >> 1, kk = (t & 1) * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) split_store(Ah, Al, ii, kk + e, ra[e]);
+    }
+    if (TB == 0) {
+      const int jj = t >> 1, kk = (t & 1) * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) split_store(Bh, Bl, jj, kk + e, rb[e]);
+    } else {
+      const int kk = t >> 3, jj = (t & 7) * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) split_store(Bh, Bl, jj + e, kk, rb[e]);
+    }
+  };
+
+  if (kb < ke) {
+    load_tiles(kb);
+    store_tiles();
+    __syncthreads();
+    for (int64_t k0 = kb; k0 < ke; k0 += BK) {
+      const bool more = (k0 + BK) < ke;
+      if (more) load_tiles(k0 + BK);
+      const int kc = (lane >> 4) * 8;
+      bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int r = wm * 64 + m * 16 + (lane & 15);
+        ah[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Ah[r][kc]));
+        al[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Al[r][kc]));
+      }
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        const int r = wn * 64 + n * 16 + (lane & 15);
+        bh[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Bh[r][kc]));
+        bl[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Bl[r][kc]));
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[m], bh[n], acc[m][n], 0, 0, 0);
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bl[n], acc[m][n], 0, 0, 0);
+          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bh[n], acc[m][n], 0, 0, 0);
+        }
+      __syncthreads();
+      if (more) {
+        store_tiles();
+        __syncthreads();
+      }
+    }
+  }
+
+  const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
+  const float alf = float(alpha), be = float(d.beta);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int64_t col = j0 + wn * 64 + n * 16 + (lane & 15);
+      if (col >= d.N) continue;
+      const int64_t rowb = i0 + wm * 64 + m * 16 + (lane >> 4) * 4;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = rowb + v;
+        if (row < d.M) {
+          float* p = C + row + col * d.ldc;
+          const float val = alf * acc[m][n][v];
+          *p = (d.beta == 0.0) ? val : val + be * (*p);
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int TBM, int TBN>
+int launch_tile(const GemmDesc& d, hipStream_t s) {
+  const int64_t tiles = cdiv(d.M, TBM) * cdiv(d.N, TBN);
+  dim3 grid((unsigned)tiles, 1, (unsigned)d.batch);
+  if (d.transA == 0 && d.transB == 0) hipLaunchKernelGGL((gemm_kernel<T, 0, 0, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
+  else if (d.transA == 0) hipLaunchKernelGGL((gemm_kernel<T, 0, 1, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
+  else if (d.transB == 0) hipLaunchKernelGGL((gemm_kernel<T, 1, 0, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
+  else hipLaunchKernelGGL((gemm_kernel<T, 1, 1, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+// Skinny operands (the width-64 compact-WY factors of the QR) take a 64-wide
+// tile so no MFMA work is spent on padding.
+template <typename T>
+int launch_typed(const GemmDesc& d, hipStream_t s) {
+  if (d.M <= 64 && d.lowerC == 0) return launch_tile<T, 64, 128>(d, s);
+  if (d.N <= 64 && d.lowerC == 0) return launch_tile<T, 128, 64>(d, s);
+  return launch_tile<T, 128, 128>(d, s);
+}
+
+int launch_bf16x3(const GemmDesc& d, hipStream_t s) {
+  const int64_t tiles = cdiv(d.M, BM) * cdiv(d.N, BN);
+  dim3 grid((unsigned)tiles, 1, (unsigned)d.batch);
+  if (d.transA == 0 && d.transB == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<0, 0>), grid, dim3(kThreads), 0, s, d);
+  else if (d.transA == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<0, 1>), grid, dim3(kThreads), 0, s, d);
+  else if (d.transB == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<1, 0>), grid, dim3(kThreads), 0, s, d);
+  else hipLaunchKernelGGL((gemm_bf16x3_kernel<1, 1>), grid, dim3(kThreads), 0, s, d);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace
+
+int gemm(Prec p, const GemmDesc& d, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return kOk;
+  if (d.K <= 0 && d.beta == 1.0) return kOk;
+  if (cdiv(d.M, BM) * cdiv(d.N, BN) > 0x7fffffffLL || d.batch > 65535) {
+    set_error("gemm: grid too large");
+    return kErrArg;
+  }
+  switch (p) {
+    case Prec::F64: return launch_typed<double>(d, s);
+    case Prec::F32: return launch_typed<float>(d, s);
+    case Prec::BF16X3: return launch_bf16x3(d, s);
+  }
+  return kErrArg;
+}
+
+}  // namespace zolo

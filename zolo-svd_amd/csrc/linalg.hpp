@@ -1,0 +1,79 @@
+// Blocked dense factorizations built on the MFMA GEMM engine.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace zolo {
+
+// Device scratch owned by the library: one grow-only allocation per process,
+// carved by a bump allocator for each top-level call.
+class Arena {
+ public:
+  explicit Arena(char* base = nullptr, size_t cap = 0) : base_(base), cap_(cap), off_(0) {}
+  template <typename T>
+  T* take(size_t count) {
+    off_ = (off_ + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(base_ + off_);
+    off_ += count * sizeof(T);
+    return p;
+  }
+  size_t used() const { return off_; }
+  bool overflow() const { return off_ > cap_; }
+  void reset() { off_ = 0; }
+ private:
+  char* base_;
+  size_t cap_;
+  size_t off_;
+};
+
+#define ZOLO_ARENA_OK(ar)                                             \
+  do {                                                                \
+    if ((ar).overflow()) {                                            \
+      ::zolo::set_error(std::string("workspace overflow in ") + __func__); \
+      return ::zolo::kErrInternal;                                    \
+    }                                                                 \
+  } while (0)
+
+// Grow-only device workspace shared by the entry points (not re-entrant).
+int workspace_acquire(size_t bytes, char** base);
+void workspace_release_all();
+
+struct Ctx {
+  hipStream_t s = nullptr;
+  Prec low = Prec::F32;  // GEMM precision used for float operands (F32 or BF16X3)
+};
+
+template <typename T> inline Prec prec_for(const Ctx& c) { return Prec::F32; }
+template <> inline Prec prec_for<double>(const Ctx&) { return Prec::F64; }
+template <> inline Prec prec_for<float>(const Ctx& c) { return c.low; }
+
+// Batched Cholesky with explicit inverse: A_q (lower) -> L_q, Li_q = L_q^{-1}.
+// tmp must hold batch * ceil(n/2)^2 elements (stride sT).
+template <typename T>
+int chol_inv(const Ctx& c, T* A, int64_t lda, int64_t sA, T* Li, int64_t ldl, int64_t sL, int64_t n, T* tmp,
+             int64_t sT, int* info, int batch);
+
+// Z_q = alpha_q * X (scale X^T X + (shift_q + add) I)^{-1}, q = 0..r-1 (one batch);
+// left = true gives alpha_q * (scale X^T X + (shift_q + add) I)^{-1} X, the
+// SYM='S' branch of zolochol.f90:99-100 / pqdwhfacs.f90:425.
+template <typename T>
+size_t chol_step_bytes(int64_t n, int r);
+template <typename T>
+int chol_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const double* d_shift, double scale, double add,
+              const double* d_alpha, T* Z, int64_t ldz, int64_t sZ, Arena& ar, int* d_info, bool left = false);
+
+// Z_q = alpha_q * Q1_q Q2_q^T where [X; s_q I] = [Q1_q; Q2_q] R_q (structured Householder QR).
+template <typename T>
+size_t qr_step_bytes(int64_t n, int r);
+template <typename T>
+int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const double* d_sdiag, const double* d_alpha,
+            T* Z, int64_t ldz, int64_t sZ, Arena& ar);
+
+// Structured GEQRF + ORGQR of a batch of 2n x n stacks (exposed for tests).
+template <typename T>
+int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
+               T* tau_out, Arena& ar);
+
+}  // namespace zolo

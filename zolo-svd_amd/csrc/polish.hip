@@ -1,0 +1,235 @@
+// FP64 polish of a polar factor computed with low-precision Cholesky steps.
+//
+// The low-precision iterations return U ~ polar(Xref) with an error of the
+// size of their unit roundoff delta, where Xref is the last FP64 iterate (the
+// output of the FP64 QR step; it has A's polar factor to FP64 accuracy and is
+// well conditioned).  One pass computes, in FP64,
+//     F = U^T U - I   and   C = U^T Xref,
+// splits C = Hs + K (symmetric + skew), and solves in low precision the
+// Lyapunov equation  S Hs + Hs S = 2K - (F Hs - Hs F)/2  for a skew S by
+// conjugate gradients (the operator has eigenvalues h_i + h_j, so it is as
+// well conditioned as Xref).  U <- U (I + S - F/2) then makes U^T U = I and
+// U^T Xref symmetric to first order, so each pass squares the error.  A pass
+// costs one FP64 SYRK-shaped GEMM (n^3) and one FP64 GEMM (2 n^3); the CG and
+// the update GEMM run in the low precision.
+#include <algorithm>
+#include <cmath>
+
+#include "driver.hpp"
+
+namespace zolo {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kNp = 1024;
+
+inline unsigned grid_for(int64_t total) {
+  int64_t g = (total + kT - 1) / kT;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+}
+
+// F = lower(G) mirrored, minus I, written as float (G holds U^T U in its lower triangle).
+__global__ __launch_bounds__(kT) void gram_to_f_kernel(const double* G, int64_t n, float* F) {
+  const int64_t total = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % n, j = e / n;
+    double v = (i >= j) ? G[i + j * n] : G[j + i * n];
+    if (i == j) v -= 1.0;
+    F[e] = (float)v;
+  }
+}
+
+// Hs = (C + C^T)/2 (float), R = (C - C^T) (= 2K, float).
+__global__ __launch_bounds__(kT) void split_kernel(const double* C, int64_t n, float* Hs, float* R) {
+  const int64_t total = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % n, j = e / n;
+    const double a = C[i + j * n], b = C[j + i * n];
+    Hs[e] = (float)(0.5 * (a + b));
+    R[e] = (float)(a - b);
+  }
+}
+
+// D += coef * (M - M^T)
+__global__ __launch_bounds__(kT) void skew_acc_kernel(float* D, const float* M, int64_t n, float coef) {
+  const int64_t total = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % n, j = e / n;
+    D[e] += coef * (M[i + j * n] - M[j + i * n]);
+  }
+}
+
+// D = M - M^T
+__global__ __launch_bounds__(kT) void skew_of_kernel(float* D, const float* M, int64_t n) {
+  const int64_t total = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % n, j = e / n;
+    D[e] = M[i + j * n] - M[j + i * n];
+  }
+}
+
+// partial[blk] = sum a*b (double accumulation)
+__global__ __launch_bounds__(kT) void dot_kernel(const float* a, const float* b, int64_t len, double* partial) {
+  __shared__ double sh[kT];
+  double s = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += (int64_t)gridDim.x * kT)
+    s += (double)a[e] * (double)b[e];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = kT / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kT) void maxabs_kernel(const float* a, int64_t len, double* partial) {
+  __shared__ double sh[kT];
+  double s = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += (int64_t)gridDim.x * kT)
+    s = fmax(s, fabs((double)a[e]));
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = kT / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kT) void finish_reduce_kernel(const double* p, int np, int is_max, double* out) {
+  __shared__ double sh[kT];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += kT) s = is_max ? fmax(s, p[i]) : s + p[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = kT / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] = is_max ? fmax(sh[threadIdx.x], sh[threadIdx.x + k]) : sh[threadIdx.x] + sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// x += a*y ; then optionally (cg direction) y2 = r + b*y2 handled by separate calls
+__global__ __launch_bounds__(kT) void axpy_f_kernel(float* x, const float* y, int64_t len, float a) {
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += (int64_t)gridDim.x * kT) x[e] += a * y[e];
+}
+// p = r + b*p
+__global__ __launch_bounds__(kT) void xpby_f_kernel(float* p, const float* r, int64_t len, float b) {
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += (int64_t)gridDim.x * kT) p[e] = r[e] + b * p[e];
+}
+// E = S - F/2
+__global__ __launch_bounds__(kT) void form_e_kernel(float* E, const float* S, const float* F, int64_t len) {
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += (int64_t)gridDim.x * kT) E[e] = S[e] - 0.5f * F[e];
+}
+// U (double) += D (float)
+__global__ __launch_bounds__(kT) void add_f_to_d_kernel(double* U, int64_t ldu, const float* D, int64_t n) {
+  const int64_t total = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % n, j = e / n;
+    U[i + j * ldu] += (double)D[e];
+  }
+}
+
+}  // namespace
+
+size_t polish_bytes(int64_t n) {
+  const size_t nn = (size_t)n * n;
+  return 2 * nn * sizeof(double) + 8 * nn * sizeof(float) + (kNp + 16) * sizeof(double) + 32 * 256;
+}
+
+int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx, int64_t n, char* ws, int* passes) {
+  const int64_t nn = n * n;
+  const hipStream_t s = c.s;
+  Arena ar(ws, polish_bytes(n));
+  double* G = ar.take<double>(nn);
+  double* C = ar.take<double>(nn);
+  float* F = ar.take<float>(nn);
+  float* Hs = ar.take<float>(nn);
+  float* R = ar.take<float>(nn);
+  float* S = ar.take<float>(nn);
+  float* P = ar.take<float>(nn);
+  float* AP = ar.take<float>(nn);
+  float* M = ar.take<float>(nn);
+  float* U32 = ar.take<float>(nn);
+  double* part = ar.take<double>(kNp);
+  double* out = ar.take<double>(16);
+  ZOLO_ARENA_OK(ar);
+  const unsigned g = grid_for(nn);
+  auto reduce = [&](bool is_max, double* h) -> int {
+    hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(kT), 0, s, part, kNp, is_max ? 1 : 0, out);
+    ZOLO_HIP(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, s));
+    ZOLO_HIP(hipStreamSynchronize(s));
+    return kOk;
+  };
+  auto dot = [&](const float* a, const float* b, double* h) -> int {
+    hipLaunchKernelGGL(dot_kernel, dim3(kNp), dim3(kT), 0, s, a, b, nn, part);
+    return reduce(false, h);
+  };
+  // M = A_ * B_ (n x n) in FP32: the CG only needs S to a relative accuracy
+  // well below the size of the correction it computes.
+  auto lgemm = [&](const float* A_, const float* B_, float* C_) -> int {
+    GemmDesc d;
+    d.M = n; d.N = n; d.K = n;
+    d.A = A_; d.lda = n; d.B = B_; d.ldb = n; d.C = C_; d.ldc = n;
+    return gemm(Prec::F32, d, s);
+  };
+  const int max_passes = 3;
+  int pass = 0;
+  for (; pass < max_passes; ++pass) {
+    // G = U^T U (lower), C = U^T Xref  -- FP64
+    GemmDesc d;
+    d.transA = 1; d.M = n; d.N = n; d.K = n;
+    d.A = U; d.lda = ldu; d.B = U; d.ldb = ldu; d.C = G; d.ldc = n; d.lowerC = 1;
+    ZOLO_CHECK(gemm(Prec::F64, d, s));
+    d = GemmDesc();
+    d.transA = 1; d.M = n; d.N = n; d.K = n;
+    d.A = U; d.lda = ldu; d.B = Xref; d.ldb = ldx; d.C = C; d.ldc = n;
+    ZOLO_CHECK(gemm(Prec::F64, d, s));
+    hipLaunchKernelGGL(gram_to_f_kernel, dim3(g), dim3(kT), 0, s, G, n, F);
+    hipLaunchKernelGGL(split_kernel, dim3(g), dim3(kT), 0, s, C, n, Hs, R);
+    // R = 2K - (F Hs - Hs F)/2 = R - 0.5 (M - M^T), M = F Hs
+    ZOLO_CHECK(lgemm(F, Hs, M));
+    hipLaunchKernelGGL(skew_acc_kernel, dim3(g), dim3(kT), 0, s, R, M, n, -0.5f);
+    ZOLO_HIP(hipGetLastError());
+    // CG on L(S) = S Hs + Hs S = (S Hs) - (S Hs)^T for skew S
+    ZOLO_HIP(hipMemsetAsync(S, 0, sizeof(float) * nn, s));
+    ZOLO_HIP(hipMemcpyAsync(P, R, sizeof(float) * nn, hipMemcpyDeviceToDevice, s));
+    double rr0, rr;
+    ZOLO_CHECK(dot(R, R, &rr0));
+    rr = rr0;
+    const double tol2 = 1e-14 * rr0;  // relative residual 1e-7
+    for (int it = 0; it < 40 && rr > tol2 && rr > 0.0; ++it) {
+      ZOLO_CHECK(lgemm(P, Hs, M));
+      hipLaunchKernelGGL(skew_of_kernel, dim3(g), dim3(kT), 0, s, AP, M, n);
+      double pap;
+      ZOLO_CHECK(dot(P, AP, &pap));
+      if (!(pap > 0.0)) break;
+      const float a = (float)(rr / pap);
+      hipLaunchKernelGGL(axpy_f_kernel, dim3(g), dim3(kT), 0, s, S, P, nn, a);
+      hipLaunchKernelGGL(axpy_f_kernel, dim3(g), dim3(kT), 0, s, R, AP, nn, -a);
+      double rn;
+      ZOLO_CHECK(dot(R, R, &rn));
+      hipLaunchKernelGGL(xpby_f_kernel, dim3(g), dim3(kT), 0, s, P, R, nn, (float)(rn / rr));
+      rr = rn;
+    }
+    // E = S - F/2 (into AP) ; U += U*E
+    hipLaunchKernelGGL(form_e_kernel, dim3(g), dim3(kT), 0, s, AP, S, F, nn);
+    ZOLO_CHECK(k_scale_copy<double, float>(U, ldu, U32, n, n, n, 1.0, s));
+    ZOLO_CHECK(lgemm(U32, AP, M));
+    hipLaunchKernelGGL(add_f_to_d_kernel, dim3(g), dim3(kT), 0, s, U, ldu, M, n);
+    hipLaunchKernelGGL(maxabs_kernel, dim3(kNp), dim3(kT), 0, s, AP, nn, part);
+    ZOLO_HIP(hipGetLastError());
+    double emax;
+    ZOLO_CHECK(reduce(true, &emax));
+    // The error left after this pass is O(emax^2) (plus the FP64 rounding of the pass).
+    if (emax < 3e-8) {
+      ++pass;
+      break;
+    }
+  }
+  *passes = pass;
+  return kOk;
+}
+
+}  // namespace zolo

@@ -1,0 +1,240 @@
+// Structured Householder QR of the stacked matrix [X; s I] (2n x n), batched
+// over the Zolotarev shifts (blockIdx.y), following mpdgeqrf.f / mpdorgqr.f:
+// the reflector of column j only has support on rows j .. n+j, because the
+// identity block below X fills in one row per eliminated column.
+//
+// Panel factorisation: one launch per column.  Launch j applies the reflector
+// of column j-1 to the rest of the panel and, in the same pass over the rows
+// each thread owns, reduces the Gram row g_c = sum_i S[i][j] S[i][c] that the
+// reflector of column j needs (its norm is g_j).  Partial sums go to a
+// per-workgroup slot and are summed in a fixed order by the next launch, so
+// the result does not depend on scheduling.  The trailing update and the
+// explicit Q use the compact WY form (I - V T V^T) through the MFMA GEMM.
+#include "kernels.hpp"
+
+namespace zolo {
+namespace {
+
+constexpr int kT = 256;
+constexpr int NBP = kQrNb;
+
+__device__ inline double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// part layout per shift: for local column jj, a record of (nwg*jb + jb) doubles:
+// [0, nwg*jb): per-workgroup Gram partials, [nwg*jb, nwg*jb+jb): pivot row.
+template <typename T>
+__global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb,
+                                                      int64_t j, int apply_prev, int reduce_cur, double* part,
+                                                      int nwg, T* tau, int64_t strideTau) {
+  __shared__ double gs[NBP], rr[NBP], wv[NBP];
+  __shared__ double sc[3];  // beta, tau, scal
+  __shared__ double red[4][NBP];
+  const int q = blockIdx.y;
+  T* Sq = S + q * strideS;
+  const int64_t rec = (int64_t)nwg * jb + jb;
+  double* base = part + (int64_t)q * jb * rec;
+  const int t = threadIdx.x, w = blockIdx.x;
+  const int64_t NR = n + jb;
+  const int64_t rpw = (NR + nwg - 1) / nwg;
+  const int64_t r0 = p + w * rpw;
+  const int64_t r1 = min(p + NR, r0 + rpw);
+
+  if (apply_prev) {
+    const int64_t jp = j - 1;
+    const int jjp = (int)(jp - p);
+    const double* G = base + (int64_t)jjp * rec;
+    if (t < jb) {
+      double s = 0.0;
+      if (t >= jjp)
+        for (int k = 0; k < nwg; ++k) s += G[(int64_t)k * jb + t];
+      gs[t] = s;
+      rr[t] = G[(int64_t)nwg * jb + t];
+    }
+    __syncthreads();
+    if (t == 0) {
+      const double alpha = rr[jjp], xn2 = gs[jjp];
+      double beta = alpha, tv = 0.0, scal = 0.0;
+      if (xn2 > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tv = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      sc[0] = beta; sc[1] = tv; sc[2] = scal;
+      if (w == 0) tau[q * strideTau + jp] = (T)tv;
+    }
+    __syncthreads();
+    if (t < jb) wv[t] = (t > jjp) ? sc[1] * (rr[t] + sc[2] * gs[t]) : 0.0;
+    __syncthreads();
+    const double scal = sc[2];
+    for (int64_t i = r0 + t; i < r1; i += kT) {
+      if (i == jp) {
+        Sq[jp + jp * lds] = (T)sc[0];
+        for (int c = jjp + 1; c < jb; ++c) {
+          T* e = Sq + jp + (p + c) * lds;
+          *e = (T)((double)*e - wv[c]);
+        }
+      } else if (i > jp && i <= n + jp) {
+        T* ej = Sq + i + jp * lds;
+        const double vi = (double)*ej * scal;
+        *ej = (T)vi;
+        for (int c = jjp + 1; c < jb; ++c) {
+          T* e = Sq + i + (p + c) * lds;
+          *e = (T)((double)*e - vi * wv[c]);
+        }
+      }
+    }
+  }
+
+  if (reduce_cur) {
+    const int jj = (int)(j - p);
+    double* G = base + (int64_t)jj * rec;
+    double acc[NBP];
+#pragma unroll
+    for (int c = 0; c < NBP; ++c) acc[c] = 0.0;
+    for (int64_t i = r0 + t; i < r1; i += kT) {
+      if (i == j) {
+        for (int c = jj; c < jb; ++c) G[(int64_t)nwg * jb + c] = (double)Sq[j + (p + c) * lds];
+      } else if (i > j && i <= n + j) {
+        const double x = (double)Sq[i + j * lds];
+#pragma unroll
+        for (int c = 0; c < NBP; ++c)
+          if (c >= jj && c < jb) acc[c] += x * (double)Sq[i + (p + c) * lds];
+      }
+    }
+    const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int c = 0; c < NBP; ++c) {
+      const double v = wsum(acc[c]);
+      if (lane == 0) red[wave][c] = v;
+    }
+    __syncthreads();
+    if (t < jb) G[(int64_t)w * jb + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+  }
+}
+
+// V (mrows x jb) with unit diagonal and explicit zeros, from the panel at (p, p).
+template <typename T>
+__global__ __launch_bounds__(kT) void qr_extract_v_kernel(const T* S, int64_t lds, int64_t strideS, T* V, int64_t ldv,
+                                                          int64_t strideV, int64_t p, int jb, int64_t mrows, int64_t n) {
+  const int q = blockIdx.y;
+  const T* Sq = S + q * strideS;
+  T* Vq = V + q * strideV;
+  const int64_t total = mrows * jb;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % mrows;
+    const int c = (int)(e / mrows);
+    T v = (T)0;
+    if (i == c) v = (T)1;
+    else if (i > c && i <= n + c) v = Sq[(p + i) + (p + c) * lds];
+    Vq[i + (int64_t)c * ldv] = v;
+  }
+}
+
+// T (jb x jb upper) of the block reflector from G = V^T V and tau (dlarft, forward columnwise).
+template <typename T>
+__global__ __launch_bounds__(kT) void qr_form_t_kernel(const T* G, int64_t ldg, int64_t strideG, const T* tau,
+                                                       int64_t strideTau, int64_t p, int jb, T* Tm, int64_t ldt,
+                                                       int64_t strideT) {
+  __shared__ double g[NBP][NBP + 1];
+  __shared__ double tt[NBP][NBP + 1];
+  const int q = blockIdx.x;
+  const T* Gq = G + q * strideG;
+  const T* tq = tau + q * strideTau;
+  T* Tq = Tm + q * strideT;
+  const int t = threadIdx.x;
+  for (int e = t; e < jb * jb; e += kT) {
+    const int r = e % jb, c = e / jb;
+    g[r][c] = (double)Gq[r + c * ldg];
+    tt[r][c] = 0.0;
+  }
+  __syncthreads();
+  for (int i = 0; i < jb; ++i) {
+    const double ti = (double)tq[p + i];
+    double s = 0.0;
+    if (t < i) {
+      for (int k = t; k < i; ++k) s += tt[t][k] * g[k][i];
+    }
+    __syncthreads();
+    if (t < i) tt[t][i] = -ti * s;
+    if (t == 0) tt[i][i] = ti;
+    __syncthreads();
+  }
+  for (int e = t; e < jb * jb; e += kT) {
+    const int r = e % jb, c = e / jb;
+    Tq[r + c * ldt] = (T)tt[r][c];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kT) void identity_top_kernel(T* Q, int64_t ldq, int64_t strideQ, int64_t m, int64_t n) {
+  T* Qq = Q + blockIdx.y * strideQ;
+  const int64_t total = m * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % m, j = e / m;
+    Qq[i + j * ldq] = (i == j) ? (T)1 : (T)0;
+  }
+}
+
+inline unsigned grid_for(int64_t total) {
+  int64_t g = (total + kT - 1) / kT;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+template <typename T>
+int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t j, int apply_prev,
+                    int reduce_cur, double* part, int nwg, T* tau, int64_t strideTau, int batch, hipStream_t s) {
+  if (jb > NBP || jb <= 0) {
+    set_error("qr panel: width out of range");
+    return kErrArg;
+  }
+  hipLaunchKernelGGL((qr_panel_kernel<T>), dim3(nwg, batch), dim3(kT), 0, s, S, lds, strideS, n, p, jb, j, apply_prev,
+                     reduce_cur, part, nwg, tau, strideTau);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+template <typename T>
+int k_qr_extract_v(const T* S, int64_t lds, int64_t strideS, T* V, int64_t ldv, int64_t strideV, int64_t p, int jb,
+                   int64_t mrows, int batch, hipStream_t s) {
+  const int64_t n = mrows - jb;
+  hipLaunchKernelGGL((qr_extract_v_kernel<T>), dim3(grid_for(mrows * jb), batch), dim3(kT), 0, s, S, lds, strideS, V,
+                     ldv, strideV, p, jb, mrows, n);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+template <typename T>
+int k_qr_form_t(const T* G, int64_t ldg, int64_t strideG, const T* tau, int64_t strideTau, int64_t p, int jb, T* Tm,
+                int64_t ldt, int64_t strideT, int batch, hipStream_t s) {
+  hipLaunchKernelGGL((qr_form_t_kernel<T>), dim3(batch), dim3(kT), 0, s, G, ldg, strideG, tau, strideTau, p, jb, Tm,
+                     ldt, strideT);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+template <typename T>
+int k_set_identity_top(T* Q, int64_t ldq, int64_t strideQ, int64_t m, int64_t n, int batch, hipStream_t s) {
+  hipLaunchKernelGGL((identity_top_kernel<T>), dim3(grid_for(m * n), batch), dim3(kT), 0, s, Q, ldq, strideQ, m, n);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+#define ZOLO_QR_INST(T)                                                                                            \
+  template int k_qr_panel_step<T>(T*, int64_t, int64_t, int64_t, int64_t, int, int64_t, int, int, double*, int, T*, \
+                                  int64_t, int, hipStream_t);                                                      \
+  template int k_qr_extract_v<T>(const T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, int64_t, int,      \
+                                 hipStream_t);                                                                     \
+  template int k_qr_form_t<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t, int, T*, int64_t, int64_t,    \
+                              int, hipStream_t);                                                                   \
+  template int k_set_identity_top<T>(T*, int64_t, int64_t, int64_t, int64_t, int, hipStream_t);
+ZOLO_QR_INST(double)
+ZOLO_QR_INST(float)
+
+}  // namespace zolo

@@ -1,0 +1,327 @@
+"""Python front end of the MI355X Zolo-PD library (ctypes over libzolo.so).
+
+Mirrors the reference's Fortran entry points on a 1x1 process grid:
+
+* ``pzolopd1``   -- PZOLOPD1 (pzolopd1.f90:2): Zolo-PD with caller-supplied alpha, beta.
+* ``pqdwhfacs``  -- PQDWHFACS (pqdwhfacs.f90:2): QDWH-PD with its own bounds.
+* ``zolochol``   -- ZOLOCHOL (zolochol.f90:2): one shifted Cholesky contribution.
+* ``zoloqr``     -- ZOLOQR (zoloqr.f90:2): one structured-QR contribution.
+* ``pdnormest``  -- PDNORMEST (pdnormest.f90:2): 2-norm power-iteration estimate.
+* ``bounds``     -- the alpha/beta estimate of test_zolodrv1.f90:179-189.
+* ``compute_coeff`` / ``choosem`` / ``hfunc`` / ``mellipke`` / ``mellipj`` -- Auxil_Zolo.f90, hfunc.f90.
+
+Host arrays are numpy (column-major copies are made as needed); the
+``*_device`` variants take torch CUDA tensors (column-major data in HBM, see
+``to_device``) and run on torch's current stream.  Every call goes to the HIP
+library; there is no CPU fallback: if libzolo.so is missing, importing this
+module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(os.path.dirname(_HERE), "libzolo.so")
+
+if not os.path.exists(_LIB_PATH):
+    raise ImportError(f"libzolo.so not found at {_LIB_PATH}; build it with `make -C zolo-svd_amd`")
+_lib = C.CDLL(_LIB_PATH)
+
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+ERR_NOT_SPD = 1002
+ERR_NO_CONVERGENCE = 1003
+
+
+class ZoloError(RuntimeError):
+    def __init__(self, func: str, status: int):
+        msg = _lib.zolo_last_error().decode(errors="replace")
+        super().__init__(f"{func} failed with status {status}: {msg}")
+        self.status = status
+
+
+class _Info(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int), ("r", C.c_int), ("info", C.c_int), ("polish_passes", C.c_int),
+        ("schedule", C.c_char * 16),
+        ("err", C.c_double * 16), ("con", C.c_double * 16), ("maxc", C.c_double * 16),
+        ("alpha", C.c_double), ("beta", C.c_double), ("time_ms", C.c_double),
+        ("phase_ms", C.c_double * 8),
+    ]
+
+
+PHASES = ("bounds", "qr", "chol", "formx2", "polish", "h", "other")
+
+
+@dataclass
+class RunInfo:
+    """Iteration record of one call (zolo_info_t)."""
+    iters: int
+    r: int
+    schedule: str
+    err: list
+    con: list
+    maxc: list
+    alpha: float
+    beta: float
+    time_ms: float
+    phase_ms: dict = field(default_factory=dict)
+    polish_passes: int = 0
+    info: int = 0
+
+    @staticmethod
+    def from_c(ci: _Info) -> "RunInfo":
+        k = ci.iters
+        return RunInfo(iters=k, r=ci.r, schedule=ci.schedule.decode()[:k], err=list(ci.err[:k]),
+                       con=list(ci.con[:k]), maxc=list(ci.maxc[:k]), alpha=ci.alpha, beta=ci.beta,
+                       time_ms=ci.time_ms, phase_ms={p: ci.phase_ms[i] for i, p in enumerate(PHASES)},
+                       polish_passes=ci.polish_passes, info=ci.info)
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I64 = C.c_int64
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("zolo_last_error", C.c_char_p, [])
+_sig("zolo_init", C.c_int, [C.c_int])
+_sig("zolo_finalize", C.c_int, [])
+_sig("zolo_version", C.c_int, [])
+_sig("zolo_compute_coeff", C.c_int, [C.c_double, C.c_double, C.c_int, _D])
+_sig("zolo_choosem", C.c_int, [C.c_double])
+_sig("zolo_hfunc", C.c_double, [C.c_double])
+_sig("zolo_mellipke", None, [C.c_double, C.c_double, _D, _D])
+_sig("zolo_mellipj", None, [C.c_double, C.c_double, C.c_double, _D, _D, _D])
+_sig("zolo_pd_device", C.c_int, [C.c_int, C.c_char, C.c_int, _I64, C.c_double, C.c_double, _P, _I64, _P, _I64, _P,
+                                 _I64, C.POINTER(_Info), _P])
+_sig("zolo_pd_host", C.c_int, [C.c_int, C.c_char, C.c_int, _I64, C.c_double, C.c_double, _D, _I64, _D, _I64, _D,
+                               _I64, C.POINTER(_Info)])
+_sig("zolo_bounds_device", C.c_int, [C.c_int, _I64, _P, _I64, _D, _D, _P])
+_sig("zolo_qdwh_device", C.c_int, [C.c_char, _I64, C.c_double, _P, _I64, _P, _I64, _P, _I64, C.POINTER(_Info), _P])
+_sig("zolo_qdwh_host", C.c_int, [C.c_char, _I64, C.c_double, _D, _I64, _D, _I64, _D, _I64, C.POINTER(_Info)])
+_sig("zolo_chol_step_host", C.c_int, [C.c_char, _I64, _D, _I64, _D, _I64, _D, C.c_int, C.c_int])
+_sig("zolo_qr_step_host", C.c_int, [_I64, _D, _I64, _D, _I64, _D, C.c_int, C.c_int])
+_sig("zolo_normest_device", C.c_int, [_I64, _I64, _P, _I64, C.c_double, _D, C.POINTER(C.c_int), _P])
+_sig("zolo_gemm_device", C.c_int, [C.c_int, C.c_int, C.c_int, _I64, _I64, _I64, C.c_double, _P, _I64, _P, _I64,
+                                   C.c_double, _P, _I64, _P])
+_sig("zolo_stacked_qr_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P])
+_sig("zolo_chol_inv_device", C.c_int, [C.c_int, _I64, _P, _I64, _P, _I64, _P])
+
+LIB = _lib
+LIB_PATH = _LIB_PATH
+
+
+def _check(func: str, st: int):
+    if st != 0:
+        raise ZoloError(func, st)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def _fortran(a) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+# ---------------------------------------------------------------------------
+# Scalar helpers (host code inside libzolo.so)
+# ---------------------------------------------------------------------------
+
+def compute_coeff(con: float, tol: float, rk: int) -> np.ndarray:
+    """Compute_Coeff (Auxil_Zolo.f90:165): c_1..c_2r followed by a_1..a_r."""
+    out = np.zeros(3 * rk)
+    _check("zolo_compute_coeff", _lib.zolo_compute_coeff(con, tol, rk, _dptr(out)))
+    return out
+
+
+def choosem(con: float) -> int:
+    """choosem (Auxil_Zolo.f90:130)."""
+    return int(_lib.zolo_choosem(con))
+
+
+def hfunc(x: float) -> float:
+    """hfunc (hfunc.f90:2)."""
+    return float(_lib.zolo_hfunc(x))
+
+
+def mellipke(alpha: float, tol: float):
+    k, e = C.c_double(), C.c_double()
+    _lib.zolo_mellipke(alpha, tol, C.byref(k), C.byref(e))
+    return k.value, e.value
+
+
+def mellipj(u: float, alpha: float, tol: float):
+    sn, cn, dn = C.c_double(), C.c_double(), C.c_double()
+    _lib.zolo_mellipj(u, alpha, tol, C.byref(sn), C.byref(cn), C.byref(dn))
+    return sn.value, cn.value, dn.value
+
+
+# ---------------------------------------------------------------------------
+# Host-array entry points (mirror the Fortran argument meaning)
+# ---------------------------------------------------------------------------
+
+def pzolopd1(A, alpha: float, beta: float, rk: int, sym: str = "N", polar: bool = True,
+             precision: str = "f64"):
+    """PZOLOPD1 on the GPU.  Returns (U, H or None, RunInfo).
+
+    X0 = A/(alpha*beta), CON = 1/beta (pzolopd1.f90:286-291); beta must be a
+    lower bound of sigma_min(A/alpha).  precision 'f64' is the reference's
+    arithmetic; 'f32' / 'bf16x3' run the Cholesky iterations in low precision
+    and finish with an FP64 polish.
+    """
+    A = _fortran(A)
+    n = A.shape[0]
+    if A.shape != (n, n):
+        raise ValueError("pzolopd1: A must be square (pzolopd1.f90:192)")
+    U = np.zeros((n, n), order="F")
+    H = np.zeros((n, n), order="F") if polar else None
+    ci = _Info()
+    st = _lib.zolo_pd_host(PREC[precision], sym.encode()[:1], rk, n, alpha, beta, _dptr(A), n, _dptr(U), n,
+                           _dptr(H) if polar else None, n, C.byref(ci))
+    _check("zolo_pd_host", st)
+    return U, H, RunInfo.from_c(ci)
+
+
+def pqdwhfacs(A, sigma: float = 0.0, polar: bool = True, sym: str = "N"):
+    """PQDWHFACS on the GPU.  Returns (U, H or None, RunInfo)."""
+    A = _fortran(A)
+    n = A.shape[0]
+    U = np.zeros((n, n), order="F")
+    H = np.zeros((n, n), order="F") if polar else None
+    ci = _Info()
+    st = _lib.zolo_qdwh_host(sym.encode()[:1], n, sigma, _dptr(A), n, _dptr(U), n, _dptr(H) if polar else None, n,
+                             C.byref(ci))
+    _check("zolo_qdwh_host", st)
+    return U, H, RunInfo.from_c(ci)
+
+
+def zolochol(X, coeff, rk: int, ip: int, sym: str = "N") -> np.ndarray:
+    """ZOLOCHOL: Z = -a_ip X (X^T X + c_{2ip-1} I)^{-1} (ip 1-based)."""
+    X = _fortran(X)
+    n = X.shape[0]
+    Z = np.zeros((n, n), order="F")
+    cf = np.ascontiguousarray(coeff, dtype=np.float64)
+    _check("zolo_chol_step_host",
+           _lib.zolo_chol_step_host(sym.encode()[:1], n, _dptr(X), n, _dptr(Z), n, _dptr(cf), rk, ip))
+    return Z
+
+
+def zoloqr(X, coeff, rk: int, ip: int) -> np.ndarray:
+    """ZOLOQR: Z = -a_ip/sqrt(c) Q1 Q2^T with [X; sqrt(c) I] = QR (ip 1-based)."""
+    X = _fortran(X)
+    n = X.shape[0]
+    Z = np.zeros((n, n), order="F")
+    cf = np.ascontiguousarray(coeff, dtype=np.float64)
+    _check("zolo_qr_step_host", _lib.zolo_qr_step_host(n, _dptr(X), n, _dptr(Z), n, _dptr(cf), rk, ip))
+    return Z
+
+
+# ---------------------------------------------------------------------------
+# Device entry points (torch CUDA tensors holding column-major matrices)
+# ---------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def to_device(A, dtype=None):
+    """Column-major device copy: returns a torch tensor T with T.T laid out as A (ld = n)."""
+    torch = _torch()
+    a = np.asarray(A)
+    dt = dtype or (torch.float64 if a.dtype == np.float64 else torch.float32)
+    # A column-major n x m matrix is a row-major m x n array of its transpose.
+    return torch.from_numpy(np.ascontiguousarray(a.T)).to(device="cuda", dtype=dt)
+
+
+def from_device(t) -> np.ndarray:
+    return t.detach().cpu().numpy().T.copy(order="F")
+
+
+def _stream():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def pzolopd1_device(dA, alpha: float, beta: float, rk: int, sym: str = "N", polar: bool = True,
+                    precision: str = "f64", dU=None, dH=None):
+    """PZOLOPD1 on device buffers (column-major, ld = n).  Returns (dU, dH, RunInfo)."""
+    torch = _torch()
+    n = dA.shape[0]
+    if dU is None:
+        dU = torch.empty_like(dA)
+    if polar and dH is None:
+        dH = torch.empty_like(dA)
+    ci = _Info()
+    st = _lib.zolo_pd_device(PREC[precision], sym.encode()[:1], rk, n, alpha, beta, C.c_void_p(dA.data_ptr()), n,
+                             C.c_void_p(dU.data_ptr()), n, C.c_void_p(dH.data_ptr()) if polar else None, n,
+                             C.byref(ci), _stream())
+    _check("zolo_pd_device", st)
+    return dU, (dH if polar else None), RunInfo.from_c(ci)
+
+
+def pqdwhfacs_device(dA, sigma: float = 0.0, polar: bool = True, sym: str = "N"):
+    torch = _torch()
+    n = dA.shape[0]
+    dU = torch.empty_like(dA)
+    dH = torch.empty_like(dA) if polar else None
+    ci = _Info()
+    st = _lib.zolo_qdwh_device(sym.encode()[:1], n, sigma, C.c_void_p(dA.data_ptr()), n, C.c_void_p(dU.data_ptr()), n,
+                               C.c_void_p(dH.data_ptr()) if polar else None, n, C.byref(ci), _stream())
+    _check("zolo_qdwh_device", st)
+    return dU, dH, RunInfo.from_c(ci)
+
+
+def bounds_device(dA, convention: str = "zolo"):
+    """alpha = ||A||_F and the lower bound beta (test_zolodrv1.f90:179-189 / pqdwhfacs.f90:318-344)."""
+    n = dA.shape[0]
+    a, b = C.c_double(), C.c_double()
+    conv = {"zolo": 0, "qdwh": 1}[convention]
+    _check("zolo_bounds_device", _lib.zolo_bounds_device(conv, n, C.c_void_p(dA.data_ptr()), n, C.byref(a), C.byref(b),
+                                                         _stream()))
+    return a.value, b.value
+
+
+def pdnormest_device(dA, tol: float = 1e-6):
+    """PDNORMEST: power-iteration estimate of ||A||_2; returns (estimate, iterations)."""
+    n, m = dA.shape  # column-major m x n stored as row-major n x m
+    est, cnt = C.c_double(), C.c_int()
+    _check("zolo_normest_device", _lib.zolo_normest_device(m, n, C.c_void_p(dA.data_ptr()), m, tol, C.byref(est),
+                                                           C.byref(cnt), _stream()))
+    return est.value, cnt.value
+
+
+def gemm_device(transa: int, transb: int, m: int, n: int, k: int, alpha: float, dA, lda: int, dB, ldb: int,
+                beta: float, dC, ldc: int, precision: str = "f64"):
+    """C = alpha op(A) op(B) + beta C on device buffers (column-major)."""
+    _check("zolo_gemm_device", _lib.zolo_gemm_device(PREC[precision], transa, transb, m, n, k, alpha,
+                                                     C.c_void_p(dA.data_ptr()), lda, C.c_void_p(dB.data_ptr()), ldb,
+                                                     beta, C.c_void_p(dC.data_ptr()), ldc, _stream()))
+
+
+def stacked_qr_device(dS, n: int, dQ=None, dtau=None):
+    """MPDGEQRF + MPDORGQR of the 2n x n stack in dS (column-major, ld 2n)."""
+    _check("zolo_stacked_qr_device", _lib.zolo_stacked_qr_device(
+        n, C.c_void_p(dS.data_ptr()), 2 * n, C.c_void_p(dQ.data_ptr()) if dQ is not None else None, 2 * n,
+        C.c_void_p(dtau.data_ptr()) if dtau is not None else None, _stream()))
+
+
+def chol_inv_device(dA, dLi, precision: str = "f64"):
+    n = dA.shape[0]
+    _check("zolo_chol_inv_device", _lib.zolo_chol_inv_device(PREC[precision], n, C.c_void_p(dA.data_ptr()), n,
+                                                             C.c_void_p(dLi.data_ptr()), n, _stream()))
+
+
+def finalize():
+    _check("zolo_finalize", _lib.zolo_finalize())
