@@ -73,11 +73,18 @@ int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, dou
 int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                  int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info);
 
+/* Replaces PZOLOPD2 (pzolopd2.f90:2): as zolo_pd_host with the iteration cap of
+ * pzolopd2.f90:394 (12 instead of 6).  The restart branch of pzolopd2 calls
+ * FormXTwo, which the reference does not define; it is not reproduced. */
+int zolo_pd2_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
+                  int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info);
+
 /* ---- Bounds: alpha = ||A||_F, beta from LU + 1-norm estimate ---------------
  * test_zolodrv1.f90:179-189 (convention 0: beta = rcond/sqrt(n)) and
  * pqdwhfacs.f90:318-344 (convention 1: beta = ||A/alpha||_1 rcond/sqrt(n)). */
 int zolo_bounds_device(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta,
                        void* stream);
+int zolo_bounds_host(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta);
 
 /* ---- QDWH-PD: replaces PQDWHFACS (pqdwhfacs.f90:2) on a 1x1 grid ---------- */
 int zolo_qdwh_device(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu,
@@ -96,6 +103,13 @@ int zolo_qr_step_host(int64_t n, const double* X, int64_t ldx, double* Z, int64_
 /* ---- 2-norm estimate: replaces PDNORMEST (pdnormest.f90:2) ----------------- */
 int zolo_normest_device(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt,
                         void* stream);
+int zolo_normest_host(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt);
+
+/* ---- Structured QR of the 2n x n stack [X; s I] on host arrays -----------------
+ * MPDGEQRF (mpdgeqrf.f:5): S <- R + reflectors, tau out.
+ * MPDORGQR (mpdorgqr.f:5): S (reflectors) + tau -> S <- explicit Q (2n x n). */
+int zolo_mpdgeqrf_host(int64_t n, double* S, int64_t lds, double* tau);
+int zolo_mpdorgqr_host(int64_t n, double* S, int64_t lds, const double* tau);
 
 /* ---- building blocks (device pointers; used by tests and the benchmark) --- */
 /* C = alpha op(A) op(B) + beta C, precision 0 = F64, 1 = F32 (float data), 2 = BF16x3 (float data) */
@@ -107,6 +121,15 @@ int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n
 int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t ldq, double* tau, void* stream);
 /* Lower Cholesky with explicit inverse: A (lower) -> L, Li = L^{-1}. */
 int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* Li, int64_t ldl, void* stream);
+
+/* ---- measurement: GEMM launch ledger (no counterpart in the reference) ----
+ * While enabled, every MFMA GEMM launch is bracketed by hipEvents on its stream
+ * and its algorithmic flop count is recorded; zolo_profile_gemm sums launches
+ * matching (precision, transa, transb, tile_m, tile_n), -1 = any.  Enabling
+ * clears the ledger. */
+int zolo_profile_enable(int on);
+int zolo_profile_gemm(int precision, int transa, int transb, int tile_m, int tile_n, int64_t* launches, double* ms,
+                      double* flops);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,125 @@ int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, doubl
   return st;
 }
 
+int zolo_pd2_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
+                  int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) {
+  if (n < 1) return -4;
+  if (lda < n) return -8;
+  if (ldu < n) return -10;
+  if (H && ldh < n) return -12;
+  ZOLO_CHECK(ensure_device());
+  DevMat dA, dU, dH;
+  ZOLO_CHECK(dA.alloc(n));
+  ZOLO_CHECK(dU.alloc(n));
+  if (H) ZOLO_CHECK(dH.alloc(n));
+  ZOLO_CHECK(dA.upload(A, lda, n));
+  const int st = zolo_pd_run(precision, sym, rk, n, alpha, beta, dA.p, n, dU.p, n, H ? dH.p : nullptr, n, info,
+                             nullptr, 12);
+  if (st < 0) return st;
+  ZOLO_HIP(hipDeviceSynchronize());
+  ZOLO_CHECK(dU.download(U, ldu, n));
+  if (H) ZOLO_CHECK(dH.download(H, ldh, n));
+  return st;
+}
+
+int zolo_bounds_host(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta) {
+  if (convention != 0 && convention != 1) return -1;
+  if (n < 1) return -2;
+  if (lda < n) return -4;
+  ZOLO_CHECK(ensure_device());
+  DevMat dA;
+  ZOLO_CHECK(dA.alloc(n));
+  ZOLO_CHECK(dA.upload(A, lda, n));
+  const size_t bytes = bounds_bytes(n);
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  const int st = bounds_run(convention, n, dA.p, n, alpha, beta, ar, nullptr);
+  ZOLO_HIP(hipDeviceSynchronize());
+  return st;
+}
+
+int zolo_normest_host(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt) {
+  if (m < 1) return -1;
+  if (n < 1) return -2;
+  if (lda < m) return -4;
+  ZOLO_CHECK(ensure_device());
+  double* dA = nullptr;
+  if (hipMalloc(&dA, sizeof(double) * (size_t)m * n) != hipSuccess) {
+    set_error("normest: hipMalloc failed");
+    return kErrAlloc;
+  }
+  int st = kOk;
+  if (hipMemcpy2D(dA, m * sizeof(double), A, lda * sizeof(double), m * sizeof(double), n, hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    set_error("normest: copy failed");
+    st = kErrHip;
+  }
+  if (st == kOk) {
+    const size_t bytes = normest_bytes(m, n);
+    char* base = nullptr;
+    st = workspace_acquire(bytes, &base);
+    if (st == kOk) {
+      Arena ar(base, bytes);
+      st = normest_run(m, n, dA, m, tol, est, cnt, ar, nullptr);
+    }
+  }
+  (void)hipDeviceSynchronize();
+  (void)hipFree(dA);
+  return st;
+}
+
+// Host stacked QR pieces: the 2n x n stack S (ld lds) is copied to the device.
+static int stacked_host(int64_t n, double* S, int64_t lds, double* tau, bool geqrf) {
+  if (n < 1) return -1;
+  if (lds < 2 * n) return -3;
+  ZOLO_CHECK(ensure_device());
+  const int64_t ls = 2 * n;
+  double *dS = nullptr, *dQ = nullptr, *dt = nullptr;
+  int st = kOk;
+  if (hipMalloc(&dS, sizeof(double) * (size_t)ls * n) != hipSuccess ||
+      hipMalloc(&dQ, sizeof(double) * (size_t)ls * n) != hipSuccess ||
+      hipMalloc(&dt, sizeof(double) * (size_t)n) != hipSuccess) {
+    set_error("stacked qr: hipMalloc failed");
+    st = kErrAlloc;
+  }
+  if (st == kOk && (hipMemcpy2D(dS, ls * sizeof(double), S, lds * sizeof(double), ls * sizeof(double), n,
+                                hipMemcpyHostToDevice) != hipSuccess ||
+                    (!geqrf && hipMemcpy(dt, tau, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess))) {
+    set_error("stacked qr: copy failed");
+    st = kErrHip;
+  }
+  if (st == kOk) {
+    const size_t bytes = qr_step_bytes<double>(n, 1);
+    char* base = nullptr;
+    st = workspace_acquire(bytes, &base);
+    if (st == kOk) {
+      Arena ar(base, bytes);
+      Ctx c;
+      st = geqrf ? stacked_geqrf<double>(c, dS, ls, 0, n, 1, dt, ar)
+                 : stacked_orgqr<double>(c, dS, ls, 0, dt, dQ, ls, 0, n, 1, ar);
+    }
+  }
+  if (st == kOk) {
+    const double* src = geqrf ? dS : dQ;
+    if (hipMemcpy2D(S, lds * sizeof(double), src, ls * sizeof(double), ls * sizeof(double), n,
+                    hipMemcpyDeviceToHost) != hipSuccess ||
+        (geqrf && hipMemcpy(tau, dt, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)) {
+      set_error("stacked qr: copy back failed");
+      st = kErrHip;
+    }
+  }
+  (void)hipFree(dS);
+  (void)hipFree(dQ);
+  (void)hipFree(dt);
+  return st;
+}
+
+int zolo_mpdgeqrf_host(int64_t n, double* S, int64_t lds, double* tau) { return stacked_host(n, S, lds, tau, true); }
+int zolo_mpdorgqr_host(int64_t n, double* S, int64_t lds, const double* tau) {
+  return stacked_host(n, S, lds, const_cast<double*>(tau), false);
+}
+
 int zolo_bounds_device(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta,
                        void* stream) {
   if (convention != 0 && convention != 1) return -1;
@@ -239,6 +358,16 @@ int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n
   d.transA = transa; d.transB = transb; d.M = m; d.N = n; d.K = k;
   d.A = A; d.lda = lda; d.B = B; d.ldb = ldb; d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
   return gemm((Prec)precision, d, (hipStream_t)stream);
+}
+
+int zolo_profile_enable(int on) {
+  ledger_enable(on != 0);
+  return kOk;
+}
+
+int zolo_profile_gemm(int precision, int transa, int transb, int tile_m, int tile_n, int64_t* launches, double* ms,
+                      double* flops) {
+  return ledger_query(precision, transa, transb, tile_m, tile_n, launches, ms, flops);
 }
 
 int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t ldq, double* tau, void* stream) {

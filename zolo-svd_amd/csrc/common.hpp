@@ -68,4 +68,8 @@ enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
 
 int gemm(Prec p, const GemmDesc& d, hipStream_t s);
 
+// GEMM launch ledger (see gemm.hip).
+void ledger_enable(bool on);
+int ledger_query(int prec, int ta, int tb, int tm, int tn, int64_t* count, double* ms, double* flops);
+
 }  // namespace zolo

@@ -15,9 +15,66 @@
 // workgroup dispatch) each own a contiguous block of tiles, and inside a block
 // tiles walk down groups of 8 tile-rows so neighbouring workgroups share the
 // same A and B panels in their XCD's L2.
+#include <vector>
+
 #include "common.hpp"
 
 namespace zolo {
+
+// ---------------------------------------------------------------------------
+// Launch ledger: while enabled, every GEMM launch is bracketed by hipEvents on
+// its own stream and its algorithmic flop count is recorded, so a caller can
+// measure the engine's average launch duration over a real run.
+// ---------------------------------------------------------------------------
+namespace {
+struct LedgerEntry {
+  int prec, ta, tb, tm, tn;
+  double flops;
+  hipEvent_t e0, e1;
+};
+bool g_ledger_on = false;
+std::vector<LedgerEntry> g_ledger;
+
+double algorithmic_flops(const GemmDesc& d) {
+  // 2 M N K, halved for each triangular operand or output (band offsets are
+  // small next to the dimensions on every call site of the path).
+  double f = 2.0 * (double)d.M * (double)d.N * (double)d.K * d.batch;
+  const bool triA = d.aLo >= 0 || d.aHi >= 0, triB = d.bLo >= 0 || d.bHi >= 0;
+  if (triA) f *= 0.5;
+  if (triB) f *= 0.5;
+  if (d.lowerC) f *= 0.5;
+  return f;
+}
+}  // namespace
+
+void ledger_enable(bool on) {
+  for (auto& e : g_ledger) {
+    (void)hipEventDestroy(e.e0);
+    (void)hipEventDestroy(e.e1);
+  }
+  g_ledger.clear();
+  g_ledger_on = on;
+}
+
+// Sum over recorded launches matching (prec, ta, tb, tm, tn); -1 matches any.
+int ledger_query(int prec, int ta, int tb, int tm, int tn, int64_t* count, double* ms, double* flops) {
+  *count = 0;
+  *ms = 0.0;
+  *flops = 0.0;
+  for (auto& e : g_ledger) {
+    if ((prec >= 0 && e.prec != prec) || (ta >= 0 && e.ta != ta) || (tb >= 0 && e.tb != tb) ||
+        (tm >= 0 && e.tm != tm) || (tn >= 0 && e.tn != tn))
+      continue;
+    float t = 0.f;
+    ZOLO_HIP(hipEventSynchronize(e.e1));
+    ZOLO_HIP(hipEventElapsedTime(&t, e.e0, e.e1));
+    *count += 1;
+    *ms += t;
+    *flops += e.flops;
+  }
+  return kOk;
+}
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -383,10 +440,29 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmDesc d) {
   }
 }
 
+struct LedgerScope {
+  bool on;
+  LedgerEntry e;
+  hipStream_t s;
+  LedgerScope(const GemmDesc& d, int prec, int tm, int tn, hipStream_t st) : on(g_ledger_on), s(st) {
+    if (!on) return;
+    e.prec = prec; e.ta = d.transA; e.tb = d.transB; e.tm = tm; e.tn = tn;
+    e.flops = algorithmic_flops(d);
+    if (hipEventCreate(&e.e0) != hipSuccess || hipEventCreate(&e.e1) != hipSuccess) { on = false; return; }
+    (void)hipEventRecord(e.e0, s);
+  }
+  ~LedgerScope() {
+    if (!on) return;
+    (void)hipEventRecord(e.e1, s);
+    g_ledger.push_back(e);
+  }
+};
+
 template <typename T, int TBM, int TBN>
 int launch_tile(const GemmDesc& d, hipStream_t s) {
   const int64_t tiles = cdiv(d.M, TBM) * cdiv(d.N, TBN);
   dim3 grid((unsigned)tiles, 1, (unsigned)d.batch);
+  LedgerScope ls(d, sizeof(T) == 8 ? 0 : 1, TBM, TBN, s);
   if (d.transA == 0 && d.transB == 0) hipLaunchKernelGGL((gemm_kernel<T, 0, 0, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
   else if (d.transA == 0) hipLaunchKernelGGL((gemm_kernel<T, 0, 1, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
   else if (d.transB == 0) hipLaunchKernelGGL((gemm_kernel<T, 1, 0, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
@@ -407,6 +483,7 @@ int launch_typed(const GemmDesc& d, hipStream_t s) {
 int launch_bf16x3(const GemmDesc& d, hipStream_t s) {
   const int64_t tiles = cdiv(d.M, BM) * cdiv(d.N, BN);
   dim3 grid((unsigned)tiles, 1, (unsigned)d.batch);
+  LedgerScope ls(d, 2, BM, BN, s);
   if (d.transA == 0 && d.transB == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<0, 0>), grid, dim3(kThreads), 0, s, d);
   else if (d.transA == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<0, 1>), grid, dim3(kThreads), 0, s, d);
   else if (d.transB == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<1, 0>), grid, dim3(kThreads), 0, s, d);

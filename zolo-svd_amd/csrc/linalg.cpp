@@ -207,22 +207,19 @@ static size_t stacked_qr_bytes(int64_t n, int r) {
 }
 
 template <typename T>
-int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
-               T* tau_out, Arena& ar) {
+int stacked_geqrf(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, Arena& ar) {
   const int nb = kQrNb;
   const int64_t mr = n + nb;
   const int nwg_max = qr_nwg(mr);
   const int64_t npan = (n + nb - 1) / nb;
   T* V = ar.take<T>((size_t)batch * mr * nb);
   T* G = ar.take<T>((size_t)batch * nb * nb);
-  T* Tall = ar.take<T>((size_t)batch * npan * nb * nb);
+  T* Tp = ar.take<T>((size_t)batch * nb * nb);
   T* W = ar.take<T>((size_t)batch * nb * n);
   T* W2 = ar.take<T>((size_t)batch * nb * n);
   double* part = ar.take<double>((size_t)batch * nb * ((size_t)nwg_max * nb + nb));
-  T* tau = tau_out ? tau_out : ar.take<T>((size_t)batch * n);
   ZOLO_ARENA_OK(ar);
-  const int64_t sV = mr * nb, sG = (int64_t)nb * nb, sT = npan * nb * nb, sW = (int64_t)nb * n;
-
+  const int64_t sV = mr * nb, sG = (int64_t)nb * nb, sW = (int64_t)nb * n;
   for (int64_t pi = 0; pi < npan; ++pi) {
     const int64_t p = pi * nb;
     const int jb = (int)std::min<int64_t>(nb, n - p);
@@ -230,53 +227,72 @@ int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, i
     const int nwg = qr_nwg(mrows);
     for (int64_t j = p; j <= p + jb; ++j)
       ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, j, j > p, j < p + jb, part, nwg, tau, n, batch, c.s));
+    const int64_t ncols = n - p - jb;
+    if (ncols <= 0) continue;
     ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, p, jb, mrows, batch, c.s));
-    GemmDesc d; d.batch = batch;
+    GemmDesc d; d.batch = batch;           // G = V^T V
     d.transA = 1; d.transB = 0; d.M = jb; d.N = jb; d.K = mrows;
     d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
     d.C = G; d.ldc = jb; d.strideC = sG;
     ZOLO_CHECK(gemm_t<T>(c, d));
-    T* Tp = Tall + pi * nb * nb;
-    ZOLO_CHECK(k_qr_form_t<T>(G, jb, sG, tau, n, p, jb, Tp, jb, sT, batch, c.s));
-    const int64_t ncols = n - p - jb;
-    if (ncols > 0) {
-      T* Cm = S + p + (p + jb) * lds;
-      d = GemmDesc(); d.batch = batch;     // W = V^T C
-      d.transA = 1; d.transB = 0; d.M = jb; d.N = ncols; d.K = mrows;
-      d.A = V; d.lda = mrows; d.strideA = sV; d.B = Cm; d.ldb = lds; d.strideB = sS;
-      d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
-      ZOLO_CHECK(gemm_t<T>(c, d));
-      d = GemmDesc(); d.batch = batch;     // W2 = T^T W
-      d.transA = 1; d.transB = 0; d.M = jb; d.N = ncols; d.K = jb;
-      d.A = Tp; d.lda = jb; d.strideA = sT; d.B = W; d.ldb = jb; d.strideB = sW;
-      d.C = W2; d.ldc = jb; d.strideC = sW;
-      ZOLO_CHECK(gemm_t<T>(c, d));
-      d = GemmDesc(); d.batch = batch;     // C -= V W2
-      d.transA = 0; d.transB = 0; d.M = mrows; d.N = ncols; d.K = jb;
-      d.A = V; d.lda = mrows; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
-      d.C = Cm; d.ldc = lds; d.strideC = sS; d.alpha = -1.0; d.beta = 1.0;
-      ZOLO_CHECK(gemm_t<T>(c, d));
-    }
+    ZOLO_CHECK(k_qr_form_t<T>(G, jb, sG, tau, n, p, jb, Tp, jb, sG, batch, c.s));
+    T* Cm = S + p + (p + jb) * lds;
+    d = GemmDesc(); d.batch = batch;       // W = V^T C
+    d.transA = 1; d.transB = 0; d.M = jb; d.N = ncols; d.K = mrows;
+    d.A = V; d.lda = mrows; d.strideA = sV; d.B = Cm; d.ldb = lds; d.strideB = sS;
+    d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
+    ZOLO_CHECK(gemm_t<T>(c, d));
+    d = GemmDesc(); d.batch = batch;       // W2 = T^T W
+    d.transA = 1; d.transB = 0; d.M = jb; d.N = ncols; d.K = jb;
+    d.A = Tp; d.lda = jb; d.strideA = sG; d.B = W; d.ldb = jb; d.strideB = sW;
+    d.C = W2; d.ldc = jb; d.strideC = sW;
+    ZOLO_CHECK(gemm_t<T>(c, d));
+    d = GemmDesc(); d.batch = batch;       // C -= V W2
+    d.transA = 0; d.transB = 0; d.M = mrows; d.N = ncols; d.K = jb;
+    d.A = V; d.lda = mrows; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
+    d.C = Cm; d.ldc = lds; d.strideC = sS; d.alpha = -1.0; d.beta = 1.0;
+    ZOLO_CHECK(gemm_t<T>(c, d));
   }
-  if (!Q) return kOk;
-  // Explicit Q = H_0 ... H_{n-1} [I; 0], panels applied last to first.
+  return kOk;
+}
+
+template <typename T>
+int stacked_orgqr(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, T* Q, int64_t ldq, int64_t sQ,
+                  int64_t n, int batch, Arena& ar) {
+  const int nb = kQrNb;
+  const int64_t mr = n + nb;
+  const int64_t npan = (n + nb - 1) / nb;
+  T* V = ar.take<T>((size_t)batch * mr * nb);
+  T* G = ar.take<T>((size_t)batch * nb * nb);
+  T* Tp = ar.take<T>((size_t)batch * nb * nb);
+  T* W = ar.take<T>((size_t)batch * nb * n);
+  T* W2 = ar.take<T>((size_t)batch * nb * n);
+  ZOLO_ARENA_OK(ar);
+  const int64_t sV = mr * nb, sG = (int64_t)nb * nb, sW = (int64_t)nb * n;
+  // Explicit Q = H_0 ... H_{n-1} [I; 0], panels applied last to first; the
+  // block reflector T of each panel is re-formed from V^T V and tau.
   ZOLO_CHECK(k_set_identity_top<T>(Q, ldq, sQ, 2 * n, n, batch, c.s));
   for (int64_t pi = npan - 1; pi >= 0; --pi) {
     const int64_t p = pi * nb;
     const int jb = (int)std::min<int64_t>(nb, n - p);
     const int64_t mrows = n + jb;
     ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, p, jb, mrows, batch, c.s));
-    T* Tp = Tall + pi * nb * nb;
+    GemmDesc d; d.batch = batch;           // G = V^T V
+    d.transA = 1; d.transB = 0; d.M = jb; d.N = jb; d.K = mrows;
+    d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
+    d.C = G; d.ldc = jb; d.strideC = sG;
+    ZOLO_CHECK(gemm_t<T>(c, d));
+    ZOLO_CHECK(k_qr_form_t<T>(G, jb, sG, tau, n, p, jb, Tp, jb, sG, batch, c.s));
     T* Qs = Q + p + p * ldq;
     const int64_t nc = n - p;
-    GemmDesc d; d.batch = batch;           // W = V^T Qs
+    d = GemmDesc(); d.batch = batch;       // W = V^T Qs
     d.transA = 1; d.transB = 0; d.M = jb; d.N = nc; d.K = mrows;
     d.A = V; d.lda = mrows; d.strideA = sV; d.B = Qs; d.ldb = ldq; d.strideB = sQ;
     d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
     ZOLO_CHECK(gemm_t<T>(c, d));
     d = GemmDesc(); d.batch = batch;       // W2 = T W
     d.transA = 0; d.transB = 0; d.M = jb; d.N = nc; d.K = jb;
-    d.A = Tp; d.lda = jb; d.strideA = sT; d.B = W; d.ldb = jb; d.strideB = sW;
+    d.A = Tp; d.lda = jb; d.strideA = sG; d.B = W; d.ldb = jb; d.strideB = sW;
     d.C = W2; d.ldc = jb; d.strideC = sW;
     ZOLO_CHECK(gemm_t<T>(c, d));
     d = GemmDesc(); d.batch = batch;       // Qs -= V W2
@@ -286,6 +302,19 @@ int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, i
     ZOLO_CHECK(gemm_t<T>(c, d));
   }
   return kOk;
+}
+
+template <typename T>
+int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
+               T* tau_out, Arena& ar) {
+  T* tau = tau_out ? tau_out : ar.take<T>((size_t)batch * n);
+  ZOLO_ARENA_OK(ar);
+  const size_t mark = ar.used();
+  ZOLO_CHECK(stacked_geqrf<T>(c, S, lds, sS, n, batch, tau, ar));
+  if (!Q) return kOk;
+  ar.reset();  // ORGQR reuses the GEQRF scratch
+  ar.take<char>(mark);
+  return stacked_orgqr<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, ar);
 }
 
 template <typename T>
@@ -322,7 +351,10 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
   template size_t qr_step_bytes<T>(int64_t, int);                                                                \
   template int qr_step<T>(const Ctx&, const T*, int64_t, int64_t, int, const double*, const double*, T*, int64_t, \
                           int64_t, Arena&);                                                                      \
-  template int stacked_qr<T>(const Ctx&, T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, T*, Arena&);
+  template int stacked_qr<T>(const Ctx&, T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, T*, Arena&);   \
+  template int stacked_geqrf<T>(const Ctx&, T*, int64_t, int64_t, int64_t, int, T*, Arena&);                      \
+  template int stacked_orgqr<T>(const Ctx&, const T*, int64_t, int64_t, const T*, T*, int64_t, int64_t, int64_t, \
+                                int, Arena&);
 ZOLO_LA_INST(double)
 ZOLO_LA_INST(float)
 

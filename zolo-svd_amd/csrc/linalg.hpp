@@ -71,7 +71,13 @@ template <typename T>
 int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const double* d_sdiag, const double* d_alpha,
             T* Z, int64_t ldz, int64_t sZ, Arena& ar);
 
-// Structured GEQRF + ORGQR of a batch of 2n x n stacks (exposed for tests).
+// Structured GEQRF (R + reflectors in place, tau out) and ORGQR (explicit
+// 2n x n Q from the reflectors) of a batch of 2n x n stacks; stacked_qr does both.
+template <typename T>
+int stacked_geqrf(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, Arena& ar);
+template <typename T>
+int stacked_orgqr(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, T* Q, int64_t ldq, int64_t sQ,
+                  int64_t n, int batch, Arena& ar);
 template <typename T>
 int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
                T* tau_out, Arena& ar);

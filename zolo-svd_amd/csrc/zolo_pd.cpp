@@ -96,7 +96,7 @@ int form_h(const Ctx& c, const double* U, int64_t ldu, const double* A, int64_t 
 // Zolo-PD
 // ---------------------------------------------------------------------------
 int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A, int64_t lda,
-                double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s) {
+                double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s, int max_it) {
   if (precision < 0 || precision > 2) return -1;
   if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -2;
   if (rk < 1 || rk > 8) return -3;
@@ -159,7 +159,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
   bool have_ref = false;     // Xref holds the last FP64 iterate
   std::vector<double> coeff(3 * rk);
   double hs[8], ha[8];
-  while (err > tol2 && it < 6) {
+  while (err > tol2 && it < max_it) {
     compute_coeff(con, tol, rk, coeff.data());
     double maxc = 0.0;
     for (int i = 0; i < 2 * rk; ++i) maxc = std::max(maxc, coeff[i]);

@@ -1,1 +1,509 @@
-! placeholder
+!> Fortran drop-in entry points of the Zolo-PD hot path on one GPU.
+!>
+!> The external subroutines keep the reference's names and argument lists
+!> (pzolopd1.f90:2, pzolopd2.f90:2, pqdwhfacs.f90:2, zolochol.f90:2,
+!> zoloqr.f90:2, pdnormest.f90:2, mpdgeqrf.f:5, mpdorgqr.f:5, FormX2.f90:2,
+!> hfunc.f90:2) and module auxil_zolo keeps Auxil_Zolo.f90's interface.  The
+!> descriptors must describe a 1x1 process grid (RSRC = CSRC = 0, LLD >= M);
+!> the BLACS context is never dereferenced, so the library links no MPI or
+!> BLACS.  Work is done by libzolo.so through the ISO_C_BINDING interfaces of
+!> module zolo_capi; arrays are host arrays, copied to and from HBM.
+module zolo_capi
+  use iso_c_binding
+  implicit none
+  private :: c_int, c_double, c_int64_t, c_char, c_ptr
+
+  type, bind(C) :: zolo_info_t
+    integer(c_int) :: iters, r, info, polish_passes
+    character(kind=c_char) :: schedule(16)
+    real(c_double) :: err(16), con(16), maxc(16)
+    real(c_double) :: alpha, beta, time_ms
+    real(c_double) :: phase_ms(8)
+  end type zolo_info_t
+
+  !> arithmetic of the Cholesky iterations: 0 FP64 (default), 1 FP32, 2 BF16x3
+  integer, save :: zolo_precision = 0
+  !> record of the last PZOLOPD1 / PZOLOPD2 / PQDWHFACS call
+  type(zolo_info_t), save, target :: zolo_last_info
+
+  interface
+    integer(c_int) function c_zolo_pd_host(precision, sym, rk, n, alpha, beta, a, lda, u, ldu, h, ldh, info) &
+        bind(C, name='zolo_pd_host')
+      import :: c_int, c_char, c_int64_t, c_double, c_ptr, zolo_info_t
+      integer(c_int), value :: precision, rk
+      character(kind=c_char), value :: sym
+      integer(c_int64_t), value :: n, lda, ldu, ldh
+      real(c_double), value :: alpha, beta
+      real(c_double), intent(in) :: a(*)
+      real(c_double) :: u(*)
+      type(c_ptr), value :: h
+      type(zolo_info_t) :: info
+    end function
+    integer(c_int) function c_zolo_pd2_host(precision, sym, rk, n, alpha, beta, a, lda, u, ldu, h, ldh, info) &
+        bind(C, name='zolo_pd2_host')
+      import :: c_int, c_char, c_int64_t, c_double, c_ptr, zolo_info_t
+      integer(c_int), value :: precision, rk
+      character(kind=c_char), value :: sym
+      integer(c_int64_t), value :: n, lda, ldu, ldh
+      real(c_double), value :: alpha, beta
+      real(c_double), intent(in) :: a(*)
+      real(c_double) :: u(*)
+      type(c_ptr), value :: h
+      type(zolo_info_t) :: info
+    end function
+    integer(c_int) function c_zolo_qdwh_host(sym, n, sigma, a, lda, u, ldu, h, ldh, info) bind(C, name='zolo_qdwh_host')
+      import :: c_int, c_char, c_int64_t, c_double, c_ptr, zolo_info_t
+      character(kind=c_char), value :: sym
+      integer(c_int64_t), value :: n, lda, ldu, ldh
+      real(c_double), value :: sigma
+      real(c_double), intent(in) :: a(*)
+      real(c_double) :: u(*)
+      type(c_ptr), value :: h
+      type(zolo_info_t) :: info
+    end function
+    integer(c_int) function c_zolo_bounds_host(conv, n, a, lda, alpha, beta) bind(C, name='zolo_bounds_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: conv
+      integer(c_int64_t), value :: n, lda
+      real(c_double), intent(in) :: a(*)
+      real(c_double) :: alpha, beta
+    end function
+    integer(c_int) function c_zolo_chol_step_host(sym, n, x, ldx, z, ldz, coeff, rk, ip) &
+        bind(C, name='zolo_chol_step_host')
+      import :: c_int, c_char, c_int64_t, c_double
+      character(kind=c_char), value :: sym
+      integer(c_int64_t), value :: n, ldx, ldz
+      real(c_double), intent(in) :: x(*), coeff(*)
+      real(c_double) :: z(*)
+      integer(c_int), value :: rk, ip
+    end function
+    integer(c_int) function c_zolo_qr_step_host(n, x, ldx, z, ldz, coeff, rk, ip) bind(C, name='zolo_qr_step_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n, ldx, ldz
+      real(c_double), intent(in) :: x(*), coeff(*)
+      real(c_double) :: z(*)
+      integer(c_int), value :: rk, ip
+    end function
+    integer(c_int) function c_zolo_normest_host(m, n, a, lda, tol, est, cnt) bind(C, name='zolo_normest_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: m, n, lda
+      real(c_double), intent(in) :: a(*)
+      real(c_double), value :: tol
+      real(c_double) :: est
+      integer(c_int) :: cnt
+    end function
+    integer(c_int) function c_zolo_mpdgeqrf_host(n, s, lds, tau) bind(C, name='zolo_mpdgeqrf_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n, lds
+      real(c_double) :: s(*), tau(*)
+    end function
+    integer(c_int) function c_zolo_mpdorgqr_host(n, s, lds, tau) bind(C, name='zolo_mpdorgqr_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n, lds
+      real(c_double) :: s(*)
+      real(c_double), intent(in) :: tau(*)
+    end function
+    integer(c_int) function c_zolo_compute_coeff(con, tol, rk, coeff) bind(C, name='zolo_compute_coeff')
+      import :: c_int, c_double
+      real(c_double), value :: con, tol
+      integer(c_int), value :: rk
+      real(c_double) :: coeff(*)
+    end function
+    integer(c_int) function c_zolo_choosem(con) bind(C, name='zolo_choosem')
+      import :: c_int, c_double
+      real(c_double), value :: con
+    end function
+    real(c_double) function c_zolo_hfunc(x) bind(C, name='zolo_hfunc')
+      import :: c_double
+      real(c_double), value :: x
+    end function
+    subroutine c_zolo_mellipke(alpha, tol, k, e) bind(C, name='zolo_mellipke')
+      import :: c_double
+      real(c_double), value :: alpha, tol
+      real(c_double) :: k, e
+    end subroutine
+    subroutine c_zolo_mellipj(u, alpha, tol, sn, cn, dn) bind(C, name='zolo_mellipj')
+      import :: c_double
+      real(c_double), value :: u, alpha, tol
+      real(c_double) :: sn, cn, dn
+    end subroutine
+  end interface
+
+contains
+
+  !> Select the arithmetic of the Cholesky iterations (0 FP64, 1 FP32, 2 BF16x3).
+  subroutine zolo_set_precision(p)
+    integer, intent(in) :: p
+    zolo_precision = p
+  end subroutine zolo_set_precision
+
+  !> 0 if DESC describes an M x N matrix on a 1x1 grid, else the index of the bad entry.
+  integer function zolo_desc_check(desc, m, n)
+    integer, intent(in) :: desc(9), m, n
+    zolo_desc_check = 0
+    if (desc(1) /= 1) then
+      zolo_desc_check = 1
+    else if (desc(3) < m) then
+      zolo_desc_check = 3
+    else if (desc(4) < n) then
+      zolo_desc_check = 4
+    else if (desc(7) /= 0) then
+      zolo_desc_check = 7
+    else if (desc(8) /= 0) then
+      zolo_desc_check = 8
+    else if (desc(9) < max(1, m)) then
+      zolo_desc_check = 9
+    end if
+  end function zolo_desc_check
+
+  !> alpha = ||A||_F and beta (test_zolodrv1.f90:179-189 for conv=0, pqdwhfacs.f90:318-344 for conv=1).
+  subroutine zolo_bounds(conv, n, a, lda, alpha, beta, info)
+    integer, intent(in) :: conv, n, lda
+    double precision, intent(in) :: a(lda, *)
+    double precision, intent(out) :: alpha, beta
+    integer, intent(out) :: info
+    info = c_zolo_bounds_host(int(conv, c_int), int(n, c_int64_t), a, int(lda, c_int64_t), alpha, beta)
+  end subroutine zolo_bounds
+end module zolo_capi
+
+!> Auxil_Zolo.f90's module interface (mellipke, mellipj, choosem, compute_coeff).
+module auxil_zolo
+  use zolo_capi
+  use iso_c_binding, only: c_int, c_double
+  implicit none
+  public :: mellipke, mellipj, choosem, compute_coeff
+contains
+  subroutine mellipke(alpha, tol, k, e)
+    double precision, intent(in) :: alpha, tol
+    double precision, intent(out) :: k, e
+    call c_zolo_mellipke(alpha, tol, k, e)
+  end subroutine mellipke
+
+  subroutine mellipj(u, alpha, tol, sn, cn, dn)
+    double precision, intent(in) :: u, alpha, tol
+    double precision, intent(out) :: sn, cn, dn
+    call c_zolo_mellipj(u, alpha, tol, sn, cn, dn)
+  end subroutine mellipj
+
+  function choosem(con) result(m)
+    double precision :: con
+    integer :: m
+    m = c_zolo_choosem(con)
+  end function choosem
+
+  subroutine compute_coeff(con, tol, rk, coeff)
+    integer, intent(in) :: rk
+    double precision, intent(in) :: con, tol
+    double precision, intent(inout) :: coeff(:)
+    double precision :: tmp(24)
+    integer :: st
+    st = c_zolo_compute_coeff(con, tol, int(rk, c_int), tmp)
+    if (st == 0) coeff(1:3*rk) = tmp(1:3*rk)
+  end subroutine compute_coeff
+end module auxil_zolo
+
+!> PZOLOPD1 (pzolopd1.f90:2): Zolo-PD with caller-supplied ALPHA, BETA.  Q = polar factor.
+subroutine pzolopd1(sym, rk, m, n, alpha, beta, as, ia, ja, desca, q, iq, jq, descq, info, polar)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  character, intent(in) :: sym, polar
+  integer, intent(in) :: rk, m, n, ia, ja, iq, jq, desca(9), descq(9)
+  double precision, intent(in) :: alpha, beta, as(*)
+  double precision, intent(inout) :: q(*)
+  integer, intent(out) :: info
+  call zolo_pd_common(1)
+contains
+  subroutine zolo_pd_common(which)
+    integer, intent(in) :: which
+    integer :: k, st, ioa, ioq
+    character(kind=c_char) :: cs
+    info = 0
+    if (sym /= 'S' .and. sym /= 's' .and. sym /= 'N' .and. sym /= 'n') then
+      info = -1
+    else if (rk < 1 .or. rk > 8) then
+      info = -2
+    else if (m /= n) then
+      info = -3
+    else if (n < 0) then
+      info = -4
+    else if (.not. (alpha > 0d0)) then
+      info = -5
+    else if (.not. (beta > 0d0 .and. beta <= 1d0)) then
+      info = -6
+    end if
+    if (info /= 0) return
+    k = zolo_desc_check(desca, ia - 1 + m, ja - 1 + n)
+    if (k /= 0) then
+      info = -(1000 + k)
+      return
+    end if
+    k = zolo_desc_check(descq, iq - 1 + m, jq - 1 + n)
+    if (k /= 0) then
+      info = -(1400 + k)
+      return
+    end if
+    if (n == 0) return
+    ioa = ia + (ja - 1) * desca(9)
+    ioq = iq + (jq - 1) * descq(9)
+    cs = sym
+    if (which == 1) then
+      st = c_zolo_pd_host(int(zolo_precision, c_int), cs, int(rk, c_int), int(n, c_int64_t), alpha, beta, as(ioa), &
+                          int(desca(9), c_int64_t), q(ioq), int(descq(9), c_int64_t), c_null_ptr, 1_c_int64_t, &
+                          zolo_last_info)
+    else
+      st = c_zolo_pd2_host(int(zolo_precision, c_int), cs, int(rk, c_int), int(n, c_int64_t), alpha, beta, as(ioa), &
+                           int(desca(9), c_int64_t), q(ioq), int(descq(9), c_int64_t), c_null_ptr, 1_c_int64_t, &
+                           zolo_last_info)
+    end if
+    info = st
+    if (polar == polar) continue  ! POLAR is ignored, as in pzolopd1.f90:504-511
+  end subroutine zolo_pd_common
+end subroutine pzolopd1
+
+!> PZOLOPD2 (pzolopd2.f90:2): as PZOLOPD1 with the iteration cap of pzolopd2.f90:394.
+subroutine pzolopd2(sym, rk, m, n, alpha, beta, as, ia, ja, desca, q, iq, jq, descq, info, polar)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  character, intent(in) :: sym, polar
+  integer, intent(in) :: rk, m, n, ia, ja, iq, jq, desca(9), descq(9)
+  double precision, intent(in) :: alpha, beta, as(*)
+  double precision, intent(inout) :: q(*)
+  integer, intent(out) :: info
+  integer :: k, ioa, ioq
+  character(kind=c_char) :: cs
+  info = 0
+  if (sym /= 'S' .and. sym /= 's' .and. sym /= 'N' .and. sym /= 'n') then
+    info = -1
+  else if (rk < 1 .or. rk > 8) then
+    info = -2
+  else if (m /= n) then
+    info = -3
+  else if (.not. (alpha > 0d0)) then
+    info = -5
+  else if (.not. (beta > 0d0 .and. beta <= 1d0)) then
+    info = -6
+  end if
+  if (info /= 0) return
+  k = zolo_desc_check(desca, ia - 1 + m, ja - 1 + n)
+  if (k /= 0) then
+    info = -(1000 + k)
+    return
+  end if
+  k = zolo_desc_check(descq, iq - 1 + m, jq - 1 + n)
+  if (k /= 0) then
+    info = -(1400 + k)
+    return
+  end if
+  if (n == 0) return
+  ioa = ia + (ja - 1) * desca(9)
+  ioq = iq + (jq - 1) * descq(9)
+  cs = sym
+  info = c_zolo_pd2_host(int(zolo_precision, c_int), cs, int(rk, c_int), int(n, c_int64_t), alpha, beta, as(ioa), &
+                         int(desca(9), c_int64_t), q(ioq), int(descq(9), c_int64_t), c_null_ptr, 1_c_int64_t, &
+                         zolo_last_info)
+  if (polar == polar) continue
+end subroutine pzolopd2
+
+!> PQDWHFACS (pqdwhfacs.f90:2): QDWH-PD with shift SIGMA; H in WORK2 when POLAR = 'T'.
+!> LWORK = -1 or LWORK2 = -1 is a workspace query (WORK(1), WORK(2)); the library
+!> needs no caller workspace but keeps the reference's size contract.
+subroutine pqdwhfacs(sym, m, n, sigma, a, ia, ja, desca, q, iq, jq, descq, work, lwork, descw, tau, work2, lwork2, &
+                     info, polar)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  character, intent(in) :: sym, polar
+  integer, intent(in) :: m, n, ia, ja, iq, jq, lwork, lwork2, desca(9), descq(9), descw(9)
+  double precision, intent(in) :: sigma, a(*)
+  double precision, intent(inout) :: q(*), work(*), tau(*)
+  double precision, intent(inout), target :: work2(*)
+  integer, intent(out) :: info
+  integer :: k, lwin, lwin2, ioa, ioq
+  logical :: lquery
+  character(kind=c_char) :: cs
+  type(c_ptr) :: hp
+  info = 0
+  if (m < n) then
+    info = -1
+    return
+  end if
+  if (m /= n) then
+    info = -2
+    return
+  end if
+  lquery = (lwork == -1 .or. lwork2 == -1)
+  if (m == 0 .or. n == 0) return
+  lwin = 2 * n * n
+  lwin2 = n * n
+  work(1) = dble(lwin)
+  work(2) = dble(lwin2)
+  if (lquery) return
+  if (lwork < lwin) then
+    info = -12
+    return
+  else if (lwork2 < lwin2) then
+    info = -16
+    return
+  end if
+  if (sym /= 'S' .and. sym /= 's' .and. sym /= 'N' .and. sym /= 'n') then
+    info = -1
+    return
+  end if
+  k = zolo_desc_check(desca, ia - 1 + m, ja - 1 + n)
+  if (k /= 0) then
+    info = -(800 + k)
+    return
+  end if
+  k = zolo_desc_check(descq, iq - 1 + m, jq - 1 + n)
+  if (k /= 0) then
+    info = -(1200 + k)
+    return
+  end if
+  if (descw(1) == -12345) tau(1) = 0d0  ! DESCW and TAU are accepted but unused
+  ioa = ia + (ja - 1) * desca(9)
+  ioq = iq + (jq - 1) * descq(9)
+  cs = sym
+  hp = c_null_ptr
+  if (polar == 'T' .or. polar == 't') hp = c_loc(work2(ioq))
+  info = c_zolo_qdwh_host(cs, int(n, c_int64_t), sigma, a(ioa), int(desca(9), c_int64_t), q(ioq), &
+                          int(descq(9), c_int64_t), hp, int(descq(9), c_int64_t), zolo_last_info)
+end subroutine pqdwhfacs
+
+!> ZOLOCHOL (zolochol.f90:2): Z = -a_IP X (X^T X + c_{2IP-1} I)^{-1} for the local n x n X in A.
+subroutine zolochol(sym, n, nblk, fone, a, z, contxt, work, coeff, rk, ip, info)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  character, intent(in) :: sym
+  integer, intent(in) :: n, nblk, contxt, rk, ip
+  double precision, intent(in) :: fone, a(*), coeff(*)
+  double precision, intent(inout) :: z(*), work(*)
+  integer, intent(out) :: info
+  character(kind=c_char) :: cs
+  cs = sym
+  if (nblk < 0 .or. contxt < -1 .or. fone /= fone) work(1) = 0d0  ! unused, as in zolochol.f90:81-84
+  info = c_zolo_chol_step_host(cs, int(n, c_int64_t), a, int(n, c_int64_t), z, int(n, c_int64_t), coeff, &
+                               int(rk, c_int), int(ip, c_int))
+end subroutine zolochol
+
+!> ZOLOQR (zoloqr.f90:2): Z = -a_IP/sqrt(c_{2IP-1}) Q1 Q2^T with [X; sqrt(c) I] = QR.
+subroutine zoloqr(m, n, nblk, a, z, contxt, workq, tau, coeff, rk, ip, info, it)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: m, n, nblk, contxt, rk, ip, it
+  double precision, intent(in) :: a(*), coeff(*)
+  double precision, intent(inout) :: z(*), workq(*), tau(*)
+  integer, intent(out) :: info
+  if (m /= n) then
+    info = -1
+    return
+  end if
+  if (nblk < 0 .or. contxt < -1 .or. it < -1) workq(1) = tau(1)
+  info = c_zolo_qr_step_host(int(n, c_int64_t), a, int(n, c_int64_t), z, int(n, c_int64_t), coeff, int(rk, c_int), &
+                             int(ip, c_int))
+end subroutine zoloqr
+
+!> PDNORMEST (pdnormest.f90:2): 2-norm estimate by power iteration (A then A^T).
+subroutine pdnormest(m, n, a, ia, ja, desca, tol, cond, cnt)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: m, n, ia, ja, desca(9)
+  double precision, intent(in) :: tol, a(*)
+  integer, intent(inout) :: cnt
+  double precision, intent(inout) :: cond
+  integer(c_int) :: c
+  integer :: st, ioa
+  ioa = ia + (ja - 1) * desca(9)
+  st = c_zolo_normest_host(int(m, c_int64_t), int(n, c_int64_t), a(ioa), int(desca(9), c_int64_t), tol, cond, c)
+  cnt = c
+  if (st /= 0) cnt = -abs(st)
+end subroutine pdnormest
+
+!> MPDGEQRF (mpdgeqrf.f:5): structured QR of the M = 2N stack [X; diag] in place.
+subroutine mpdgeqrf(m, n, a, ia, ja, desca, tau, work, lwork, info)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: m, n, ia, ja, lwork, desca(9)
+  double precision, intent(inout) :: a(*), tau(*), work(*)
+  integer, intent(out) :: info
+  integer :: ioa
+  info = 0
+  work(1) = dble(max(1, n))
+  if (lwork == -1) return
+  if (m /= 2 * n) then
+    info = -1
+    return
+  end if
+  if (zolo_desc_check(desca, ia - 1 + m, ja - 1 + n) /= 0) then
+    info = -(600 + zolo_desc_check(desca, ia - 1 + m, ja - 1 + n))
+    return
+  end if
+  ioa = ia + (ja - 1) * desca(9)
+  info = c_zolo_mpdgeqrf_host(int(n, c_int64_t), a(ioa), int(desca(9), c_int64_t), tau)
+end subroutine mpdgeqrf
+
+!> MPDORGQR (mpdorgqr.f:5): explicit Q of the structured stack from its reflectors, in place.
+subroutine mpdorgqr(m, n, k, a, ia, ja, desca, tau, work, lwork, info)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: m, n, k, ia, ja, lwork, desca(9)
+  double precision, intent(inout) :: a(*), work(*)
+  double precision, intent(in) :: tau(*)
+  integer, intent(out) :: info
+  integer :: ioa
+  info = 0
+  work(1) = dble(max(1, n))
+  if (lwork == -1) return
+  if (m /= 2 * n) then
+    info = -1
+    return
+  end if
+  if (k /= n) then
+    info = -3
+    return
+  end if
+  if (zolo_desc_check(desca, ia - 1 + m, ja - 1 + n) /= 0) then
+    info = -(700 + zolo_desc_check(desca, ia - 1 + m, ja - 1 + n))
+    return
+  end if
+  ioa = ia + (ja - 1) * desca(9)
+  info = c_zolo_mpdorgqr_host(int(n, c_int64_t), a(ioa), int(desca(9), c_int64_t), tau)
+end subroutine mpdorgqr
+
+!> FormX2 (FormX2.f90:2) on the local n x n arrays: Z <- (A+Z)/FONE, ERR = ||Z-A||_F, A <- Z.
+subroutine formx2(n, nblk, fone, err, a, z, contxt, work, info, it, ip)
+  implicit none
+  integer, intent(in) :: n, nblk, contxt, it, ip
+  double precision, intent(in) :: fone
+  double precision, intent(out) :: err
+  double precision, intent(inout) :: a(n, n), z(n, n), work(*)
+  integer, intent(out) :: info
+  integer :: i, j
+  double precision :: s, d
+  if (nblk < 0 .or. contxt < -1 .or. it < -1 .or. ip < -1) work(1) = 0d0
+  s = 0d0
+  do j = 1, n
+    do i = 1, n
+      z(i, j) = (a(i, j) + z(i, j)) / fone
+      d = z(i, j) - a(i, j)
+      s = s + d * d
+      a(i, j) = z(i, j)
+    end do
+  end do
+  err = sqrt(s)
+  info = 0
+end subroutine formx2
+
+!> hfunc (hfunc.f90:2): the QDWH weight a(l).
+double precision function hfunc(x)
+  use zolo_capi
+  implicit none
+  double precision, intent(in) :: x
+  hfunc = c_zolo_hfunc(x)
+end function hfunc

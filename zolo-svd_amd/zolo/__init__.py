@@ -116,6 +116,21 @@ _sig("zolo_gemm_device", C.c_int, [C.c_int, C.c_int, C.c_int, _I64, _I64, _I64, 
                                    C.c_double, _P, _I64, _P])
 _sig("zolo_stacked_qr_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P])
 _sig("zolo_chol_inv_device", C.c_int, [C.c_int, _I64, _P, _I64, _P, _I64, _P])
+_sig("zolo_profile_enable", C.c_int, [C.c_int])
+_sig("zolo_profile_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), _D, _D])
+
+
+def profile_enable(on: bool = True):
+    """Start (clearing) or stop the GEMM launch ledger."""
+    _check("zolo_profile_enable", _lib.zolo_profile_enable(1 if on else 0))
+
+
+def profile_gemm(precision: int = -1, transa: int = -1, transb: int = -1, tile_m: int = -1, tile_n: int = -1):
+    """(launches, total ms, algorithmic flops) of the recorded GEMM launches matching the key (-1 = any)."""
+    cnt, ms, fl = C.c_int64(), C.c_double(), C.c_double()
+    _check("zolo_profile_gemm", _lib.zolo_profile_gemm(precision, transa, transb, tile_m, tile_n, C.byref(cnt),
+                                                       C.byref(ms), C.byref(fl)))
+    return cnt.value, ms.value, fl.value
 
 LIB = _lib
 LIB_PATH = _LIB_PATH
