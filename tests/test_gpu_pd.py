@@ -65,7 +65,9 @@ def test_pqdwhfacs_matches_oracle(zl, n, kappa, seed):
     sched = "".join("Q" if x == "QR" else "C" for x in lo.schedule)
     assert info.schedule == sched
     assert abs(info.alpha - lo.alpha) <= 1e-13 * lo.alpha
-    assert abs(info.beta - lo.beta) <= 1e-8 * lo.beta
+    # the 1-norm estimate runs on a different LU (blocked GPU vs LAPACK): it
+    # agrees to about kappa * eps
+    assert abs(info.beta - lo.beta) <= max(1e-8, 1e-15 * kappa) * lo.beta
     be, orth = _errs(A, U, H)
     beo, ortho = _errs(A, Uo, Ho)
     assert be <= 10 * max(beo, 1e-15) and orth <= 10 * max(ortho, 1e-15)
@@ -79,7 +81,7 @@ def test_bounds(torch_gpu, zl, n, conv):
     a, b = zl.bounds_device(dA, conv)
     ao, bo = zo.bounds_estimate(A, conv)
     assert abs(a - ao) <= 1e-13 * ao
-    assert abs(b - bo) <= 1e-8 * bo, (b, bo)
+    assert abs(b - bo) <= max(1e-8, 1e-15 * 1e7) * bo, (b, bo)
 
 
 @pytest.mark.parametrize("m,n", [(100, 100), (300, 120), (77, 200)])

@@ -31,17 +31,10 @@ __device__ inline bool piv_better(double v, int64_t i, double bv, int64_t bi) {
   return v > bv || (v == bv && i < bi);
 }
 
-// Argmax of |A[i][j]| over rows [j, n) restricted to this workgroup's rows.
-__device__ inline void argmax_block(const double* A, int64_t lda, int64_t n, int64_t j, int64_t r0, int64_t r1,
-                                    Piv* out) {
+// Block-wide argmax of (|value|, row) pairs, ties to the smaller row (as IDAMAX).
+__device__ inline void argmax_reduce(double bv, int64_t bi, Piv* out) {
   __shared__ double sv[kT];
   __shared__ int64_t si[kT];
-  double bv = -1.0;
-  int64_t bi = INT64_MAX;
-  for (int64_t i = max(r0, j) + threadIdx.x; i < r1; i += kT) {
-    const double v = fabs(A[i + j * lda]);
-    if (piv_better(v, i, bv, bi)) { bv = v; bi = i; }
-  }
   sv[threadIdx.x] = bv;
   si[threadIdx.x] = bi;
   __syncthreads();
@@ -55,11 +48,14 @@ __device__ inline void argmax_block(const double* A, int64_t lda, int64_t n, int
   if (threadIdx.x == 0) { out->v = sv[0]; out->i = si[0]; }
 }
 
+// One thread per row: argmax of |A[i][j]| over rows i >= j.
 __global__ __launch_bounds__(kT) void lu_argmax_kernel(const double* A, int64_t lda, int64_t n, int64_t j, int nwg,
                                                        Piv* part) {
-  const int64_t rpw = (n + nwg - 1) / nwg;
-  const int64_t r0 = blockIdx.x * rpw, r1 = min(n, r0 + rpw);
-  argmax_block(A, lda, n, j, r0, r1, &part[blockIdx.x]);
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  double bv = -1.0;
+  int64_t bi = INT64_MAX;
+  if (i >= j && i < n) { bv = fabs(A[i + j * lda]); bi = i; }
+  argmax_reduce(bv, bi, &part[blockIdx.x]);
 }
 
 // Reduce the partials, swap rows j <-> piv over all n columns, record ipiv.
@@ -88,28 +84,36 @@ __global__ __launch_bounds__(kT) void lu_swap_kernel(double* A, int64_t lda, int
   }
 }
 
-// Rows i > j: l = A[i][j]/A[j][j]; A[i][c] -= l*A[j][c] for c in (j, pe); then
-// argmax partial of column j+1 (if j+1 < pe).
+// One thread per row i > j: l = A[i][j]/A[j][j]; A[i][c] -= l*A[j][c] for c in
+// (j, pe), with the row held in registers; then the argmax partial of column
+// j+1 from the updated values.
 __global__ __launch_bounds__(kT) void lu_update_kernel(double* A, int64_t lda, int64_t n, int64_t j, int64_t pe,
                                                        int nwg, Piv* part) {
   __shared__ double prow[kLuNb];
-  const int64_t rpw = (n + nwg - 1) / nwg;
-  const int64_t r0 = blockIdx.x * rpw, r1 = min(n, r0 + rpw);
   const int nc = (int)(pe - j - 1);
   if ((int)threadIdx.x < nc) prow[threadIdx.x] = A[j + (j + 1 + threadIdx.x) * lda];
   __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   const double d = A[j + j * lda];
-  if (d != 0.0) {
-    const double inv = 1.0 / d;
-    for (int64_t i = max(r0, j + 1) + threadIdx.x; i < r1; i += kT) {
-      const double l = A[i + j * lda] * inv;
-      A[i + j * lda] = l;
-      for (int c = 0; c < nc; ++c) A[i + (j + 1 + c) * lda] -= l * prow[c];
-    }
+  const bool live = i > j && i < n;
+  double v[kLuNb];
+#pragma unroll
+  for (int c = 0; c < kLuNb; ++c) v[c] = (live && c < nc) ? A[i + (j + 1 + c) * lda] : 0.0;
+  if (live && d != 0.0) {
+    const double l = A[i + j * lda] / d;
+    A[i + j * lda] = l;
+#pragma unroll
+    for (int c = 0; c < kLuNb; ++c)
+      if (c < nc) v[c] -= l * prow[c];
+#pragma unroll
+    for (int c = 0; c < kLuNb; ++c)
+      if (c < nc) A[i + (j + 1 + c) * lda] = v[c];
   }
   if (j + 1 < pe) {
-    __syncthreads();
-    argmax_block(A, lda, n, j + 1, r0, r1, &part[blockIdx.x]);
+    double bv = -1.0;
+    int64_t bi = INT64_MAX;
+    if (live) { bv = fabs(v[0]); bi = i; }
+    argmax_reduce(bv, bi, &part[blockIdx.x]);
   }
 }
 
@@ -226,7 +230,8 @@ inline unsigned grid_for(int64_t total) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
 }
 
-int lu_nwg(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 1023) / 1024)); }
+// One thread per row in the panel kernels.
+int lu_nwg(int64_t n) { return (int)std::max<int64_t>(1, (n + kT - 1) / kT); }
 
 // In-place LU with partial pivoting (row swaps applied to full rows).
 int getrf(const Ctx& c, double* A, int64_t lda, int64_t n, int64_t* ipiv, int* info, Piv* part, double* L11i,
@@ -280,7 +285,7 @@ int trsv(const Ctx& c, const double* M, int64_t ld, int64_t n, int trans, int lo
 
 size_t bounds_bytes(int64_t n) {
   return sizeof(double) * (size_t)(n * n + 2 * n + kLuNb * kLuNb + kLuNb * n + 8) + sizeof(int64_t) * n +
-         sizeof(Piv) * 512 + 16 * 256;
+         sizeof(Piv) * (size_t)(n / kT + 2) + 16 * 256;
 }
 
 int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta, Arena& ar,
@@ -294,7 +299,7 @@ int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* 
   double* W = ar.take<double>((size_t)kLuNb * n);
   double* out = ar.take<double>(8);
   int64_t* ipiv = ar.take<int64_t>(n);
-  Piv* part = ar.take<Piv>(512);
+  Piv* part = ar.take<Piv>((size_t)(n / kT + 2));
   int* dinfo = ar.take<int>(4);
   ZOLO_ARENA_OK(ar);
   // alpha = ||A||_F ; X = A/alpha

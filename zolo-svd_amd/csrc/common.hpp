@@ -62,6 +62,11 @@ struct GemmDesc {
   int batch = 1;
   int aLo = -1, aHi = -1, bLo = -1, bHi = -1;
   int lowerC = 0;
+  // split-K (set by the launcher): blockIdx.y selects a K chunk whose raw
+  // partial product goes to C + blockIdx.y * splitStride
+  int splitk = 1;
+  int64_t kchunk = 0;
+  int64_t splitStride = 0;
 };
 
 enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };

@@ -15,6 +15,7 @@
 // workgroup dispatch) each own a contiguous block of tiles, and inside a block
 // tiles walk down groups of 8 tile-rows so neighbouring workgroups share the
 // same A and B panels in their XCD's L2.
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -131,7 +132,7 @@ __device__ inline void k_range(const GemmDesc& d, int64_t i0, int64_t j0, int64_
 }
 
 // ---------------------------------------------------------------------------
-// F64 / F32 kernel.  Tile TBM x TBN x 16; the four waves form a WM x WN grid,
+// F64 / F32 kernel.  Tile TBM x TBN x 16, LDS double-buffered; the four waves form a WM x WN grid,
 // each wave owning FM x FN blocks of 16x16 MFMA outputs.
 // ---------------------------------------------------------------------------
 template <typename T, int TA, int TB, int TBM, int TBN>
@@ -143,8 +144,8 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
   constexpr int FM = TBM / WM / 16, FN = TBN / WN / 16;
   constexpr int EA = TBM * BK / kThreads;   // A elements per thread
   constexpr int EB = TBN * BK / kThreads;   // B elements per thread
-  __shared__ T As[BK][TBM + PAD];
-  __shared__ T Bs[BK][TBN + PAD];
+  __shared__ T As[2][BK][TBM + PAD];
+  __shared__ T Bs[2][BK][TBN + PAD];
 
   const TileCoord tc = tile_of<TBM, TBN>(d.M, d.N, d.lowerC);
   if (tc.skip) return;
@@ -156,6 +157,12 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
 
   int64_t kb, ke;
   k_range<TBM, TBN>(d, i0, j0, kb, ke);
+  if (d.splitk > 1) {
+    const int64_t y = blockIdx.y;
+    kb = max(kb, y * d.kchunk);
+    ke = min(ke, (y + 1) * d.kchunk);
+    C += y * d.splitStride;
+  }
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -210,27 +217,30 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
       }
     }
   };
-  auto store_tiles = [&]() {
+  auto store_tiles = [&](int b) {
     if (TA == 0) {
 #pragma unroll
-      for (int e = 0; e < EA; ++e) As[a_r][a_c + e] = ra[e];
+      for (int e = 0; e < EA; ++e) As[b][a_r][a_c + e] = ra[e];
     } else {
 #pragma unroll
-      for (int e = 0; e < EA; ++e) As[a_c + e][a_r] = ra[e];
+      for (int e = 0; e < EA; ++e) As[b][a_c + e][a_r] = ra[e];
     }
     if (TB == 0) {
 #pragma unroll
-      for (int e = 0; e < EB; ++e) Bs[b_c + e][b_r] = rb[e];
+      for (int e = 0; e < EB; ++e) Bs[b][b_c + e][b_r] = rb[e];
     } else {
 #pragma unroll
-      for (int e = 0; e < EB; ++e) Bs[b_r][b_c + e] = rb[e];
+      for (int e = 0; e < EB; ++e) Bs[b][b_r][b_c + e] = rb[e];
     }
   };
 
   if (kb < ke) {
+    // Two LDS stages: tile k+1 is staged into the other buffer while tile k
+    // feeds the MFMAs, so each K step needs a single barrier.
     load_tiles(kb);
-    store_tiles();
+    store_tiles(0);
     __syncthreads();
+    int buf = 0;
     for (int64_t k0 = kb; k0 < ke; k0 += BK) {
       const bool more = (k0 + BK) < ke;
       if (more) load_tiles(k0 + BK);
@@ -239,19 +249,17 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
         const int kr = ks + (lane >> 4);
         T av[FM], bv[FN];
 #pragma unroll
-        for (int m = 0; m < FM; ++m) av[m] = As[kr][wm * (FM * 16) + m * 16 + (lane & 15)];
+        for (int m = 0; m < FM; ++m) av[m] = As[buf][kr][wm * (FM * 16) + m * 16 + (lane & 15)];
 #pragma unroll
-        for (int n = 0; n < FN; ++n) bv[n] = Bs[kr][wn * (FN * 16) + n * 16 + (lane & 15)];
+        for (int n = 0; n < FN; ++n) bv[n] = Bs[buf][kr][wn * (FN * 16) + n * 16 + (lane & 15)];
 #pragma unroll
         for (int m = 0; m < FM; ++m)
 #pragma unroll
           for (int n = 0; n < FN; ++n) acc[m][n] = mfma16(av[m], bv[n], acc[m][n]);
       }
+      if (more) store_tiles(buf ^ 1);
       __syncthreads();
-      if (more) {
-        store_tiles();
-        __syncthreads();
-      }
+      buf ^= 1;
     }
   }
 
@@ -440,6 +448,42 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmDesc d) {
   }
 }
 
+// C = alpha_z * sum_y P[z][y] + beta * C  (split-K reduction, fixed order)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void splitk_reduce_kernel(GemmDesc d, const T* P) {
+  const int64_t z = blockIdx.y;
+  const int64_t mn = d.M * d.N;
+  const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
+  T* C = reinterpret_cast<T*>(d.C) + z * d.strideC;
+  const T* Pz = P + z * d.splitk * mn;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < mn; e += (int64_t)gridDim.x * kThreads) {
+    const int64_t i = e % d.M, j = e / d.M;
+    double s = 0.0;
+    for (int y = 0; y < d.splitk; ++y) s += (double)Pz[y * mn + e];
+    T* p = C + i + j * d.ldc;
+    const double v = alpha * s;
+    *p = (d.beta == 0.0) ? (T)v : (T)(v + d.beta * (double)(*p));
+  }
+}
+
+void* g_splitk_ws = nullptr;
+size_t g_splitk_cap = 0;
+
+int splitk_workspace(size_t bytes, void** p) {
+  if (bytes > g_splitk_cap) {
+    if (g_splitk_ws) {
+      ZOLO_HIP(hipDeviceSynchronize());
+      ZOLO_HIP(hipFree(g_splitk_ws));
+    }
+    g_splitk_ws = nullptr;
+    g_splitk_cap = 0;
+    ZOLO_HIP(hipMalloc(&g_splitk_ws, bytes));
+    g_splitk_cap = bytes;
+  }
+  *p = g_splitk_ws;
+  return kOk;
+}
+
 struct LedgerScope {
   bool on;
   LedgerEntry e;
@@ -459,15 +503,46 @@ struct LedgerScope {
 };
 
 template <typename T, int TBM, int TBN>
-int launch_tile(const GemmDesc& d, hipStream_t s) {
-  const int64_t tiles = cdiv(d.M, TBM) * cdiv(d.N, TBN);
-  dim3 grid((unsigned)tiles, 1, (unsigned)d.batch);
-  LedgerScope ls(d, sizeof(T) == 8 ? 0 : 1, TBM, TBN, s);
+int launch_tile(const GemmDesc& d0, hipStream_t s) {
+  const int64_t tiles = cdiv(d0.M, TBM) * cdiv(d0.N, TBN);
+  LedgerScope ls(d0, sizeof(T) == 8 ? 0 : 1, TBM, TBN, s);
+  // Few output tiles over a long K (the width-64 V^T C products of the QR):
+  // split K so the chip has enough workgroups, then reduce in a fixed order.
+  GemmDesc d = d0;
+  int split = 1;
+  const int64_t wgs = tiles * d0.batch;
+  if (!d0.lowerC && wgs < 512 && d0.K >= 1024) {
+    split = (int)std::min<int64_t>(std::min<int64_t>(32, cdiv(1024, wgs)), d0.K / 512);
+    if (split < 2) split = 1;
+  }
+  if (split > 1) {
+    void* ws = nullptr;
+    ZOLO_CHECK(splitk_workspace(sizeof(T) * (size_t)split * d0.batch * d0.M * d0.N, &ws));
+    d.splitk = split;
+    d.kchunk = cdiv(cdiv(d0.K, split), 32) * 32;
+    d.splitStride = d0.M * d0.N;
+    d.C = ws;
+    d.ldc = d0.M;
+    d.strideC = (int64_t)split * d0.M * d0.N;
+    d.alpha = 1.0;
+    d.alphaVec = nullptr;
+    d.beta = 0.0;
+  }
+  dim3 grid((unsigned)tiles, (unsigned)split, (unsigned)d.batch);
   if (d.transA == 0 && d.transB == 0) hipLaunchKernelGGL((gemm_kernel<T, 0, 0, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
   else if (d.transA == 0) hipLaunchKernelGGL((gemm_kernel<T, 0, 1, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
   else if (d.transB == 0) hipLaunchKernelGGL((gemm_kernel<T, 1, 0, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
   else hipLaunchKernelGGL((gemm_kernel<T, 1, 1, TBM, TBN>), grid, dim3(kThreads), 0, s, d);
   ZOLO_HIP(hipGetLastError());
+  if (split > 1) {
+    GemmDesc r = d0;
+    r.splitk = split;
+    const int64_t mn = d0.M * d0.N;
+    const unsigned gx = (unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, cdiv(mn, kThreads)));
+    hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(gx, d0.batch), dim3(kThreads), 0, s, r,
+                       reinterpret_cast<const T*>(d.C));
+    ZOLO_HIP(hipGetLastError());
+  }
   return kOk;
 }
 

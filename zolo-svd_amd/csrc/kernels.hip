@@ -149,6 +149,15 @@ __global__ __launch_bounds__(kT) void shift_copy_kernel(const T* W, int64_t ldw,
   }
 }
 
+// A = value (m x n, batched over blockIdx.y).
+template <typename T>
+__global__ __launch_bounds__(kT) void fill_kernel(T* A, int64_t lda, int64_t strideA, int64_t m, int64_t n, T value) {
+  T* D = A + blockIdx.y * strideA;
+  const int64_t total = m * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT)
+    D[(e % m) + (e / m) * lda] = value;
+}
+
 // Zero the strict upper triangle (batched).
 template <typename T>
 __global__ __launch_bounds__(kT) void zero_upper_kernel(T* A, int64_t lda, int64_t strideA, int64_t n) {
@@ -317,6 +326,13 @@ int k_shift_copy(const T* W, int64_t ldw, T* Wq, int64_t ldq, int64_t strideQ, i
 }
 
 template <typename T>
+int k_fill(T* A, int64_t lda, int64_t strideA, int64_t m, int64_t n, double value, int batch, hipStream_t s) {
+  hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(m * n), batch), dim3(kT), 0, s, A, lda, strideA, m, n, (T)value);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+template <typename T>
 int k_zero_upper(T* A, int64_t lda, int64_t strideA, int64_t n, int batch, hipStream_t s) {
   hipLaunchKernelGGL((zero_upper_kernel<T>), dim3(grid_for(n * n), batch), dim3(kT), 0, s, A, lda, strideA, n);
   ZOLO_HIP(hipGetLastError());
@@ -361,6 +377,7 @@ int k_chol_leaf(T* A, int64_t lda, int64_t strideA, T* Li, int64_t ldl, int64_t 
   template int k_shift_copy<T>(const T*, int64_t, T*, int64_t, int64_t, int64_t, const double*, double, double, int,    \
                                hipStream_t);                                                                            \
   template int k_zero_upper<T>(T*, int64_t, int64_t, int64_t, int, hipStream_t);                                        \
+  template int k_fill<T>(T*, int64_t, int64_t, int64_t, int64_t, double, int, hipStream_t);                             \
   template int k_symmetrize<T>(T*, int64_t, int64_t, hipStream_t);                                                      \
   template int k_add_diag<T>(T*, int64_t, int64_t, double, hipStream_t);                                                \
   template int k_chol_leaf<T>(T*, int64_t, int64_t, T*, int64_t, int64_t, int, int64_t, int*, int, hipStream_t);

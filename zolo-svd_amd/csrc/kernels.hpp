@@ -25,6 +25,8 @@ template <typename T>
 int k_shift_copy(const T* W, int64_t ldw, T* Wq, int64_t ldq, int64_t strideQ, int64_t n, const double* shift,
                  double scale, double add, int batch, hipStream_t s);
 template <typename T>
+int k_fill(T* A, int64_t lda, int64_t strideA, int64_t m, int64_t n, double value, int batch, hipStream_t s);
+template <typename T>
 int k_zero_upper(T* A, int64_t lda, int64_t strideA, int64_t n, int batch, hipStream_t s);
 template <typename T>
 int k_symmetrize(T* A, int64_t lda, int64_t n, hipStream_t s);

@@ -184,139 +184,8 @@ int chol_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const dou
 // ---------------------------------------------------------------------------
 // Structured Householder QR of [X; s I] and the QR-based step.
 // ---------------------------------------------------------------------------
-static int qr_nwg(int64_t mrows) {
-  int64_t w = (mrows + 255) / 256;
-  if (w > 256) w = 256;
-  return (int)std::max<int64_t>(w, 1);
-}
-
-template <typename T>
-static size_t stacked_qr_bytes(int64_t n, int r) {
-  const int nb = kQrNb;
-  const int64_t mr = n + nb;
-  const int nwg = qr_nwg(mr);
-  const int64_t npan = (n + nb - 1) / nb;
-  size_t b = 0;
-  b += sizeof(T) * (size_t)r * mr * nb;            // V
-  b += sizeof(T) * (size_t)r * nb * nb;            // G
-  b += sizeof(T) * (size_t)r * npan * nb * nb;     // T per panel
-  b += 2 * sizeof(T) * (size_t)r * nb * n;         // W, W2
-  b += sizeof(double) * (size_t)r * nb * ((size_t)nwg * nb + nb);  // panel partials
-  b += sizeof(T) * (size_t)r * n;                  // tau
-  return b + 8 * 256;
-}
-
-template <typename T>
-int stacked_geqrf(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, Arena& ar) {
-  const int nb = kQrNb;
-  const int64_t mr = n + nb;
-  const int nwg_max = qr_nwg(mr);
-  const int64_t npan = (n + nb - 1) / nb;
-  T* V = ar.take<T>((size_t)batch * mr * nb);
-  T* G = ar.take<T>((size_t)batch * nb * nb);
-  T* Tp = ar.take<T>((size_t)batch * nb * nb);
-  T* W = ar.take<T>((size_t)batch * nb * n);
-  T* W2 = ar.take<T>((size_t)batch * nb * n);
-  double* part = ar.take<double>((size_t)batch * nb * ((size_t)nwg_max * nb + nb));
-  ZOLO_ARENA_OK(ar);
-  const int64_t sV = mr * nb, sG = (int64_t)nb * nb, sW = (int64_t)nb * n;
-  for (int64_t pi = 0; pi < npan; ++pi) {
-    const int64_t p = pi * nb;
-    const int jb = (int)std::min<int64_t>(nb, n - p);
-    const int64_t mrows = n + jb;
-    const int nwg = qr_nwg(mrows);
-    for (int64_t j = p; j <= p + jb; ++j)
-      ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, j, j > p, j < p + jb, part, nwg, tau, n, batch, c.s));
-    const int64_t ncols = n - p - jb;
-    if (ncols <= 0) continue;
-    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, p, jb, mrows, batch, c.s));
-    GemmDesc d; d.batch = batch;           // G = V^T V
-    d.transA = 1; d.transB = 0; d.M = jb; d.N = jb; d.K = mrows;
-    d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
-    d.C = G; d.ldc = jb; d.strideC = sG;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-    ZOLO_CHECK(k_qr_form_t<T>(G, jb, sG, tau, n, p, jb, Tp, jb, sG, batch, c.s));
-    T* Cm = S + p + (p + jb) * lds;
-    d = GemmDesc(); d.batch = batch;       // W = V^T C
-    d.transA = 1; d.transB = 0; d.M = jb; d.N = ncols; d.K = mrows;
-    d.A = V; d.lda = mrows; d.strideA = sV; d.B = Cm; d.ldb = lds; d.strideB = sS;
-    d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-    d = GemmDesc(); d.batch = batch;       // W2 = T^T W
-    d.transA = 1; d.transB = 0; d.M = jb; d.N = ncols; d.K = jb;
-    d.A = Tp; d.lda = jb; d.strideA = sG; d.B = W; d.ldb = jb; d.strideB = sW;
-    d.C = W2; d.ldc = jb; d.strideC = sW;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-    d = GemmDesc(); d.batch = batch;       // C -= V W2
-    d.transA = 0; d.transB = 0; d.M = mrows; d.N = ncols; d.K = jb;
-    d.A = V; d.lda = mrows; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
-    d.C = Cm; d.ldc = lds; d.strideC = sS; d.alpha = -1.0; d.beta = 1.0;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-  }
-  return kOk;
-}
-
-template <typename T>
-int stacked_orgqr(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, T* Q, int64_t ldq, int64_t sQ,
-                  int64_t n, int batch, Arena& ar) {
-  const int nb = kQrNb;
-  const int64_t mr = n + nb;
-  const int64_t npan = (n + nb - 1) / nb;
-  T* V = ar.take<T>((size_t)batch * mr * nb);
-  T* G = ar.take<T>((size_t)batch * nb * nb);
-  T* Tp = ar.take<T>((size_t)batch * nb * nb);
-  T* W = ar.take<T>((size_t)batch * nb * n);
-  T* W2 = ar.take<T>((size_t)batch * nb * n);
-  ZOLO_ARENA_OK(ar);
-  const int64_t sV = mr * nb, sG = (int64_t)nb * nb, sW = (int64_t)nb * n;
-  // Explicit Q = H_0 ... H_{n-1} [I; 0], panels applied last to first; the
-  // block reflector T of each panel is re-formed from V^T V and tau.
-  ZOLO_CHECK(k_set_identity_top<T>(Q, ldq, sQ, 2 * n, n, batch, c.s));
-  for (int64_t pi = npan - 1; pi >= 0; --pi) {
-    const int64_t p = pi * nb;
-    const int jb = (int)std::min<int64_t>(nb, n - p);
-    const int64_t mrows = n + jb;
-    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, p, jb, mrows, batch, c.s));
-    GemmDesc d; d.batch = batch;           // G = V^T V
-    d.transA = 1; d.transB = 0; d.M = jb; d.N = jb; d.K = mrows;
-    d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
-    d.C = G; d.ldc = jb; d.strideC = sG;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-    ZOLO_CHECK(k_qr_form_t<T>(G, jb, sG, tau, n, p, jb, Tp, jb, sG, batch, c.s));
-    T* Qs = Q + p + p * ldq;
-    const int64_t nc = n - p;
-    d = GemmDesc(); d.batch = batch;       // W = V^T Qs
-    d.transA = 1; d.transB = 0; d.M = jb; d.N = nc; d.K = mrows;
-    d.A = V; d.lda = mrows; d.strideA = sV; d.B = Qs; d.ldb = ldq; d.strideB = sQ;
-    d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-    d = GemmDesc(); d.batch = batch;       // W2 = T W
-    d.transA = 0; d.transB = 0; d.M = jb; d.N = nc; d.K = jb;
-    d.A = Tp; d.lda = jb; d.strideA = sG; d.B = W; d.ldb = jb; d.strideB = sW;
-    d.C = W2; d.ldc = jb; d.strideC = sW;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-    d = GemmDesc(); d.batch = batch;       // Qs -= V W2
-    d.transA = 0; d.transB = 0; d.M = mrows; d.N = nc; d.K = jb;
-    d.A = V; d.lda = mrows; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
-    d.C = Qs; d.ldc = ldq; d.strideC = sQ; d.alpha = -1.0; d.beta = 1.0;
-    ZOLO_CHECK(gemm_t<T>(c, d));
-  }
-  return kOk;
-}
-
-template <typename T>
-int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
-               T* tau_out, Arena& ar) {
-  T* tau = tau_out ? tau_out : ar.take<T>((size_t)batch * n);
-  ZOLO_ARENA_OK(ar);
-  const size_t mark = ar.used();
-  ZOLO_CHECK(stacked_geqrf<T>(c, S, lds, sS, n, batch, tau, ar));
-  if (!Q) return kOk;
-  ar.reset();  // ORGQR reuses the GEQRF scratch
-  ar.take<char>(mark);
-  return stacked_orgqr<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, ar);
-}
-
+// QR-based step (the structured QR itself is in qr_blocked.cpp).
+// ---------------------------------------------------------------------------
 template <typename T>
 size_t qr_step_bytes(int64_t n, int r) {
   return sizeof(T) * (size_t)4 * r * n * n + stacked_qr_bytes<T>(n, r) + 4096;
@@ -331,13 +200,14 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
   ZOLO_ARENA_OK(ar);
   ZOLO_CHECK(k_build_stack<T>(X, ldx, S, ls, ss, n, d_sdiag, r, c.s));
   ZOLO_CHECK(stacked_qr<T>(c, S, ls, ss, Q, ls, ss, n, r, nullptr, ar));
-  // Z_q = alpha_q * Q1 Q2^T ; Q2 is zero below its (kQrNb-1)-th subdiagonal.
+  // Z_q = alpha_q * Q1 Q2^T ; Q2 is exactly zero below its 255th subdiagonal
+  // (the block reflectors applied to Q are 256 columns wide).
   GemmDesc d; d.batch = r;
   d.transA = 0; d.transB = 1; d.M = n; d.N = n; d.K = n;
   d.A = Q; d.lda = ls; d.strideA = ss;
   d.B = Q + n; d.ldb = ls; d.strideB = ss;
   d.C = Z; d.ldc = ldz; d.strideC = sZ;
-  d.alpha = 1.0; d.alphaVec = d_alpha; d.beta = 0.0; d.bLo = kQrNb;
+  d.alpha = 1.0; d.alphaVec = d_alpha; d.beta = 0.0; d.bLo = 256;
   ZOLO_CHECK(gemm_t<T>(c, d));
   return kOk;
 }
@@ -350,11 +220,8 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
                             const double*, T*, int64_t, int64_t, Arena&, int*, bool);                                \
   template size_t qr_step_bytes<T>(int64_t, int);                                                                \
   template int qr_step<T>(const Ctx&, const T*, int64_t, int64_t, int, const double*, const double*, T*, int64_t, \
-                          int64_t, Arena&);                                                                      \
-  template int stacked_qr<T>(const Ctx&, T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, T*, Arena&);   \
-  template int stacked_geqrf<T>(const Ctx&, T*, int64_t, int64_t, int64_t, int, T*, Arena&);                      \
-  template int stacked_orgqr<T>(const Ctx&, const T*, int64_t, int64_t, const T*, T*, int64_t, int64_t, int64_t, \
-                                int, Arena&);
+                          int64_t, Arena&);
+
 ZOLO_LA_INST(double)
 ZOLO_LA_INST(float)
 

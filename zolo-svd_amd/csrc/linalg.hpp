@@ -71,6 +71,9 @@ template <typename T>
 int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const double* d_sdiag, const double* d_alpha,
             T* Z, int64_t ldz, int64_t sZ, Arena& ar);
 
+template <typename T>
+size_t stacked_qr_bytes(int64_t n, int r);
+
 // Structured GEQRF (R + reflectors in place, tau out) and ORGQR (explicit
 // 2n x n Q from the reflectors) of a batch of 2n x n stacks; stacked_qr does both.
 template <typename T>

@@ -25,6 +25,9 @@ __device__ inline double wsum(double v) {
 
 // part layout per shift: for local column jj, a record of (nwg*jb + jb) doubles:
 // [0, nwg*jb): per-workgroup Gram partials, [nwg*jb, nwg*jb+jb): pivot row.
+// Each thread owns one row of the panel and keeps its jb values in registers
+// for the whole launch: one batch of loads, the rank-1 update and the Gram
+// products in registers, one batch of stores.
 template <typename T>
 __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb,
                                                       int64_t j, int apply_prev, int reduce_cur, double* part,
@@ -37,14 +40,17 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
   const int64_t rec = (int64_t)nwg * jb + jb;
   double* base = part + (int64_t)q * jb * rec;
   const int t = threadIdx.x, w = blockIdx.x;
-  const int64_t NR = n + jb;
-  const int64_t rpw = (NR + nwg - 1) / nwg;
-  const int64_t r0 = p + w * rpw;
-  const int64_t r1 = min(p + NR, r0 + rpw);
+  const int64_t i = p + (int64_t)w * kT + t;           // the row this thread owns
+  const int64_t jp = j - 1;
+  const int jjp = (int)(jp - p), jj = (int)(j - p);
+  // rows beyond n + j take no part in either reflector of this launch
+  const bool live = i < p + n + jb && i <= n + j && i >= (apply_prev ? jp : j);
+
+  double v[NBP];
+#pragma unroll
+  for (int c = 0; c < NBP; ++c) v[c] = (live && c < jb) ? (double)Sq[i + (p + c) * lds] : 0.0;
 
   if (apply_prev) {
-    const int64_t jp = j - 1;
-    const int jjp = (int)(jp - p);
     const double* G = base + (int64_t)jjp * rec;
     if (t < jb) {
       double s = 0.0;
@@ -68,47 +74,44 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
     __syncthreads();
     if (t < jb) wv[t] = (t > jjp) ? sc[1] * (rr[t] + sc[2] * gs[t]) : 0.0;
     __syncthreads();
-    const double scal = sc[2];
-    for (int64_t i = r0 + t; i < r1; i += kT) {
-      if (i == jp) {
-        Sq[jp + jp * lds] = (T)sc[0];
-        for (int c = jjp + 1; c < jb; ++c) {
-          T* e = Sq + jp + (p + c) * lds;
-          *e = (T)((double)*e - wv[c]);
-        }
-      } else if (i > jp && i <= n + jp) {
-        T* ej = Sq + i + jp * lds;
-        const double vi = (double)*ej * scal;
-        *ej = (T)vi;
-        for (int c = jjp + 1; c < jb; ++c) {
-          T* e = Sq + i + (p + c) * lds;
-          *e = (T)((double)*e - vi * wv[c]);
-        }
+    if (live && i <= n + jp) {
+      // v[jjp] without a dynamic register index (which would spill v to scratch)
+      double vj = 0.0;
+#pragma unroll
+      for (int c = 0; c < NBP; ++c)
+        if (c == jjp) vj = v[c];
+      const double vi = (i == jp) ? 1.0 : vj * sc[2];
+      const double newj = (i == jp) ? sc[0] : vi;
+#pragma unroll
+      for (int c = 0; c < NBP; ++c) {
+        if (c == jjp) v[c] = newj;
+        if (c > jjp && c < jb) v[c] -= vi * wv[c];
       }
+#pragma unroll
+      for (int c = 0; c < NBP; ++c)
+        if (c >= jjp && c < jb) Sq[i + (p + c) * lds] = (T)v[c];
     }
   }
 
   if (reduce_cur) {
-    const int jj = (int)(j - p);
     double* G = base + (int64_t)jj * rec;
-    double acc[NBP];
+    if (live && i == j) {
 #pragma unroll
-    for (int c = 0; c < NBP; ++c) acc[c] = 0.0;
-    for (int64_t i = r0 + t; i < r1; i += kT) {
-      if (i == j) {
-        for (int c = jj; c < jb; ++c) G[(int64_t)nwg * jb + c] = (double)Sq[j + (p + c) * lds];
-      } else if (i > j && i <= n + j) {
-        const double x = (double)Sq[i + j * lds];
-#pragma unroll
-        for (int c = 0; c < NBP; ++c)
-          if (c >= jj && c < jb) acc[c] += x * (double)Sq[i + (p + c) * lds];
-      }
+      for (int c = 0; c < NBP; ++c)
+        if (c >= jj && c < jb) G[(int64_t)nwg * jb + c] = v[c];
     }
+    double x = 0.0;
+#pragma unroll
+    for (int c = 0; c < NBP; ++c)
+      if (c == jj) x = v[c];
+    if (!(live && i > j)) x = 0.0;
     const int lane = t & 63, wave = t >> 6;
 #pragma unroll
     for (int c = 0; c < NBP; ++c) {
-      const double v = wsum(acc[c]);
-      if (lane == 0) red[wave][c] = v;
+      if (c < jb) {  // uniform
+        const double r = wsum((c >= jj) ? x * v[c] : 0.0);
+        if (lane == 0) red[wave][c] = r;
+      }
     }
     __syncthreads();
     if (t < jb) G[(int64_t)w * jb + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];

@@ -1,0 +1,262 @@
+// Two-level blocked structured Householder QR of the stacked matrix [X; s I]
+// (mpdgeqrf.f:262-313, mpdorgqr.f:253-337), batched over the Zolotarev shifts.
+//
+// Inner level: panels of kQrNb = 64 columns are factored column by column
+// (qr.hip) and update only the rest of their 256-wide outer block.  Outer
+// level: the block reflector of all 256 columns, I - V T V^T, is assembled
+// from the inner T factors (T_AB = -T_AA (V_A^T V_B) T_BB for adjacent column
+// groups) and applied to the trailing matrix, and later to Q, as GEMMs with
+// K = 256.  Every block reflector only touches rows p .. n+p+jb-1: rows below
+// are still zero in the identity block of the stack.
+#include <algorithm>
+
+#include "linalg.hpp"
+
+namespace zolo {
+namespace {
+
+constexpr int kQrOuter = 256;
+
+template <typename T>
+int gemm_t(const Ctx& c, const GemmDesc& d) {
+  return gemm(prec_for<T>(c), d, c.s);
+}
+
+// One thread per panel row (the panel kernel keeps each row in registers).
+int qr_nwg(int64_t mrows) { return (int)std::max<int64_t>((mrows + 255) / 256, 1); }
+
+// T of the outer block from its diagonal 64-blocks (already in Tb, ld
+// kQrOuter) and G = V^T V: for adjacent column groups A | B,
+// T_AB = -T_AA G_AB T_BB (the upper-triangular T of H_A H_B).
+template <typename T>
+int combine_t(const Ctx& c, T* Tb, int64_t sT, const T* G, int64_t sG, T* tmp, int64_t sTmp, int lo, int hi,
+              int batch) {
+  if (hi - lo <= kQrNb) return kOk;
+  int mid = lo + ((hi - lo) / 2 + kQrNb - 1) / kQrNb * kQrNb;
+  if (mid >= hi) mid = hi - kQrNb;
+  ZOLO_CHECK(combine_t<T>(c, Tb, sT, G, sG, tmp, sTmp, lo, mid, batch));
+  ZOLO_CHECK(combine_t<T>(c, Tb, sT, G, sG, tmp, sTmp, mid, hi, batch));
+  const int64_t ld = kQrOuter;
+  const int a = mid - lo, b = hi - mid;
+  GemmDesc d;  // tmp = G_AB T_BB   (T_BB upper: op(B)[k][j] = 0 for k > j)
+  d.batch = batch;
+  d.M = a; d.N = b; d.K = b;
+  d.A = G + lo + (int64_t)mid * ld; d.lda = ld; d.strideA = sG;
+  d.B = Tb + mid + (int64_t)mid * ld; d.ldb = ld; d.strideB = sT;
+  d.C = tmp; d.ldc = a; d.strideC = sTmp; d.bHi = 0;
+  ZOLO_CHECK(gemm_t<T>(c, d));
+  d = GemmDesc();  // T_AB = -T_AA tmp   (T_AA upper: op(A)[i][k] = 0 for k < i)
+  d.batch = batch;
+  d.M = a; d.N = b; d.K = a;
+  d.A = Tb + lo + (int64_t)lo * ld; d.lda = ld; d.strideA = sT;
+  d.B = tmp; d.ldb = a; d.strideB = sTmp;
+  d.C = Tb + lo + (int64_t)mid * ld; d.ldc = ld; d.strideC = sT; d.alpha = -1.0; d.aLo = 0;
+  ZOLO_CHECK(gemm_t<T>(c, d));
+  return kOk;
+}
+
+struct QrScratch {
+  void* V;
+  void* G;
+  void* tmp;
+  void* W;
+  void* W2;
+};
+
+template <typename T>
+int take_scratch(Arena& ar, int64_t n, int batch, QrScratch& q) {
+  const int64_t nbo = kQrOuter, mr = n + nbo;
+  q.V = ar.take<T>((size_t)batch * mr * nbo);
+  q.G = ar.take<T>((size_t)batch * nbo * nbo);
+  q.tmp = ar.take<T>((size_t)batch * nbo * nbo);
+  q.W = ar.take<T>((size_t)batch * nbo * n);
+  q.W2 = ar.take<T>((size_t)batch * nbo * n);
+  ZOLO_ARENA_OK(ar);
+  return kOk;
+}
+
+// Apply H^T = (I - V T V^T)^T (left) to the ncols columns at Cm, rows m.
+template <typename T>
+int apply_block_reflector(const Ctx& c, const T* V, int64_t ldv, int64_t sV, const T* Tm, int64_t ldt, int64_t sT,
+                          int jb, T* Cm, int64_t ldc, int64_t sC, int64_t mrows, int64_t ncols, bool transpose,
+                          T* W, T* W2, int64_t sW, int batch) {
+  GemmDesc d;  // W = V^T C
+  d.batch = batch;
+  d.transA = 1; d.M = jb; d.N = ncols; d.K = mrows;
+  d.A = V; d.lda = ldv; d.strideA = sV; d.B = Cm; d.ldb = ldc; d.strideB = sC;
+  d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
+  ZOLO_CHECK(gemm_t<T>(c, d));
+  d = GemmDesc();  // W2 = T^T W (GEQRF) or T W (ORGQR)
+  d.batch = batch;
+  d.transA = transpose ? 1 : 0; d.M = jb; d.N = ncols; d.K = jb;
+  d.A = Tm; d.lda = ldt; d.strideA = sT; d.B = W; d.ldb = jb; d.strideB = sW;
+  d.C = W2; d.ldc = jb; d.strideC = sW;
+  // op(A) = T^T is lower triangular (zero for k > i), op(A) = T upper (zero for k < i)
+  if (transpose) d.aHi = 0; else d.aLo = 0;
+  ZOLO_CHECK(gemm_t<T>(c, d));
+  d = GemmDesc();  // C -= V W2
+  d.batch = batch;
+  d.M = mrows; d.N = ncols; d.K = jb;
+  d.A = V; d.lda = ldv; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
+  d.C = Cm; d.ldc = ldc; d.strideC = sC; d.alpha = -1.0; d.beta = 1.0;
+  return gemm_t<T>(c, d);
+}
+
+template <typename T>
+int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, T* Tall, Arena& ar) {
+  const int nb = kQrNb, nbo = kQrOuter;
+  QrScratch q;
+  ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
+  const int nwg_max = qr_nwg(n + nb);
+  double* part = ar.take<double>((size_t)batch * nb * ((size_t)nwg_max * nb + nb));
+  ZOLO_ARENA_OK(ar);
+  T* V = (T*)q.V;
+  T* G = (T*)q.G;
+  const int64_t mr = n + nbo;
+  const int64_t sV = mr * nbo, sG = (int64_t)nbo * nbo, sW = (int64_t)nbo * n;
+  const int64_t nout = (n + nbo - 1) / nbo;
+  const int64_t sTall = nout * sG;
+  for (int64_t P = 0; P < n; P += nbo) {
+    const int JB = (int)std::min<int64_t>(nbo, n - P);
+    T* Tb = Tall + (P / nbo) * sG;
+    ZOLO_CHECK(k_fill<T>(Tb, nbo, sTall, nbo, nbo, 0.0, batch, c.s));
+    for (int64_t p = P; p < P + JB; p += nb) {
+      const int jb = (int)std::min<int64_t>(nb, P + JB - p);
+      const int64_t mrows = n + jb;
+      const int nwg = qr_nwg(mrows);
+      for (int64_t j = p; j <= p + jb; ++j)
+        ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, j, j > p, j < p + jb, part, nwg, tau, n, batch, c.s));
+      ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, p, jb, mrows, batch, c.s));
+      GemmDesc d;  // G = V^T V -> T of the panel, stored on the diagonal of the outer T
+      d.batch = batch;
+      d.transA = 1; d.M = jb; d.N = jb; d.K = mrows;
+      d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
+      d.C = G; d.ldc = jb; d.strideC = sG;
+      ZOLO_CHECK(gemm_t<T>(c, d));
+      T* Tp = Tb + (p - P) * (nbo + 1);
+      ZOLO_CHECK(k_qr_form_t<T>(G, jb, sG, tau, n, p, jb, Tp, nbo, sTall, batch, c.s));
+      const int64_t ncols = P + JB - p - jb;  // rest of the outer block only
+      if (ncols > 0)
+        ZOLO_CHECK(apply_block_reflector<T>(c, V, mrows, sV, Tp, nbo, sTall, jb, S + p + (p + jb) * lds, lds, sS, mrows,
+                                            ncols, true, (T*)q.W, (T*)q.W2, sW, batch));
+    }
+    const int64_t mrows = n + JB;
+    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, P, JB, mrows, batch, c.s));
+    if (JB > nb) {
+      GemmDesc d;
+      d.batch = batch;
+      d.transA = 1; d.M = JB; d.N = JB; d.K = mrows;
+      d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
+      d.C = G; d.ldc = nbo; d.strideC = sG;
+      ZOLO_CHECK(gemm_t<T>(c, d));
+      ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch));
+    }
+    const int64_t ncols = n - P - JB;
+    if (ncols > 0)
+      ZOLO_CHECK(apply_block_reflector<T>(c, V, mrows, sV, Tb, nbo, sTall, JB, S + P + (P + JB) * lds, lds, sS, mrows,
+                                          ncols, true, (T*)q.W, (T*)q.W2, sW, batch));
+  }
+  return kOk;
+}
+
+template <typename T>
+int orgqr_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, T* Q, int64_t ldq, int64_t sQ,
+               int64_t n, int batch, T* Tall, bool have_t, Arena& ar) {
+  const int nb = kQrNb, nbo = kQrOuter;
+  QrScratch q;
+  ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
+  T* V = (T*)q.V;
+  T* G = (T*)q.G;
+  const int64_t mr = n + nbo;
+  const int64_t sV = mr * nbo, sG = (int64_t)nbo * nbo, sW = (int64_t)nbo * n;
+  const int64_t nout = (n + nbo - 1) / nbo;
+  const int64_t sTall = nout * sG;
+  // Q = H_0 ... H_{n-1} [I; 0]: one outer block reflector at a time, last to first.
+  ZOLO_CHECK(k_set_identity_top<T>(Q, ldq, sQ, 2 * n, n, batch, c.s));
+  for (int64_t ob = nout - 1; ob >= 0; --ob) {
+    const int64_t P = ob * nbo;
+    const int JB = (int)std::min<int64_t>(nbo, n - P);
+    const int64_t mrows = n + JB;
+    T* Tb = Tall + ob * sG;
+    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, P, JB, mrows, batch, c.s));
+    if (!have_t) {  // re-form T of the outer block from V and tau
+      ZOLO_CHECK(k_fill<T>(Tb, nbo, sTall, nbo, nbo, 0.0, batch, c.s));
+      GemmDesc d;
+      d.batch = batch;
+      d.transA = 1; d.M = JB; d.N = JB; d.K = mrows;
+      d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
+      d.C = G; d.ldc = nbo; d.strideC = sG;
+      ZOLO_CHECK(gemm_t<T>(c, d));
+      for (int64_t p = P; p < P + JB; p += nb) {
+        const int jb = (int)std::min<int64_t>(nb, P + JB - p);
+        const int64_t o = p - P;
+        ZOLO_CHECK(k_qr_form_t<T>(G + o * (nbo + 1), nbo, sG, tau, n, p, jb, Tb + o * (nbo + 1), nbo, sTall, batch,
+                                  c.s));
+      }
+      ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch));
+    }
+    ZOLO_CHECK(apply_block_reflector<T>(c, V, mrows, sV, Tb, nbo, sTall, JB, Q + P + P * ldq, ldq, sQ, mrows, n - P,
+                                        false, (T*)q.W, (T*)q.W2, sW, batch));
+  }
+  return kOk;
+}
+
+}  // namespace
+
+template <typename T>
+size_t stacked_qr_bytes(int64_t n, int r) {
+  const int64_t nbo = kQrOuter, mr = n + nbo;
+  const int nwg = qr_nwg(n + kQrNb);
+  const int64_t nout = (n + nbo - 1) / nbo;
+  size_t b = 0;
+  b += sizeof(T) * (size_t)r * mr * nbo;                               // V
+  b += 2 * sizeof(T) * (size_t)r * nbo * nbo;                          // G, combine scratch
+  b += sizeof(T) * (size_t)r * nout * nbo * nbo;                       // T of every outer block
+  b += 2 * sizeof(T) * (size_t)r * nbo * n;                            // W, W2
+  b += sizeof(double) * (size_t)r * kQrNb * ((size_t)nwg * kQrNb + kQrNb);  // panel partials
+  b += sizeof(T) * (size_t)r * n;                                      // tau
+  return b + 16 * 256;
+}
+
+template <typename T>
+int stacked_geqrf(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, Arena& ar) {
+  const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
+  T* Tall = ar.take<T>((size_t)batch * nout * kQrOuter * kQrOuter);
+  ZOLO_ARENA_OK(ar);
+  return geqrf_impl<T>(c, S, lds, sS, n, batch, tau, Tall, ar);
+}
+
+template <typename T>
+int stacked_orgqr(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, T* Q, int64_t ldq, int64_t sQ,
+                  int64_t n, int batch, Arena& ar) {
+  const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
+  T* Tall = ar.take<T>((size_t)batch * nout * kQrOuter * kQrOuter);
+  ZOLO_ARENA_OK(ar);
+  return orgqr_impl<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, Tall, false, ar);
+}
+
+template <typename T>
+int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
+               T* tau_out, Arena& ar) {
+  T* tau = tau_out ? tau_out : ar.take<T>((size_t)batch * n);
+  const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
+  T* Tall = ar.take<T>((size_t)batch * nout * kQrOuter * kQrOuter);
+  ZOLO_ARENA_OK(ar);
+  const size_t mark = ar.used();
+  ZOLO_CHECK(geqrf_impl<T>(c, S, lds, sS, n, batch, tau, Tall, ar));
+  if (!Q) return kOk;
+  ar.reset();  // ORGQR reuses the GEQRF scratch; the outer-block T factors are kept
+  ar.take<char>(mark);
+  return orgqr_impl<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, Tall, true, ar);
+}
+
+#define ZOLO_QRB_INST(T)                                                                                            \
+  template size_t stacked_qr_bytes<T>(int64_t, int);                                                                \
+  template int stacked_qr<T>(const Ctx&, T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, T*, Arena&);     \
+  template int stacked_geqrf<T>(const Ctx&, T*, int64_t, int64_t, int64_t, int, T*, Arena&);                        \
+  template int stacked_orgqr<T>(const Ctx&, const T*, int64_t, int64_t, const T*, T*, int64_t, int64_t, int64_t,   \
+                                int, Arena&);
+ZOLO_QRB_INST(double)
+ZOLO_QRB_INST(float)
+
+}  // namespace zolo
