@@ -183,7 +183,34 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
   const int b_r = (TB == 0) ? t / (BK / EB) : t / (TBN / EB);
   const int b_c = (TB == 0) ? (t % (BK / EB)) * EB : (t % (TBN / EB)) * EB;
 
+  // Interior tiles take unguarded loads (no per-element exec branches); edge
+  // tiles and the K remainder take the guarded path.
+  const bool interior = (i0 + TBM <= d.M) && (j0 + TBN <= d.N);
+  auto load_fast = [&](int64_t k0) {
+    if (TA == 0) {
+      const T* src = A + (i0 + a_c) + (k0 + a_r) * d.lda;
+#pragma unroll
+      for (int e = 0; e < EA; ++e) ra[e] = src[e];
+    } else {
+      const T* src = A + (k0 + a_c) + (i0 + a_r) * d.lda;
+#pragma unroll
+      for (int e = 0; e < EA; ++e) ra[e] = src[e];
+    }
+    if (TB == 0) {
+      const T* src = B + (k0 + b_c) + (j0 + b_r) * d.ldb;
+#pragma unroll
+      for (int e = 0; e < EB; ++e) rb[e] = src[e];
+    } else {
+      const T* src = B + (j0 + b_c) + (k0 + b_r) * d.ldb;
+#pragma unroll
+      for (int e = 0; e < EB; ++e) rb[e] = src[e];
+    }
+  };
   auto load_tiles = [&](int64_t k0) {
+    if (interior && k0 + BK <= ke) {
+      load_fast(k0);
+      return;
+    }
     if (TA == 0) {
       const int64_t k = k0 + a_r;
       const bool kin = k < ke;
@@ -325,7 +352,32 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmDesc d) {
     for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
 
   float ra[16], rb[16];
+  const bool interior = (i0 + BM <= d.M) && (j0 + BN <= d.N);
+  auto load_fast = [&](int64_t k0) {
+    if (TA == 0) {
+      const float* src = A + (i0 + (t & 7) * 16) + (k0 + (t >> 3)) * d.lda;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) ra[e] = src[e];
+    } else {
+      const float* src = A + (k0 + (t & 1) * 16) + (i0 + (t >> 1)) * d.lda;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) ra[e] = src[e];
+    }
+    if (TB == 0) {
+      const float* src = B + (k0 + (t & 1) * 16) + (j0 + (t >> 1)) * d.ldb;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) rb[e] = src[e];
+    } else {
+      const float* src = B + (j0 + (t & 7) * 16) + (k0 + (t >> 3)) * d.ldb;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) rb[e] = src[e];
+    }
+  };
   auto load_tiles = [&](int64_t k0) {
+    if (interior && k0 + BK <= ke) {
+      load_fast(k0);
+      return;
+    }
     if (TA == 0) {  // thread: k = t>>3, 16 consecutive i
       const int kk = t >> 3, ii = (t & 7) * 16;
       const int64_t k = k0 + kk;

@@ -18,9 +18,28 @@ namespace {
 constexpr int kT = 256;
 constexpr int NBP = kQrNb;
 
-__device__ inline double wsum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
+// Sum of a[c] over the 64 lanes of the wave for all 64 columns c at once:
+// a butterfly in which each step exchanges half of the remaining columns with
+// the partner lane (63 shuffles instead of 64 x 6).  Lane l returns the sum of
+// column l.  All indices are compile-time so a[] stays in registers.
+template <int H>
+__device__ inline void butterfly_step(double* a, int lane) {
+  const bool up = (lane & H) != 0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const double keep = up ? a[j + H] : a[j];
+    const double send = up ? a[j] : a[j + H];
+    a[j] = keep + __shfl_xor(send, H, 64);
+  }
+}
+__device__ inline double wave_colsum64(double* a, int lane) {
+  butterfly_step<32>(a, lane);
+  butterfly_step<16>(a, lane);
+  butterfly_step<8>(a, lane);
+  butterfly_step<4>(a, lane);
+  butterfly_step<2>(a, lane);
+  butterfly_step<1>(a, lane);
+  return a[0];
 }
 
 // part layout per shift: for local column jj, a record of (nwg*jb + jb) doubles:
@@ -52,13 +71,21 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
 
   if (apply_prev) {
     const double* G = base + (int64_t)jjp * rec;
-    if (t < jb) {
+    {  // Gram partials of the nwg workgroups: 4 groups of 64 threads, 4 loads in flight each
+      const int c = t & 63, g = t >> 6;
       double s = 0.0;
-      if (t >= jjp)
-        for (int k = 0; k < nwg; ++k) s += G[(int64_t)k * jb + t];
-      gs[t] = s;
-      rr[t] = G[(int64_t)nwg * jb + t];
+      if (c < jb && c >= jjp) {
+        int k = g;
+        for (; k + 12 < nwg; k += 16)
+          s += (G[(int64_t)k * jb + c] + G[(int64_t)(k + 4) * jb + c]) +
+               (G[(int64_t)(k + 8) * jb + c] + G[(int64_t)(k + 12) * jb + c]);
+        for (; k < nwg; k += 4) s += G[(int64_t)k * jb + c];
+      }
+      red[g][c] = s;
+      if (t < jb) rr[t] = G[(int64_t)nwg * jb + t];
     }
+    __syncthreads();
+    if (t < jb) gs[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
     __syncthreads();
     if (t == 0) {
       const double alpha = rr[jjp], xn2 = gs[jjp];
@@ -107,12 +134,8 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
     if (!(live && i > j)) x = 0.0;
     const int lane = t & 63, wave = t >> 6;
 #pragma unroll
-    for (int c = 0; c < NBP; ++c) {
-      if (c < jb) {  // uniform
-        const double r = wsum((c >= jj) ? x * v[c] : 0.0);
-        if (lane == 0) red[wave][c] = r;
-      }
-    }
+    for (int c = 0; c < NBP; ++c) v[c] = (c >= jj && c < jb) ? x * v[c] : 0.0;
+    red[wave][lane] = wave_colsum64(v, lane);
     __syncthreads();
     if (t < jb) G[(int64_t)w * jb + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
   }
