@@ -16,6 +16,7 @@
 // tiles walk down groups of 8 tile-rows so neighbouring workgroups share the
 // same A and B panels in their XCD's L2.
 #include <algorithm>
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -36,14 +37,26 @@ struct LedgerEntry {
 bool g_ledger_on = false;
 std::vector<LedgerEntry> g_ledger;
 
+// Number of (r, k) pairs, r < R, k < K, inside the band lo/hi (k >= r - lo,
+// k <= r + hi; a negative offset means no bound on that side).
+double band_pairs(int64_t R, int64_t K, int lo, int hi) {
+  double n = 0.0;
+  for (int64_t r = 0; r < R; ++r) {
+    const int64_t kb = lo >= 0 ? std::max<int64_t>(0, r - lo) : 0;
+    const int64_t ke = hi >= 0 ? std::min<int64_t>(K, r + hi + 1) : K;
+    if (ke > kb) n += (double)(ke - kb);
+  }
+  return n;
+}
+
 double algorithmic_flops(const GemmDesc& d) {
-  // 2 M N K, halved for each triangular operand or output (band offsets are
-  // small next to the dimensions on every call site of the path).
+  // 2 M N K scaled by the nonzero fraction of each banded operand (exact for
+  // the triangular and trapezoidal operands of the path) and by the computed
+  // fraction of C for lower-triangle-only outputs.
   double f = 2.0 * (double)d.M * (double)d.N * (double)d.K * d.batch;
-  const bool triA = d.aLo >= 0 || d.aHi >= 0, triB = d.bLo >= 0 || d.bHi >= 0;
-  if (triA) f *= 0.5;
-  if (triB) f *= 0.5;
-  if (d.lowerC) f *= 0.5;
+  if (d.K > 0 && (d.aLo >= 0 || d.aHi >= 0)) f *= band_pairs(d.M, d.K, d.aLo, d.aHi) / ((double)d.M * d.K);
+  if (d.K > 0 && (d.bLo >= 0 || d.bHi >= 0)) f *= band_pairs(d.N, d.K, d.bLo, d.bHi) / ((double)d.N * d.K);
+  if (d.lowerC) f *= 0.5 * (double)(d.N + 1) / (double)d.N;
   return f;
 }
 }  // namespace

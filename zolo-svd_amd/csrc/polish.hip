@@ -166,13 +166,16 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
     hipLaunchKernelGGL(dot_kernel, dim3(kNp), dim3(kT), 0, s, a, b, nn, part);
     return reduce(false, h);
   };
-  // M = A_ * B_ (n x n) in FP32: the CG only needs S to a relative accuracy
+  // M = A_ * B_ (n x n) in the low precision of the run (FP32, or BF16x3 with
+  // its ~2^-16 relative accuracy): the CG only needs S to a relative accuracy
   // well below the size of the correction it computes.
+  const Prec lp = (c.low == Prec::BF16X3) ? Prec::BF16X3 : Prec::F32;
+  const double rho_floor = (lp == Prec::BF16X3) ? 1e-5 : 2e-7;
   auto lgemm = [&](const float* A_, const float* B_, float* C_) -> int {
     GemmDesc d;
     d.M = n; d.N = n; d.K = n;
     d.A = A_; d.lda = n; d.B = B_; d.ldb = n; d.C = C_; d.ldc = n;
-    return gemm(Prec::F32, d, s);
+    return gemm(lp, d, s);
   };
   const int max_passes = 3;
   int pass = 0;
@@ -203,7 +206,7 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
     // ~ delta * rho + delta^2 (rho = CG relative residual), so ask CG for no
     // more than 1e-15/delta, within what FP32 arithmetic can deliver.
     const double delta = std::sqrt(rr0 / std::max(hh, 1e-300));
-    const double rho = std::max(2e-7, std::min(1e-2, 1e-15 / std::max(delta, 1e-300)));
+    const double rho = std::max(rho_floor, std::min(1e-2, 1e-15 / std::max(delta, 1e-300)));
     const double tol2 = rho * rho * rr0;
     for (int it = 0; it < 40 && rr > tol2 && rr > 0.0; ++it) {
       ZOLO_CHECK(lgemm(P, Hs, M));
