@@ -204,7 +204,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(n, info.schedule.upper(), info.r)
+        # the CPU port follows the reference's schedule: the FP64 run's, or for a
+        # mixed run without it, the QR step followed by two Cholesky steps
+        ref_sched = compare["f64"]["schedule"] if "f64" in compare else (
+            info.schedule if args.precision == "f64" else "QCC")
+        cpu = cpu_baseline(n, ref_sched.upper(), info.r)
         log(f"cpu baseline {cpu['value']:.1f} s ({cpu['sample']})")
 
     out = {

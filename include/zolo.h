@@ -122,6 +122,42 @@ int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t
 /* Lower Cholesky with explicit inverse: A (lower) -> L, Li = L^{-1}. */
 int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* Li, int64_t ldl, void* stream);
 
+/* ---- batched small polar decompositions (FP32 data) -------------------------
+ * count independent matrices A_i (n_i x n_i, 1 <= n_i <= 512, column-major FP32
+ * in device memory, leading dimension lda_i) -> polar factors U_i (ldu_i).  Per
+ * matrix this is PZOLOPD1 (pzolopd1.f90:280-487) on the bounds of
+ * test_zolodrv1.f90:179-189 (alpha = ||A||_F, beta = rcond/sqrt(n) from an FP64
+ * LU and the 1-norm estimator); the reference calls PZOLOPD1 once per matrix.
+ * The iterations run in FP32 (precision 1) or BF16x3 (precision 2); the
+ * schedule of every matrix follows from its beta, so all iterations of all
+ * matrices run as one captured hipGraph.  A matrix stops after the step whose
+ * CON - 1 is below the FP32 unit roundoff.
+ * l0_source: 0 = the LU estimate, 1 = the learned estimate (zolo_l0_set_weights
+ * first), 2 = beta_in[i] from the caller.  With 1 and 2, a matrix whose
+ * ||U^T U - I||_F / sqrt(n_i) exceeds orth_tol is recomputed with the LU
+ * estimate.  rk > 0 overrides choosem.  tile: -1 autotune per bucket (cached
+ * per process), 0 automatic, 1..4 fixed (128x128, 64x128, 128x64, 64x64).
+ * use_graph = 0 launches eagerly.  Per-matrix outputs (host arrays, each may be
+ * NULL): beta_out, orth_out, iters_out, flags_out (bit 0: fallback used,
+ * bit 1: check failed or no positive bound). */
+typedef struct zolo_batch_info {
+  int count;             /* matrices                                              */
+  int buckets;           /* size buckets (n rounded up to a multiple of 32)       */
+  int fallbacks;         /* matrices recomputed with the LU estimate              */
+  int failures;          /* matrices failing the orthogonality check at the end   */
+  int max_iters;         /* largest iteration count                               */
+  int graph_nodes;       /* nodes of the captured graphs                          */
+  double time_ms;        /* wall time of the call                                 */
+  double phase_ms[8];    /* 0 alpha + beta, 1 plan, 2 tile choice, 3 iterations,
+                            4 fallback, 5 learned features + network              */
+} zolo_batch_info_t;
+int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* const* U,
+                           const int64_t* ldu, int precision, int l0_source, const double* beta_in, double orth_tol,
+                           int rk, int tile, int use_graph, double* beta_out, double* orth_out, int* iters_out,
+                           int* flags_out, zolo_batch_info_t* info, void* stream);
+/* GEMM tile the autotuner chose for matrices of order n (-1 if not tuned yet). */
+int zolo_batched_tile(int64_t n, int precision);
+
 /* ---- measurement: GEMM launch ledger (no counterpart in the reference) ----
  * While enabled, every MFMA GEMM launch is bracketed by hipEvents on its stream
  * and its algorithmic flop count is recorded; zolo_profile_gemm sums launches

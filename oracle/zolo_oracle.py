@@ -23,6 +23,9 @@ How each function is pinned (checked by ``tests/test_oracle.py``):
   by a direct LAPACK solve, and against each other.
 * ``pzolopd1`` / ``pqdwhfacs`` -- against the closed form U V^T of the
   generator A = U diag(s) V^T and against scipy.linalg.polar.
+* ``batched_plan`` -- by applying the planned steps to diagonal matrices whose
+  singular values fill [beta, 1]: the result is orthogonal to the FP32 unit
+  roundoff after the last step and not before it.
 * ``lacn2_norm1_inv`` -- against LAPACK dgecon (scipy) and the exact
   ||X^{-1}||_1 on small matrices.
 * ``normest`` -- against numpy's exact 2-norm.
@@ -415,6 +418,33 @@ def pzolopd1(A: np.ndarray, alpha: float, beta: float, rk: int, sym: str = "N",
         it += 1
     log.iters = it
     return X, log
+
+
+BATCH_CONVERGED = 6e-8   # CON - 1 at which an FP32 iterate has converged
+
+
+def batched_plan(beta: float, rk: int = 0, max_it: int = 6) -> tuple[int, str]:
+    """Schedule of one matrix of the batched FP32 path (pzolopd1.f90:286-470).
+
+    CON follows a scalar recurrence that only depends on beta, so the whole
+    schedule is known in advance: r = choosem(1/beta), step k is a QR step when
+    (k == 0 or CON > 1e3) and max c > 300, else a Cholesky step, and the matrix
+    stops after the step whose CON - 1 is at most BATCH_CONVERGED.  Returns
+    (r, schedule string of 'Q' / 'C').
+    """
+    con = 1.0 / min(beta, 1.0)
+    r = rk if rk > 0 else choosem(con)
+    sched = ""
+    if con - 1.0 <= BATCH_CONVERGED:
+        return r, sched
+    for it in range(max_it):
+        coeff = compute_coeff(con, DLAMCH_P, r)
+        maxc = float(coeff[:2 * r].max())
+        sched += "Q" if (it == 0 or con > 1e3) and maxc > 3e2 else "C"
+        _, con = zolo_fone_con(con, coeff, r)
+        if con - 1.0 <= BATCH_CONVERGED:
+            break
+    return r, sched
 
 
 @dataclass

@@ -310,3 +310,28 @@ def test_mutation_qdwh_weight_is_caught():
     x = np.geomspace(ell, 1.0, 10001)
     g = x * (a + b * x * x) / (1 + c * x * x)
     assert g.max() > 1 + 1e-6 or abs(a - zo.hfunc(ell)) > 1e-3
+
+
+def _scalar_steps(beta, r, nsteps, npts=400):
+    """Singular values of X0 = A / (alpha beta) filling [1, 1/beta], mapped by nsteps Zolotarev steps."""
+    con = 1.0 / beta
+    x = np.geomspace(1.0, con, npts)
+    for _ in range(nsteps):
+        coeff = zo.compute_coeff(con, zo.DLAMCH_P, r)
+        a = coeff[2 * r:]
+        c_odd = coeff[0:2 * r:2]
+        fone, con = zo.zolo_fone_con(con, coeff, r)
+        x = (x - sum(a[j] * x / (x * x + c_odd[j]) for j in range(r))) / fone
+    return x
+
+
+@pytest.mark.parametrize("beta", [0.3, 1e-2, 3e-4, 1e-6, 2e-9, 1e-11])
+def test_batched_plan_converges_exactly_at_last_step(beta):
+    r, sched = zo.batched_plan(beta)
+    assert r == zo.choosem(1.0 / beta)
+    assert 1 <= len(sched) <= 3 and set(sched) <= {"Q", "C"}
+    assert sched[0] == ("Q" if beta < 1e-2 else sched[0])
+    x = _scalar_steps(beta, r, len(sched))
+    assert np.max(np.abs(x - 1.0)) <= 2 * zo.BATCH_CONVERGED
+    x_short = _scalar_steps(beta, r, len(sched) - 1)
+    assert np.max(np.abs(x_short - 1.0)) > zo.BATCH_CONVERGED

@@ -1,0 +1,661 @@
+// Batched Zolo-PD of many independent small FP32 matrices of uneven size.
+//
+// Per matrix the algorithm is pzolopd1.f90:280-487 with the bounds of
+// test_zolodrv1.f90:179-189: alpha = ||A||_F, beta = rcond / sqrt(n) from an
+// FP64 LU and the 1-norm estimator (or a caller / learned estimate, with the
+// LU estimate as the fallback), X0 = A / (alpha beta), r = choosem(1/beta),
+// a QR step when (IT == 0 or CON > 1e3) and max c > 300, else a Cholesky step,
+// X <- (X + sum_j Z_j) / FONE.  The iterations run in FP32 (or BF16x3 on
+// MFMA).  CON follows a scalar recurrence that only depends on beta, so the
+// whole schedule of every matrix is known once beta is: the host plans it,
+// and the iterations of all matrices are captured into one hipGraph and
+// launched once.  A matrix stops after the step whose CON - 1 is below the
+// FP32 unit roundoff (the same rule as the mixed-precision single-matrix
+// path); its orthogonality ||U^T U - I||_F / sqrt(n) is then measured in
+// FP64 and a matrix whose learned or supplied bound fails that check is
+// recomputed with the LU estimate.
+//
+// Layout in HBM: matrices are grouped by nb = n rounded up to a multiple of
+// 32 and stored per bucket as nb x nb column-major slabs, padded as
+// diag(A_i, I).  The padding block stays exactly decoupled through every
+// step (its entries only ever multiply exact zeros), so the leading block of
+// the result is the polar factor of A_i.  All (matrix, shift) items of one
+// bucket and one step go into the same batched launches; GEMM operands are
+// addressed through per-item matrix tables.
+#include "batched.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "coeff.hpp"
+#include "driver.hpp"
+
+namespace zolo {
+
+namespace {
+
+constexpr int kMaxSteps = 6;
+constexpr double kConverged = 6e-8;      // CON - 1 below the FP32 unit roundoff
+constexpr int kMaxPerBucket = 2048;      // matrices per bucket chunk
+constexpr int kFormChunks = 16;          // workgroups per matrix in FormX2
+
+int64_t bucket_of(int64_t n) { return std::max<int64_t>(32, (n + 31) / 32 * 32); }
+
+// Schedule of one matrix, from its beta (host, FP64, as pzolopd1.f90:286-470).
+struct Plan {
+  int r = 0;
+  int steps = 0;
+  char type[kMaxSteps] = {};       // 'Q' or 'C'
+  double shift[kMaxSteps][8];      // QR: sqrt(c_{2j-1}); Cholesky: c_{2j-1}
+  double alpha[kMaxSteps][8];      // QR: -a_j / sqrt(c_{2j-1}); Cholesky: -a_j
+  double inv_fone[kMaxSteps];
+};
+
+bool make_plan(double beta, int rk_override, Plan& p) {
+  if (!(beta > 0.0) || !std::isfinite(beta)) return false;
+  double con = 1.0 / std::min(beta, 1.0);
+  p.r = rk_override > 0 ? rk_override : choosem(con);
+  p.steps = 0;
+  if (con - 1.0 <= kConverged) return true;  // X0 = A / (alpha beta) is already orthogonal
+  const int rk = p.r;
+  std::vector<double> coeff(3 * rk);
+  const double tol = dlamch_p();
+  for (int it = 0; it < kMaxSteps; ++it) {
+    compute_coeff(con, tol, rk, coeff.data());
+    double maxc = 0.0;
+    for (int i = 0; i < 2 * rk; ++i) maxc = std::max(maxc, coeff[i]);
+    const bool qr = (it == 0 || con > 1e3) && maxc > 3e2;
+    p.type[it] = qr ? 'Q' : 'C';
+    for (int q = 0; q < rk; ++q) {
+      if (qr) {
+        p.shift[it][q] = std::sqrt(coeff[2 * q]);
+        p.alpha[it][q] = -coeff[2 * rk + q] / p.shift[it][q];
+      } else {
+        p.shift[it][q] = coeff[2 * q];
+        p.alpha[it][q] = -coeff[2 * rk + q];
+      }
+    }
+    double fone;
+    zolo_fone_con(con, coeff.data(), rk, &fone, &con);
+    p.inv_fone[it] = 1.0 / fone;
+    p.steps = it + 1;
+    if (con - 1.0 <= kConverged) break;
+  }
+  return true;
+}
+
+int own_stream(hipStream_t* s) {
+  static hipStream_t st = nullptr;
+  if (!st) ZOLO_HIP(hipStreamCreate(&st));
+  *s = st;
+  return kOk;
+}
+
+// Per-process cache of the GEMM tile chosen for each (bucket size, precision).
+std::map<std::pair<int64_t, int>, int> g_tile_cache;
+
+// Host-side tables of one (bucket, step), offsets into the device table blob.
+struct StepTables {
+  int nq = 0, nc = 0, bc = 0;            // QR items, Cholesky items, matrices doing Cholesky
+  size_t q_map, q_sd, q_al;              // per QR item: local matrix, sqrt(c), alpha
+  size_t c_mat;                          // per Cholesky matrix: local matrix index
+  size_t c_map, c_slot, c_sh, c_al;      // per Cholesky item: local matrix, W slot, c, alpha
+  size_t zoff, zcnt, ifone;              // per local matrix
+};
+
+struct Bucket {
+  int64_t nb = 0;
+  std::vector<int> mats;                 // global matrix indices
+  size_t idx;                            // device table: global index per local slot
+  std::vector<StepTables> steps;
+  int max_items = 0, max_q = 0, max_c = 0, max_bc = 0;
+};
+
+class Blob {
+ public:
+  template <typename T>
+  size_t put(const std::vector<T>& v) {
+    size_t off = (bytes_.size() + 255) & ~size_t(255);
+    bytes_.resize(off + std::max<size_t>(1, v.size()) * sizeof(T), 0);
+    if (!v.empty()) std::memcpy(bytes_.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+  size_t size() const { return bytes_.size(); }
+  const char* data() const { return bytes_.data(); }
+ private:
+  std::vector<char> bytes_;
+};
+
+template <typename T>
+T* at(char* base, size_t off) {
+  return reinterpret_cast<T*>(base + off);
+}
+
+// GEMM of the batched path (float operands in the low precision of the context).
+int bgemm(const Ctx& c, const GemmDesc& d) { return gemm(c.low, with_tile(c, d), c.s); }
+
+// Bytes of the per-bucket scratch of the Zolotarev steps and the check.
+size_t bucket_bytes(const Bucket& b) {
+  const int64_t nb = b.nb, nn = nb * nb;
+  const size_t B = b.mats.size();
+  const int64_t h = (nb + 1) / 2 + kCholLeaf;
+  size_t x = sizeof(float) * B * nn;                                       // X
+  size_t z = sizeof(float) * (size_t)b.max_items * nn;                     // Z
+  size_t qr = 2 * sizeof(float) * (size_t)b.max_q * 2 * nn + stacked_qr_bytes<float>(nb, std::max(1, b.max_q));
+  size_t ch = sizeof(float) * ((size_t)b.max_bc * nn + 2 * (size_t)b.max_c * nn + (size_t)b.max_c * h * h) +
+              sizeof(int) * 64;
+  size_t chk = 2 * sizeof(double) * B * nn;                                // FP64 copy + Gram
+  size_t part = sizeof(double) * B * kFormChunks;
+  return x + z + std::max(std::max(qr, ch), chk) + part + 16 * 256;
+}
+
+// Bytes of the LU-based estimate for one bucket.
+size_t lu_bytes(int64_t nb, size_t B) {
+  return sizeof(double) * B * ((size_t)nb * nb + (size_t)kBatchLuNb * kBatchLuNb + (size_t)kBatchLuNb * nb) +
+         sizeof(int) * B + 8 * 256;
+}
+
+// Captures what `body` launches on s into a graph and launches it once
+// (use_graph = false runs it eagerly, for debugging and comparison).
+template <typename F>
+int run_captured(hipStream_t s, bool use_graph, size_t* nodes, F&& body) {
+  if (!use_graph) return body();
+  ZOLO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int st = body();
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &g);
+  if (st != kOk) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  ZOLO_HIP(ec);
+  if (nodes) {
+    size_t nn = 0;
+    if (hipGraphGetNodes(g, nullptr, &nn) == hipSuccess) *nodes += nn;
+  }
+  hipGraphExec_t ex = nullptr;
+  hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    ZOLO_HIP(ei);
+  }
+  hipError_t el = hipGraphLaunch(ex, s);
+  hipError_t es = hipStreamSynchronize(s);
+  (void)hipGraphExecDestroy(ex);
+  (void)hipGraphDestroy(g);
+  ZOLO_HIP(el);
+  ZOLO_HIP(es);
+  return kOk;
+}
+
+// FP64 LU + Higham estimate of every matrix of one bucket: beta[g].
+int bucket_condest(const Ctx& c, const BatchMat* d_mats, const int* d_idx, const double* d_alpha, double* d_beta,
+                   int64_t nb, int B, char* ws) {
+  Arena ar(ws, lu_bytes(nb, B));
+  const int64_t nn = nb * nb;
+  const int P = kBatchLuNb;
+  double* W = ar.take<double>((size_t)B * nn);
+  double* Linv = ar.take<double>((size_t)B * P * P);
+  double* T12 = ar.take<double>((size_t)B * P * nb);
+  int* info = ar.take<int>(B);
+  ZOLO_ARENA_OK(ar);
+  ZOLO_CHECK(kb_fill_int(info, B, 0, c.s));
+  ZOLO_CHECK(kb_lu_pad(d_mats, d_idx, d_alpha, W, nb, B, c.s));
+  for (int64_t k0 = 0; k0 < nb; k0 += P) {
+    ZOLO_CHECK(kb_lu_panel(W, nb, k0, Linv, info, B, c.s));
+    const int64_t jb = std::min<int64_t>(P, nb - k0), rest = nb - k0 - jb;
+    if (rest <= 0) continue;
+    GemmDesc d;  // T12 = L11^{-1} A12
+    d.batch = B;
+    d.M = jb; d.N = rest; d.K = jb;
+    d.A = Linv; d.lda = P; d.strideA = (int64_t)P * P;
+    d.B = W + k0 + (k0 + jb) * nb; d.ldb = nb; d.strideB = nn;
+    d.C = T12; d.ldc = jb; d.strideC = (int64_t)P * nb;
+    d.aHi = 0;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    d = GemmDesc();  // A22 -= L21 T12
+    d.batch = B;
+    d.M = rest; d.N = rest; d.K = jb;
+    d.A = W + (k0 + jb) + k0 * nb; d.lda = nb; d.strideA = nn;
+    d.B = T12; d.ldb = jb; d.strideB = (int64_t)P * nb;
+    d.C = W + (k0 + jb) + (k0 + jb) * nb; d.ldc = nb; d.strideC = nn;
+    d.alpha = -1.0; d.beta = 1.0;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    ZOLO_CHECK(kb_copy_block(T12, jb, (int64_t)P * nb, W + k0 + (k0 + jb) * nb, nb, nn, jb, rest, B, c.s));
+  }
+  return kb_condest(d_mats, d_idx, W, nb, info, d_beta, B, c.s);
+}
+
+// The planned Zolotarev steps of one bucket, the FP64 check and the unpadding.
+int bucket_iterate(const Ctx& c, const BatchMat* d_mats, const Bucket& bk, char* tab, const double* d_alpha,
+                   const double* d_beta, double* d_err2, double* d_orth2, char* ws) {
+  const int64_t nb = bk.nb, nn = nb * nb;
+  const int B = (int)bk.mats.size();
+  const int64_t h = (nb + 1) / 2 + kCholLeaf;
+  Arena ar(ws, bucket_bytes(bk));
+  float* X = ar.take<float>((size_t)B * nn);
+  float* Z = ar.take<float>((size_t)std::max(1, bk.max_items) * nn);
+  double* part = ar.take<double>((size_t)B * kFormChunks);
+  const size_t mark = ar.used();
+  ZOLO_ARENA_OK(ar);
+  const int* idx = at<int>(tab, bk.idx);
+  ZOLO_CHECK(kb_pad_scale(d_mats, idx, d_alpha, d_beta, X, nb, B, c.s));
+  for (const StepTables& st : bk.steps) {
+    if (st.nq > 0) {
+      ar.reset();
+      ar.take<char>(mark);
+      float* S = ar.take<float>((size_t)st.nq * 2 * nn);
+      float* Q = ar.take<float>((size_t)st.nq * 2 * nn);
+      ZOLO_ARENA_OK(ar);
+      ZOLO_CHECK(kb_build_stack(X, nb, at<int>(tab, st.q_map), at<double>(tab, st.q_sd), S, st.nq, c.s));
+      ZOLO_CHECK(stacked_qr<float>(c, S, 2 * nb, 2 * nn, Q, 2 * nb, 2 * nn, nb, st.nq, nullptr, ar));
+      GemmDesc d;  // Z_k = alpha_k Q1_k Q2_k^T
+      d.batch = st.nq;
+      d.transA = 0; d.transB = 1; d.M = nb; d.N = nb; d.K = nb;
+      d.A = Q; d.lda = 2 * nb; d.strideA = 2 * nn;
+      d.B = Q + nb; d.ldb = 2 * nb; d.strideB = 2 * nn;
+      d.C = Z; d.ldc = nb; d.strideC = nn;
+      d.alphaVec = at<double>(tab, st.q_al); d.bLo = 256;
+      ZOLO_CHECK(bgemm(c, d));
+    }
+    if (st.nc > 0) {
+      ar.reset();
+      ar.take<char>(mark);
+      float* W = ar.take<float>((size_t)st.bc * nn);
+      float* Wq = ar.take<float>((size_t)st.nc * nn);
+      float* Lq = ar.take<float>((size_t)st.nc * nn);
+      float* tmp = ar.take<float>((size_t)st.nc * h * h);
+      int* info = ar.take<int>(16);
+      ZOLO_ARENA_OK(ar);
+      ZOLO_CHECK(kb_fill_int(info, 16, 0, c.s));
+      GemmDesc d;  // W_s = X_m^T X_m (lower) for the matrices doing a Cholesky step
+      d.batch = st.bc;
+      d.transA = 1; d.M = nb; d.N = nb; d.K = nb;
+      d.A = X; d.lda = nb; d.strideA = nn; d.mapA = at<int>(tab, st.c_mat);
+      d.B = X; d.ldb = nb; d.strideB = nn; d.mapB = at<int>(tab, st.c_mat);
+      d.C = W; d.ldc = nb; d.strideC = nn; d.lowerC = 1;
+      ZOLO_CHECK(bgemm(c, d));
+      ZOLO_CHECK(kb_shift_copy(W, nb, at<int>(tab, st.c_slot), at<double>(tab, st.c_sh), Wq, st.nc, c.s));
+      ZOLO_CHECK(chol_inv<float>(c, Wq, nb, nn, Lq, nb, nn, nb, tmp, h * h, info, st.nc));
+      d = GemmDesc();  // T_k = X_m L_k^{-T}
+      d.batch = st.nc;
+      d.transA = 0; d.transB = 1; d.M = nb; d.N = nb; d.K = nb;
+      d.A = X; d.lda = nb; d.strideA = nn; d.mapA = at<int>(tab, st.c_map);
+      d.B = Lq; d.ldb = nb; d.strideB = nn;
+      d.C = Wq; d.ldc = nb; d.strideC = nn; d.bHi = 0;
+      ZOLO_CHECK(bgemm(c, d));
+      d = GemmDesc();  // Z_{nq+k} = alpha_k T_k L_k^{-1}
+      d.batch = st.nc;
+      d.M = nb; d.N = nb; d.K = nb;
+      d.A = Wq; d.lda = nb; d.strideA = nn;
+      d.B = Lq; d.ldb = nb; d.strideB = nn;
+      d.C = Z + (size_t)st.nq * nn; d.ldc = nb; d.strideC = nn;
+      d.alphaVec = at<double>(tab, st.c_al); d.bLo = 0;
+      ZOLO_CHECK(bgemm(c, d));
+    }
+    ZOLO_CHECK(kb_formx2(X, Z, nb, at<int>(tab, st.zoff), at<int>(tab, st.zcnt), at<double>(tab, st.ifone), part,
+                         kFormChunks, idx, d_err2, B, c.s));
+  }
+  // orthogonality in FP64 of the FP32 result
+  ar.reset();
+  ar.take<char>(mark);
+  double* Xd = ar.take<double>((size_t)B * nn);
+  double* G = ar.take<double>((size_t)B * nn);
+  ZOLO_ARENA_OK(ar);
+  ZOLO_CHECK(kb_f2d(X, Xd, (int64_t)B * nn, c.s));
+  GemmDesc d;
+  d.batch = B;
+  d.transA = 1; d.M = nb; d.N = nb; d.K = nb;
+  d.A = Xd; d.lda = nb; d.strideA = nn;
+  d.B = Xd; d.ldb = nb; d.strideB = nn;
+  d.C = G; d.ldc = nb; d.strideC = nn; d.lowerC = 1;
+  ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+  ZOLO_CHECK(kb_gram_err(G, nb, d_mats, idx, d_orth2, B, c.s));
+  return kb_unpad(X, nb, d_mats, idx, B, c.s);
+}
+
+// Times each candidate tile on the bucket's Gram product and keeps the fastest.
+int tune_tile(const Ctx& c, int64_t nb, int batch, char* ws, size_t ws_bytes, int* tile) {
+  const auto key = std::make_pair(nb, (int)c.low);
+  auto it = g_tile_cache.find(key);
+  if (it != g_tile_cache.end()) {
+    *tile = it->second;
+    return kOk;
+  }
+  const int64_t nn = nb * nb;
+  const size_t need = 2 * sizeof(float) * (size_t)batch * nn;
+  if (nb <= 64 || need > ws_bytes) {
+    *tile = 0;
+    g_tile_cache[key] = 0;
+    return kOk;
+  }
+  float* X = reinterpret_cast<float*>(ws);
+  float* W = X + (size_t)batch * nn;
+  ZOLO_CHECK(k_fill<float>(X, nb, nn, nb, nb, 0.5, batch, c.s));
+  const int cand[2] = {1, 4};  // 128 x 128, 64 x 64
+  float best = 1e30f;
+  int best_tile = 0;
+  hipEvent_t e0, e1;
+  ZOLO_HIP(hipEventCreate(&e0));
+  ZOLO_HIP(hipEventCreate(&e1));
+  for (int t : cand) {
+    GemmDesc d;
+    d.batch = batch;
+    d.transA = 1; d.M = nb; d.N = nb; d.K = nb;
+    d.A = X; d.lda = nb; d.strideA = nn;
+    d.B = X; d.ldb = nb; d.strideB = nn;
+    d.C = W; d.ldc = nb; d.strideC = nn; d.tile = t;
+    int st = bgemm(c, d);  // warm-up
+    if (st == kOk) st = hipEventRecord(e0, c.s) == hipSuccess ? kOk : kErrHip;
+    for (int r = 0; r < 3 && st == kOk; ++r) st = bgemm(c, d);
+    if (st == kOk) st = hipEventRecord(e1, c.s) == hipSuccess ? kOk : kErrHip;
+    float ms = 0.f;
+    if (st == kOk && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+        ms < best) {
+      best = ms;
+      best_tile = t;
+    }
+    if (st != kOk) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return st;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *tile = best_tile;
+  g_tile_cache[key] = best_tile;
+  return kOk;
+}
+
+}  // namespace
+
+int batched_tile_of(int64_t nb, int precision) {
+  auto it = g_tile_cache.find(std::make_pair(bucket_of(nb), precision == 2 ? (int)Prec::BF16X3 : (int)Prec::F32));
+  return it == g_tile_cache.end() ? -1 : it->second;
+}
+
+int pd_batched_run(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* const* U,
+                   const int64_t* ldu, const BatchOptions& opt, BatchResult& res, hipStream_t s) {
+  using clk = std::chrono::steady_clock;
+  const auto t_start = clk::now();
+  auto ms_since = [](clk::time_point t0) {
+    return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+  };
+  if (count < 0) return -1;
+  if (count == 0) return kOk;
+  for (int i = 0; i < count; ++i) {
+    if (n[i] < 1 || n[i] > kMaxBatchN) {
+      set_error("batched PD: matrix " + std::to_string(i) + " has order outside [1, 512]");
+      return -2;
+    }
+    if (!A[i]) return -3;
+    if (lda[i] < n[i]) return -4;
+    if (!U[i]) return -5;
+    if (ldu[i] < n[i]) return -6;
+  }
+  if (opt.precision != 1 && opt.precision != 2) return -7;
+  if (opt.l0_source < 0 || opt.l0_source > 2) return -8;
+  if (opt.l0_source == 2 && !opt.beta_in) return -9;
+  res.beta.assign(count, 0.0);
+  res.orth.assign(count, INFINITY);
+  res.err.assign(count, 0.0);
+  res.iters.assign(count, 0);
+  res.rk.assign(count, 0);
+  res.fallback.assign(count, 0);
+  res.failed.assign(count, 0);
+
+  // A graph cannot be captured on the legacy default stream: work on the
+  // library's own stream after the caller's default-stream work has drained.
+  if (opt.use_graph && s == nullptr) {
+    ZOLO_HIP(hipStreamSynchronize(nullptr));
+    ZOLO_CHECK(own_stream(&s));
+  }
+  Ctx c;
+  c.s = s;
+  c.low = opt.precision == 2 ? Prec::BF16X3 : Prec::F32;
+
+  // buckets (chunks of at most kMaxPerBucket matrices)
+  std::map<int64_t, std::vector<int>> by_nb;
+  for (int i = 0; i < count; ++i) by_nb[bucket_of(n[i])].push_back(i);
+  std::vector<Bucket> buckets;
+  for (auto& kv : by_nb)
+    for (size_t o = 0; o < kv.second.size(); o += kMaxPerBucket) {
+      Bucket b;
+      b.nb = kv.first;
+      b.mats.assign(kv.second.begin() + o, kv.second.begin() + std::min(kv.second.size(), o + kMaxPerBucket));
+      buckets.push_back(std::move(b));
+    }
+
+  // The library workspace holds a persistent region (matrices, alpha, beta,
+  // ERR^2, orth^2), then the table blob of a phase, then that phase's scratch.
+  // It only grows; when it moves, the persistent inputs are uploaded again.
+  const size_t pers = 5 * 256 + sizeof(BatchMat) * (size_t)count + 4 * sizeof(double) * (size_t)count;
+  std::vector<BatchMat> hmats(count);
+  for (int i = 0; i < count; ++i) hmats[i] = BatchMat{n[i], A[i], lda[i], U[i], ldu[i]};
+  char* ws = nullptr;
+  BatchMat* d_mats = nullptr;
+  double *d_alpha = nullptr, *d_beta = nullptr, *d_err2 = nullptr, *d_orth2 = nullptr;
+  char* scratch = nullptr;
+  bool have_bounds = false;
+  auto carve = [&](size_t extra) -> int {
+    char* base = nullptr;
+    ZOLO_CHECK(workspace_acquire(pers + extra + 1024, &base));
+    if (base == ws) return kOk;
+    ws = base;
+    Arena pa(ws, pers);
+    d_mats = pa.take<BatchMat>(count);
+    d_alpha = pa.take<double>(count);
+    d_beta = pa.take<double>(count);
+    d_err2 = pa.take<double>(count);
+    d_orth2 = pa.take<double>(count);
+    ZOLO_ARENA_OK(pa);
+    scratch = ws + ((pa.used() + 255) & ~size_t(255));
+    ZOLO_HIP(hipMemcpyAsync(d_mats, hmats.data(), sizeof(BatchMat) * count, hipMemcpyHostToDevice, s));
+    if (have_bounds) {
+      ZOLO_HIP(hipMemcpyAsync(d_alpha, res.alpha.data(), sizeof(double) * count, hipMemcpyHostToDevice, s));
+      ZOLO_HIP(hipMemcpyAsync(d_beta, res.beta.data(), sizeof(double) * count, hipMemcpyHostToDevice, s));
+    }
+    return kOk;
+  };
+  gemm_set_capture(true);
+  struct CaptureGuard {
+    ~CaptureGuard() { gemm_set_capture(false); }
+  } capture_guard;
+
+  // ---- phase 1: alpha and the lower bound beta --------------------------------
+  const auto t1 = clk::now();
+  {
+    Blob blob;
+    for (auto& b : buckets) b.idx = blob.put(b.mats);
+    const size_t tab_bytes = (blob.size() + 255) & ~size_t(255);
+    size_t lu_max = 0;
+    if (opt.l0_source == 0)
+      for (auto& b : buckets) lu_max = std::max(lu_max, lu_bytes(b.nb, b.mats.size()));
+    ZOLO_CHECK(carve(tab_bytes + lu_max));
+    char* tab = scratch;
+    char* lu_ws = scratch + tab_bytes;
+    ZOLO_HIP(hipMemcpyAsync(tab, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+    ZOLO_CHECK(run_captured(s, opt.use_graph, &res.graph_nodes, [&]() -> int {
+      ZOLO_CHECK(kb_alpha(d_mats, count, d_alpha, s));
+      if (opt.l0_source == 0) {
+        for (auto& b : buckets)
+          ZOLO_CHECK(bucket_condest(c, d_mats, at<int>(tab, b.idx), d_alpha, d_beta, b.nb, (int)b.mats.size(),
+                                    lu_ws));
+      }
+      return kOk;
+    }));
+    res.alpha.assign(count, 0.0);
+    ZOLO_HIP(hipMemcpy(res.alpha.data(), d_alpha, sizeof(double) * count, hipMemcpyDeviceToHost));
+    if (opt.l0_source == 0) {
+      ZOLO_HIP(hipMemcpy(res.beta.data(), d_beta, sizeof(double) * count, hipMemcpyDeviceToHost));
+    } else {
+      for (int i = 0; i < count; ++i) res.beta[i] = opt.beta_in[i];
+      ZOLO_HIP(hipMemcpyAsync(d_beta, res.beta.data(), sizeof(double) * count, hipMemcpyHostToDevice, s));
+    }
+    have_bounds = true;
+  }
+  res.phase_ms[0] = ms_since(t1);
+
+  // ---- phase 2: plans and tables ------------------------------------------------
+  const auto t2 = clk::now();
+  std::vector<Plan> plans(count);
+  std::vector<char> ok(count, 1);
+  for (int i = 0; i < count; ++i) {
+    if (!make_plan(res.beta[i], opt.rk, plans[i])) {
+      ok[i] = 0;
+      res.failed[i] = 1;
+    } else {
+      res.iters[i] = plans[i].steps;
+      res.rk[i] = plans[i].r;
+    }
+  }
+  Blob blob;
+  std::vector<Bucket> live;
+  for (auto& b0 : buckets) {
+    Bucket b;
+    b.nb = b0.nb;
+    for (int g : b0.mats)
+      if (ok[g]) b.mats.push_back(g);
+    if (b.mats.empty()) continue;
+    b.idx = blob.put(b.mats);
+    const int B = (int)b.mats.size();
+    int maxs = 0;
+    for (int g : b.mats) maxs = std::max(maxs, plans[g].steps);
+    for (int k = 0; k < maxs; ++k) {
+      StepTables st;
+      std::vector<int> q_map, c_mat, c_map, c_slot, zoff(B, 0), zcnt(B, 0);
+      std::vector<double> q_sd, q_al, c_sh, c_al, ifone(B, 1.0);
+      for (int m = 0; m < B; ++m) {
+        const Plan& p = plans[b.mats[m]];
+        if (k >= p.steps || p.type[k] != 'Q') continue;
+        zoff[m] = (int)q_map.size();
+        zcnt[m] = p.r;
+        ifone[m] = p.inv_fone[k];
+        for (int q = 0; q < p.r; ++q) {
+          q_map.push_back(m);
+          q_sd.push_back(p.shift[k][q]);
+          q_al.push_back(p.alpha[k][q]);
+        }
+      }
+      const int nq = (int)q_map.size();
+      for (int m = 0; m < B; ++m) {
+        const Plan& p = plans[b.mats[m]];
+        if (k >= p.steps || p.type[k] != 'C') continue;
+        zoff[m] = nq + (int)c_map.size();
+        zcnt[m] = p.r;
+        ifone[m] = p.inv_fone[k];
+        const int slot = (int)c_mat.size();
+        c_mat.push_back(m);
+        for (int q = 0; q < p.r; ++q) {
+          c_map.push_back(m);
+          c_slot.push_back(slot);
+          c_sh.push_back(p.shift[k][q]);
+          c_al.push_back(p.alpha[k][q]);
+        }
+      }
+      st.nq = nq;
+      st.nc = (int)c_map.size();
+      st.bc = (int)c_mat.size();
+      st.q_map = blob.put(q_map); st.q_sd = blob.put(q_sd); st.q_al = blob.put(q_al);
+      st.c_mat = blob.put(c_mat); st.c_map = blob.put(c_map); st.c_slot = blob.put(c_slot);
+      st.c_sh = blob.put(c_sh); st.c_al = blob.put(c_al);
+      st.zoff = blob.put(zoff); st.zcnt = blob.put(zcnt); st.ifone = blob.put(ifone);
+      b.max_items = std::max(b.max_items, st.nq + st.nc);
+      b.max_q = std::max(b.max_q, st.nq);
+      b.max_c = std::max(b.max_c, st.nc);
+      b.max_bc = std::max(b.max_bc, st.bc);
+      b.steps.push_back(st);
+    }
+    live.push_back(std::move(b));
+  }
+  size_t it_max = 0;
+  for (auto& b : live) it_max = std::max(it_max, bucket_bytes(b));
+  const size_t tab_bytes = (blob.size() + 255) & ~size_t(255);
+  res.phase_ms[1] = ms_since(t2);
+
+  // ---- phase 3: tile choice, then all iterations in one graph ------------------
+  const auto t3 = clk::now();
+  {
+    ZOLO_CHECK(carve(tab_bytes + it_max));
+    char* tab = scratch;
+    char* it_ws = scratch + tab_bytes;
+    ZOLO_HIP(hipMemcpyAsync(tab, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+    std::vector<int> tiles(live.size(), 0);
+    for (size_t bi = 0; bi < live.size(); ++bi) {
+      if (opt.tile >= 0) {
+        tiles[bi] = opt.tile;
+      } else {
+        int avg_r = 0;
+        for (int g : live[bi].mats) avg_r += plans[g].r;
+        avg_r = std::max(1, avg_r / (int)live[bi].mats.size());
+        ZOLO_CHECK(tune_tile(c, live[bi].nb, (int)live[bi].mats.size() * avg_r, it_ws, it_max, &tiles[bi]));
+      }
+    }
+    res.phase_ms[2] = ms_since(t3);
+    const auto t4 = clk::now();
+    if (!live.empty()) ZOLO_CHECK(run_captured(s, opt.use_graph, &res.graph_nodes, [&]() -> int {
+      for (size_t bi = 0; bi < live.size(); ++bi) {
+        Ctx cb = c;
+        cb.tile = tiles[bi];
+        ZOLO_CHECK(bucket_iterate(cb, d_mats, live[bi], tab, d_alpha, d_beta, d_err2, d_orth2, it_ws));
+      }
+      return kOk;
+    }));
+    ZOLO_HIP(hipStreamSynchronize(s));
+    res.phase_ms[3] = ms_since(t4);
+    std::vector<double> o2(count), e2(count);
+    ZOLO_HIP(hipMemcpy(o2.data(), d_orth2, sizeof(double) * count, hipMemcpyDeviceToHost));
+    ZOLO_HIP(hipMemcpy(e2.data(), d_err2, sizeof(double) * count, hipMemcpyDeviceToHost));
+    for (int i = 0; i < count; ++i) {
+      if (!ok[i]) continue;
+      res.orth[i] = std::sqrt(o2[i]) / std::sqrt((double)n[i]);
+      res.err[i] = std::sqrt(e2[i]);
+    }
+  }
+  res.buckets = (int)live.size();
+
+  // ---- phase 4: fallback to the LU estimate ------------------------------------
+  const auto t5 = clk::now();
+  if (opt.l0_source != 0) {
+    std::vector<int> redo;
+    for (int i = 0; i < count; ++i)
+      if (!(res.orth[i] <= opt.orth_tol)) redo.push_back(i);
+    if (!redo.empty()) {
+      std::vector<int64_t> rn, rla, rlu;
+      std::vector<const float*> rA;
+      std::vector<float*> rU;
+      for (int i : redo) {
+        rn.push_back(n[i]); rA.push_back(A[i]); rla.push_back(lda[i]); rU.push_back(U[i]); rlu.push_back(ldu[i]);
+      }
+      BatchOptions o2 = opt;
+      o2.l0_source = 0;
+      o2.beta_in = nullptr;
+      BatchResult r2;
+      ZOLO_CHECK(pd_batched_run((int)redo.size(), rn.data(), rA.data(), rla.data(), rU.data(), rlu.data(), o2, r2,
+                                s));
+      for (size_t k = 0; k < redo.size(); ++k) {
+        const int i = redo[k];
+        res.fallback[i] = 1;
+        res.beta[i] = r2.beta[k];
+        res.orth[i] = r2.orth[k];
+        res.err[i] = r2.err[k];
+        res.iters[i] += r2.iters[k];
+        res.rk[i] = r2.rk[k];
+        res.failed[i] = r2.failed[k];
+      }
+      res.graph_nodes += r2.graph_nodes;
+    }
+  }
+  for (int i = 0; i < count; ++i)
+    if (!(res.orth[i] <= opt.orth_tol)) res.failed[i] = 1;
+  res.phase_ms[4] = ms_since(t5);
+  res.time_ms = ms_since(t_start);
+  return kOk;
+}
+
+}  // namespace zolo

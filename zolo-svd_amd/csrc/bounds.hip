@@ -62,11 +62,27 @@ __global__ __launch_bounds__(kT) void lu_argmax_kernel(const double* A, int64_t 
 __global__ __launch_bounds__(kT) void lu_swap_kernel(double* A, int64_t lda, int64_t n, int64_t j, int nwg,
                                                      const Piv* part, int64_t* ipiv, int* info) {
   __shared__ int64_t piv;
-  if (threadIdx.x == 0) {
+  __shared__ double sv[kT];
+  __shared__ int64_t si[kT];
+  {  // reduce the nwg partials with the whole block (fixed order)
     double bv = -1.0;
     int64_t bi = INT64_MAX;
-    for (int k = 0; k < nwg; ++k)
+    for (int k = threadIdx.x; k < nwg; k += kT)
       if (piv_better(part[k].v, part[k].i, bv, bi)) { bv = part[k].v; bi = part[k].i; }
+    sv[threadIdx.x] = bv;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = kT / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s && piv_better(sv[threadIdx.x + s], si[threadIdx.x + s], sv[threadIdx.x], si[threadIdx.x])) {
+        sv[threadIdx.x] = sv[threadIdx.x + s];
+        si[threadIdx.x] = si[threadIdx.x + s];
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    const double bv = sv[0];
+    int64_t bi = si[0];
     if (bi == INT64_MAX) bi = j;
     piv = bi;
     if (blockIdx.x == 0) {

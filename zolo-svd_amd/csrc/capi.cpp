@@ -1,9 +1,11 @@
 // extern "C" entry points of libzolo.so (declared in include/zolo.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "batched.hpp"
 #include "coeff.hpp"
 #include "driver.hpp"
 
@@ -418,5 +420,68 @@ int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* L
   }
   return kOk;
 }
+
+int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* const* U,
+                           const int64_t* ldu, int precision, int l0_source, const double* beta_in, double orth_tol,
+                           int rk, int tile, int use_graph, double* beta_out, double* orth_out, int* iters_out,
+                           int* flags_out, zolo_batch_info_t* info, void* stream) {
+  if (count < 0) return -1;
+  if (count > 0 && (!n || !A || !lda || !U || !ldu)) return -2;
+  if (precision != 1 && precision != 2) return -7;
+  if (l0_source < 0 || l0_source > 2) return -8;
+  if (l0_source == 2 && !beta_in) return -9;
+  if (!(orth_tol > 0.0)) return -10;
+  if (rk < 0 || rk > 8) return -11;
+  if (tile < -1 || tile > 4) return -12;
+  ZOLO_CHECK(ensure_device());
+  BatchOptions opt;
+  opt.precision = precision;
+  opt.orth_tol = orth_tol;
+  opt.rk = rk;
+  opt.tile = tile;
+  opt.use_graph = use_graph != 0;
+  std::vector<double> learned;
+  double learned_ms = 0.0;
+  if (l0_source == 1) {
+    ZOLO_CHECK(l0_learned_beta(count, n, A, lda, learned, &learned_ms, (hipStream_t)stream));
+    opt.l0_source = 2;
+    opt.beta_in = learned.data();
+  } else {
+    opt.l0_source = l0_source;
+    opt.beta_in = beta_in;
+  }
+  BatchResult res;
+  const int st = pd_batched_run(count, n, A, lda, U, ldu, opt, res, (hipStream_t)stream);
+  if (st < 0) {
+    // argument numbers of pd_batched_run follow this function's own list
+    return st;
+  }
+  if (st) return st;
+  int fb = 0, fail = 0, mx = 0;
+  for (int i = 0; i < count; ++i) {
+    if (beta_out) beta_out[i] = res.beta[i];
+    if (orth_out) orth_out[i] = res.orth[i];
+    if (iters_out) iters_out[i] = res.iters[i];
+    if (flags_out) flags_out[i] = (res.fallback[i] ? 1 : 0) | (res.failed[i] ? 2 : 0);
+    fb += res.fallback[i] ? 1 : 0;
+    fail += res.failed[i] ? 1 : 0;
+    mx = std::max(mx, res.iters[i]);
+  }
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    info->count = count;
+    info->buckets = res.buckets;
+    info->fallbacks = fb;
+    info->failures = fail;
+    info->max_iters = mx;
+    info->graph_nodes = (int)res.graph_nodes;
+    for (int k = 0; k < 5; ++k) info->phase_ms[k] = res.phase_ms[k];
+    info->phase_ms[5] = learned_ms;
+    info->time_ms = res.time_ms + learned_ms;
+  }
+  return kOk;
+}
+
+int zolo_batched_tile(int64_t n, int precision) { return batched_tile_of(n, precision); }
 
 }  // extern "C"

@@ -62,6 +62,13 @@ struct GemmDesc {
   int batch = 1;
   int aLo = -1, aHi = -1, bLo = -1, bHi = -1;
   int lowerC = 0;
+  // optional device tables of the batch index of each operand (map[z] instead
+  // of z): a batch over (matrix, shift) items reads the one X of its matrix
+  const int* mapA = nullptr;
+  const int* mapB = nullptr;
+  const int* mapC = nullptr;
+  // tile hint: 0 auto, 1 128x128, 2 64x128, 3 128x64, 4 64x64 (F64 / F32)
+  int tile = 0;
   // split-K (set by the launcher): blockIdx.y selects a K chunk whose raw
   // partial product goes to C + blockIdx.y * splitStride
   int splitk = 1;
@@ -72,6 +79,9 @@ struct GemmDesc {
 enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
 
 int gemm(Prec p, const GemmDesc& d, hipStream_t s);
+// Graph-capture mode: no split-K (its workspace may need an allocation) and
+// no launch ledger events while on.
+void gemm_set_capture(bool on);
 
 // GEMM launch ledger (see gemm.hip).
 void ledger_enable(bool on);

@@ -152,7 +152,7 @@ template <typename T, int TA, int TB, int TBM, int TBN>
 __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
   constexpr int BK = 16;
   constexpr int PAD = (sizeof(T) == 8) ? 4 : 8;
-  constexpr int WM = (TBM == 128 && TBN == 128) ? 2 : (TBM == 64 ? 1 : 4);
+  constexpr int WM = (TBM == TBN) ? 2 : (TBM == 64 ? 1 : 4);
   constexpr int WN = 4 / WM;
   constexpr int FM = TBM / WM / 16, FN = TBN / WN / 16;
   constexpr int EA = TBM * BK / kThreads;   // A elements per thread
@@ -164,9 +164,9 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;
-  const T* __restrict__ A = reinterpret_cast<const T*>(d.A) + z * d.strideA;
-  const T* __restrict__ B = reinterpret_cast<const T*>(d.B) + z * d.strideB;
-  T* __restrict__ C = reinterpret_cast<T*>(d.C) + z * d.strideC;
+  const T* __restrict__ A = reinterpret_cast<const T*>(d.A) + (d.mapA ? (int64_t)d.mapA[z] : z) * d.strideA;
+  const T* __restrict__ B = reinterpret_cast<const T*>(d.B) + (d.mapB ? (int64_t)d.mapB[z] : z) * d.strideB;
+  T* __restrict__ C = reinterpret_cast<T*>(d.C) + (d.mapC ? (int64_t)d.mapC[z] : z) * d.strideC;
 
   int64_t kb, ke;
   k_range<TBM, TBN>(d, i0, j0, kb, ke);
@@ -347,9 +347,9 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmDesc d) {
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;
-  const float* __restrict__ A = reinterpret_cast<const float*>(d.A) + z * d.strideA;
-  const float* __restrict__ B = reinterpret_cast<const float*>(d.B) + z * d.strideB;
-  float* __restrict__ C = reinterpret_cast<float*>(d.C) + z * d.strideC;
+  const float* __restrict__ A = reinterpret_cast<const float*>(d.A) + (d.mapA ? (int64_t)d.mapA[z] : z) * d.strideA;
+  const float* __restrict__ B = reinterpret_cast<const float*>(d.B) + (d.mapB ? (int64_t)d.mapB[z] : z) * d.strideB;
+  float* __restrict__ C = reinterpret_cast<float*>(d.C) + (d.mapC ? (int64_t)d.mapC[z] : z) * d.strideC;
 
   int64_t kb, ke;
   k_range<BM, BN>(d, i0, j0, kb, ke);
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kThreads) void splitk_reduce_kernel(GemmDesc d, con
   const int64_t z = blockIdx.y;
   const int64_t mn = d.M * d.N;
   const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
-  T* C = reinterpret_cast<T*>(d.C) + z * d.strideC;
+  T* C = reinterpret_cast<T*>(d.C) + (d.mapC ? (int64_t)d.mapC[z] : z) * d.strideC;
   const T* Pz = P + z * d.splitk * mn;
   for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < mn; e += (int64_t)gridDim.x * kThreads) {
     const int64_t i = e % d.M, j = e / d.M;
@@ -530,6 +530,8 @@ __global__ __launch_bounds__(kThreads) void splitk_reduce_kernel(GemmDesc d, con
     *p = (d.beta == 0.0) ? (T)v : (T)(v + d.beta * (double)(*p));
   }
 }
+
+bool g_capture = false;
 
 void* g_splitk_ws = nullptr;
 size_t g_splitk_cap = 0;
@@ -553,7 +555,7 @@ struct LedgerScope {
   bool on;
   LedgerEntry e;
   hipStream_t s;
-  LedgerScope(const GemmDesc& d, int prec, int tm, int tn, hipStream_t st) : on(g_ledger_on), s(st) {
+  LedgerScope(const GemmDesc& d, int prec, int tm, int tn, hipStream_t st) : on(g_ledger_on && !g_capture), s(st) {
     if (!on) return;
     e.prec = prec; e.ta = d.transA; e.tb = d.transB; e.tm = tm; e.tn = tn;
     e.flops = algorithmic_flops(d);
@@ -576,7 +578,7 @@ int launch_tile(const GemmDesc& d0, hipStream_t s) {
   GemmDesc d = d0;
   int split = 1;
   const int64_t wgs = tiles * d0.batch;
-  if (!d0.lowerC && wgs < 512 && d0.K >= 1024) {
+  if (!g_capture && !d0.lowerC && wgs < 512 && d0.K >= 1024) {
     split = (int)std::min<int64_t>(std::min<int64_t>(32, cdiv(1024, wgs)), d0.K / 512);
     if (split < 2) split = 1;
   }
@@ -588,6 +590,7 @@ int launch_tile(const GemmDesc& d0, hipStream_t s) {
     d.splitStride = d0.M * d0.N;
     d.C = ws;
     d.ldc = d0.M;
+    d.mapC = nullptr;
     d.strideC = (int64_t)split * d0.M * d0.N;
     d.alpha = 1.0;
     d.alphaVec = nullptr;
@@ -615,6 +618,14 @@ int launch_tile(const GemmDesc& d0, hipStream_t s) {
 // tile so no MFMA work is spent on padding.
 template <typename T>
 int launch_typed(const GemmDesc& d, hipStream_t s) {
+  switch (d.tile) {
+    case 1: return launch_tile<T, 128, 128>(d, s);
+    case 2: return launch_tile<T, 64, 128>(d, s);
+    case 3: return launch_tile<T, 128, 64>(d, s);
+    case 4: return launch_tile<T, 64, 64>(d, s);
+    default: break;
+  }
+  if (d.M <= 64 && d.N <= 64) return launch_tile<T, 64, 64>(d, s);
   if (d.M <= 64 && d.lowerC == 0) return launch_tile<T, 64, 128>(d, s);
   if (d.N <= 64 && d.lowerC == 0) return launch_tile<T, 128, 64>(d, s);
   return launch_tile<T, 128, 128>(d, s);
@@ -633,6 +644,8 @@ int launch_bf16x3(const GemmDesc& d, hipStream_t s) {
 }
 
 }  // namespace
+
+void gemm_set_capture(bool on) { g_capture = on; }
 
 int gemm(Prec p, const GemmDesc& d, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return kOk;

@@ -18,8 +18,8 @@ char* g_ws = nullptr;
 size_t g_ws_cap = 0;
 
 template <typename T>
-int gemm_t(const Ctx& c, GemmDesc d) {
-  return gemm(prec_for<T>(c), d, c.s);
+int gemm_t(const Ctx& c, const GemmDesc& d) {
+  return gemm(prec_for<T>(c), with_tile(c, d), c.s);
 }
 }  // namespace
 

@@ -43,7 +43,14 @@ void workspace_release_all();
 struct Ctx {
   hipStream_t s = nullptr;
   Prec low = Prec::F32;  // GEMM precision used for float operands (F32 or BF16X3)
+  int tile = 0;          // GEMM tile hint for products with M, N > 64 (0: automatic)
 };
+
+// Applies the context's tile hint to a GEMM descriptor.
+inline GemmDesc with_tile(const Ctx& c, GemmDesc d) {
+  if (c.tile && d.tile == 0 && d.M > 64 && d.N > 64) d.tile = c.tile;
+  return d;
+}
 
 template <typename T> inline Prec prec_for(const Ctx& c) { return Prec::F32; }
 template <> inline Prec prec_for<double>(const Ctx&) { return Prec::F64; }

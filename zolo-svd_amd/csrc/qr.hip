@@ -22,17 +22,18 @@ constexpr int NBP = kQrNb;
 // a butterfly in which each step exchanges half of the remaining columns with
 // the partner lane (63 shuffles instead of 64 x 6).  Lane l returns the sum of
 // column l.  All indices are compile-time so a[] stays in registers.
-template <int H>
-__device__ inline void butterfly_step(double* a, int lane) {
+template <int H, typename V>
+__device__ inline void butterfly_step(V* a, int lane) {
   const bool up = (lane & H) != 0;
 #pragma unroll
   for (int j = 0; j < H; ++j) {
-    const double keep = up ? a[j + H] : a[j];
-    const double send = up ? a[j] : a[j + H];
+    const V keep = up ? a[j + H] : a[j];
+    const V send = up ? a[j] : a[j + H];
     a[j] = keep + __shfl_xor(send, H, 64);
   }
 }
-__device__ inline double wave_colsum64(double* a, int lane) {
+template <typename V>
+__device__ inline V wave_colsum64(V* a, int lane) {
   butterfly_step<32>(a, lane);
   butterfly_step<16>(a, lane);
   butterfly_step<8>(a, lane);
@@ -65,9 +66,11 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
   // rows beyond n + j take no part in either reflector of this launch
   const bool live = i < p + n + jb && i <= n + j && i >= (apply_prev ? jp : j);
 
+  // only the columns this launch updates or reduces: [jjp, jb) or [jj, jb)
+  const int cmin = apply_prev ? jjp : jj;
   double v[NBP];
 #pragma unroll
-  for (int c = 0; c < NBP; ++c) v[c] = (live && c < jb) ? (double)Sq[i + (p + c) * lds] : 0.0;
+  for (int c = 0; c < NBP; ++c) v[c] = (live && c >= cmin && c < jb) ? (double)Sq[i + (p + c) * lds] : 0.0;
 
   if (apply_prev) {
     const double* G = base + (int64_t)jjp * rec;
@@ -138,6 +141,82 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
     red[wave][lane] = wave_colsum64(v, lane);
     __syncthreads();
     if (t < jb) G[(int64_t)w * jb + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+  }
+}
+
+// Whole-panel factorisation in one workgroup (FP32 panels of at most
+// kQrPanelWgRows rows, the small matrices of the batched path): one thread per
+// row with the row's jb values in registers, and for every column one
+// workgroup-wide reduction of the Gram row g_c = sum_{i>j} S[i][j] S[i][c]
+// (a wave butterfly, then the waves summed in FP64 in a fixed order).
+constexpr int kPanelWgRows = kQrPanelWgRows;
+
+__global__ __launch_bounds__(kPanelWgRows) void qr_panel_wg_kernel(float* S, int64_t lds, int64_t strideS, int64_t n,
+                                                                   int64_t p, int jb, float* tau, int64_t strideTau) {
+  __shared__ double red[kPanelWgRows / 64][NBP];
+  __shared__ double rr[NBP];
+  __shared__ float wv[NBP];
+  __shared__ float sc[2];
+  const int q = blockIdx.y;
+  float* Sq = S + q * strideS;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nwave = blockDim.x >> 6;
+  const int64_t mrows = n + jb;
+  const int64_t i = p + t;
+  const bool mine = t < mrows;
+  // columns >= jb are zero in registers and get zero updates throughout
+  float v[NBP];
+#pragma unroll
+  for (int c = 0; c < NBP; ++c) v[c] = (mine && c < jb) ? Sq[i + (p + c) * lds] : 0.0f;
+  for (int jj = 0; jj < jb; ++jj) {
+    const int64_t j = p + jj;
+    const bool live = mine && i >= j && i <= n + j;
+    float x = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NBP; ++c)
+      if (c == jj) x = v[c];
+    if (live && i == j) {
+#pragma unroll
+      for (int c = 0; c < NBP; ++c) rr[c] = (double)v[c];
+    }
+    if (!(live && i > j)) x = 0.0f;
+    // lanes c > jj of the sum are the Gram row, lane jj the squared norm below
+    // the diagonal; lanes c < jj are not used
+    float g[NBP];
+#pragma unroll
+    for (int c = 0; c < NBP; ++c) g[c] = x * v[c];
+    red[wave][lane] = (double)wave_colsum64(g, lane);
+    __syncthreads();
+    if (wave == 0) {
+      double gc = 0.0;
+      for (int w = 0; w < nwave; ++w) gc += red[w][lane];
+      const double xn2 = __shfl(gc, jj, 64);
+      const double alpha = rr[jj];
+      double beta = alpha, tv = 0.0, scal = 0.0;
+      if (xn2 > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tv = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      // w_c = tau v^T S[:, c] for c > jj, zero elsewhere
+      wv[lane] = (lane > jj && lane < jb) ? (float)(tv * (rr[lane] + scal * gc)) : 0.0f;
+      if (lane == 0) {
+        sc[0] = (float)beta;
+        sc[1] = (float)scal;
+        tau[q * strideTau + j] = (float)tv;
+      }
+    }
+    __syncthreads();
+    if (live) {
+      const float vi = (i == j) ? 1.0f : x * sc[1];
+      const float newj = (i == j) ? sc[0] : vi;
+#pragma unroll
+      for (int c = 0; c < NBP; ++c) v[c] = (c == jj) ? newj : fmaf(-vi, wv[c], v[c]);
+    }
+  }
+  if (mine) {
+#pragma unroll
+    for (int c = 0; c < NBP; ++c)
+      if (c < jb) Sq[i + (p + c) * lds] = v[c];
   }
 }
 
@@ -222,6 +301,20 @@ int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   }
   hipLaunchKernelGGL((qr_panel_kernel<T>), dim3(nwg, batch), dim3(kT), 0, s, S, lds, strideS, n, p, jb, j, apply_prev,
                      reduce_cur, part, nwg, tau, strideTau);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+int k_qr_panel_wg(float* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, float* tau,
+                  int64_t strideTau, int batch, hipStream_t s) {
+  const int64_t mrows = n + jb;
+  if (jb > NBP || jb <= 0 || mrows > kPanelWgRows) {
+    set_error("qr panel (one workgroup): size out of range");
+    return kErrArg;
+  }
+  const unsigned threads = (unsigned)((mrows + 63) / 64 * 64);
+  hipLaunchKernelGGL(qr_panel_wg_kernel, dim3(1, batch), dim3(threads), 0, s, S, lds, strideS, n, p, jb, tau,
+                     strideTau);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

@@ -2,13 +2,14 @@
 // (mpdgeqrf.f:262-313, mpdorgqr.f:253-337), batched over the Zolotarev shifts.
 //
 // Inner level: panels of kQrNb = 64 columns are factored column by column
-// (qr.hip) and update only the rest of their 256-wide outer block.  Outer
+// (qr.hip; FP32 panels of small stacks in a single workgroup) and update only the rest of their 256-wide outer block.  Outer
 // level: the block reflector of all 256 columns, I - V T V^T, is assembled
 // from the inner T factors (T_AB = -T_AA (V_A^T V_B) T_BB for adjacent column
 // groups) and applied to the trailing matrix, and later to Q, as GEMMs with
 // K = 256.  Every block reflector only touches rows p .. n+p+jb-1: rows below
 // are still zero in the identity block of the stack.
 #include <algorithm>
+#include <type_traits>
 
 #include "linalg.hpp"
 
@@ -19,7 +20,7 @@ constexpr int kQrOuter = 256;
 
 template <typename T>
 int gemm_t(const Ctx& c, const GemmDesc& d) {
-  return gemm(prec_for<T>(c), d, c.s);
+  return gemm(prec_for<T>(c), with_tile(c, d), c.s);
 }
 
 // One thread per panel row (the panel kernel keeps each row in registers).
@@ -124,8 +125,13 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
       const int jb = (int)std::min<int64_t>(nb, P + JB - p);
       const int64_t mrows = n + jb;
       const int nwg = qr_nwg(mrows);
-      for (int64_t j = p; j <= p + jb; ++j)
-        ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, j, j > p, j < p + jb, part, nwg, tau, n, batch, c.s));
+      if (std::is_same<T, float>::value && mrows <= kQrPanelWgRows) {
+        ZOLO_CHECK(k_qr_panel_wg(reinterpret_cast<float*>(S), lds, sS, n, p, jb, reinterpret_cast<float*>(tau), n,
+                                 batch, c.s));
+      } else {
+        for (int64_t j = p; j <= p + jb; ++j)
+          ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, j, j > p, j < p + jb, part, nwg, tau, n, batch, c.s));
+      }
       ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, p, jb, mrows, batch, c.s));
       GemmDesc d;  // G = V^T V -> T of the panel, stored on the diagonal of the outer T
       d.batch = batch;

@@ -9,6 +9,7 @@ Mirrors the reference's Fortran entry points on a 1x1 process grid:
 * ``pdnormest``  -- PDNORMEST (pdnormest.f90:2): 2-norm power-iteration estimate.
 * ``bounds``     -- the alpha/beta estimate of test_zolodrv1.f90:179-189.
 * ``compute_coeff`` / ``choosem`` / ``hfunc`` / ``mellipke`` / ``mellipj`` -- Auxil_Zolo.f90, hfunc.f90.
+* ``pd_batched_device`` -- PZOLOPD1 over many small FP32 matrices of uneven size in one hipGraph.
 
 Host arrays are numpy (column-major copies are made as needed); the
 ``*_device`` variants take torch CUDA tensors (column-major data in HBM, see
@@ -29,6 +30,14 @@ _LIB_PATH = os.path.join(os.path.dirname(_HERE), "libzolo.so")
 
 if not os.path.exists(_LIB_PATH):
     raise ImportError(f"libzolo.so not found at {_LIB_PATH}; build it with `make -C zolo-svd_amd`")
+# One HIP runtime per process: torch ships its own libamdhip64 (found by torch
+# under an unversioned name), so it has to be loaded first; libzolo.so then
+# binds to that copy through the shared soname instead of loading a second
+# runtime that would not see the device.
+try:
+    import torch as _torch_runtime  # noqa: F401
+except ImportError:  # pragma: no cover - host-only use of the scalar helpers
+    _torch_runtime = None
 _lib = C.CDLL(_LIB_PATH)
 
 PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
@@ -116,6 +125,9 @@ _sig("zolo_gemm_device", C.c_int, [C.c_int, C.c_int, C.c_int, _I64, _I64, _I64, 
                                    C.c_double, _P, _I64, _P])
 _sig("zolo_stacked_qr_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P])
 _sig("zolo_chol_inv_device", C.c_int, [C.c_int, _I64, _P, _I64, _P, _I64, _P])
+_sig("zolo_pd_batched_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_double, C.c_int,
+                                         C.c_int, C.c_int, _P, _P, _P, _P, _P, _P])
+_sig("zolo_batched_tile", C.c_int, [_I64, C.c_int])
 _sig("zolo_profile_enable", C.c_int, [C.c_int])
 _sig("zolo_profile_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), _D, _D])
 
@@ -336,6 +348,83 @@ def chol_inv_device(dA, dLi, precision: str = "f64"):
     n = dA.shape[0]
     _check("zolo_chol_inv_device", _lib.zolo_chol_inv_device(PREC[precision], n, C.c_void_p(dA.data_ptr()), n,
                                                              C.c_void_p(dLi.data_ptr()), n, _stream()))
+
+
+class _BatchInfo(C.Structure):
+    _fields_ = [
+        ("count", C.c_int), ("buckets", C.c_int), ("fallbacks", C.c_int), ("failures", C.c_int),
+        ("max_iters", C.c_int), ("graph_nodes", C.c_int), ("time_ms", C.c_double), ("phase_ms", C.c_double * 8),
+    ]
+
+
+BATCH_PHASES = ("bounds", "plan", "tiles", "iterations", "fallback", "learned")
+L0_SOURCE = {"lu": 0, "learned": 1, "given": 2}
+
+
+@dataclass
+class BatchRunInfo:
+    """Per-matrix results and phase times of one batched call (zolo_batch_info_t)."""
+    beta: np.ndarray
+    orth: np.ndarray
+    iters: np.ndarray
+    fallback: np.ndarray
+    failed: np.ndarray
+    buckets: int
+    graph_nodes: int
+    time_ms: float
+    phase_ms: dict
+
+
+def pd_batched_device(mats, precision: str = "f32", l0: str = "lu", beta=None, orth_tol: float = 1e-6, rk: int = 0,
+                      tile: int = -1, use_graph: bool = True, out=None):
+    """Polar factors of many small FP32 matrices (orders 1..512) in one captured hipGraph.
+
+    ``mats`` are float32 CUDA tensors holding column-major matrices (``to_device``
+    layout: the row-major n x n storage of A^T).  ``l0`` selects the lower bound:
+    ``"lu"`` (FP64 LU + 1-norm estimate, test_zolodrv1.f90:179-189), ``"learned"``
+    (the network set by ``l0_set_weights``) or ``"given"`` (``beta``, one per
+    matrix); the last two fall back to ``"lu"`` for a matrix that fails the
+    orthogonality check.  Returns (list of U tensors, BatchRunInfo).
+    """
+    torch = _torch()
+    k = len(mats)
+    for m in mats:
+        if m.dtype != torch.float32 or not m.is_cuda or m.dim() != 2 or m.shape[0] != m.shape[1]:
+            raise ValueError("pd_batched_device: every matrix must be a square float32 CUDA tensor")
+        if not m.is_contiguous():
+            raise ValueError("pd_batched_device: matrices must be contiguous")
+    outs = out if out is not None else [torch.empty_like(m) for m in mats]
+    n = np.array([m.shape[0] for m in mats], dtype=np.int64)
+    pa = np.array([m.data_ptr() for m in mats], dtype=np.uint64)
+    pu = np.array([u.data_ptr() for u in outs], dtype=np.uint64)
+    ld = n.copy()
+    b_in = None
+    if l0 == "given":
+        b_in = np.ascontiguousarray(beta, dtype=np.float64)
+        if b_in.shape != (k,):
+            raise ValueError("pd_batched_device: beta needs one value per matrix")
+    b_out = np.zeros(k)
+    o_out = np.zeros(k)
+    it_out = np.zeros(k, dtype=np.int32)
+    fl_out = np.zeros(k, dtype=np.int32)
+    ci = _BatchInfo()
+
+    def ptr(a):
+        return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+    st = _lib.zolo_pd_batched_device(k, ptr(n), ptr(pa), ptr(ld), ptr(pu), ptr(ld), PREC[precision], L0_SOURCE[l0],
+                                     ptr(b_in), orth_tol, rk, tile, 1 if use_graph else 0, ptr(b_out), ptr(o_out),
+                                     ptr(it_out), ptr(fl_out), C.byref(ci), _stream())
+    _check("zolo_pd_batched_device", st)
+    info = BatchRunInfo(beta=b_out, orth=o_out, iters=it_out, fallback=(fl_out & 1) != 0, failed=(fl_out & 2) != 0,
+                        buckets=ci.buckets, graph_nodes=ci.graph_nodes, time_ms=ci.time_ms,
+                        phase_ms={p: ci.phase_ms[i] for i, p in enumerate(BATCH_PHASES)})
+    return outs, info
+
+
+def batched_tile(n: int, precision: str = "f32") -> int:
+    """GEMM tile chosen by the autotuner for matrices of order n (-1: not tuned yet)."""
+    return int(_lib.zolo_batched_tile(n, PREC[precision]))
 
 
 def finalize():
