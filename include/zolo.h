@@ -128,7 +128,8 @@ int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* L
  * matrix this is PZOLOPD1 (pzolopd1.f90:280-487) on the bounds of
  * test_zolodrv1.f90:179-189 (alpha = ||A||_F, beta = rcond/sqrt(n) from an FP64
  * LU and the 1-norm estimator); the reference calls PZOLOPD1 once per matrix.
- * The iterations run in FP32 (precision 1) or BF16x3 (precision 2); the
+ * The iterations run in FP32 (precision must be 1: BF16x3 products leave the
+ * orthogonality near 1e-5, above this path's 1e-6 target); the
  * schedule of every matrix follows from its beta, so all iterations of all
  * matrices run as one captured hipGraph.  A matrix stops after the step whose
  * CON - 1 is below the FP32 unit roundoff.
@@ -157,6 +158,30 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
                            int* flags_out, zolo_batch_info_t* info, void* stream);
 /* GEMM tile the autotuner chose for matrices of order n (-1 if not tuned yet). */
 int zolo_batched_tile(int64_t n, int precision);
+
+/* ---- learned lower bound (replaces the condition estimate of
+ * test_zolodrv1.f90:185-189 for the batched path, with it as the fallback) ----
+ * Features of X = A/||A||_F (16 per matrix: log2 n, log10 of 8 Lanczos Ritz
+ * values of X^T X, log10 ||X||_1, log10 ||X||_inf, log10 |r_kk| of the R factor
+ * of a 4-column Gaussian sketch X Omega, n ||X||_2^2 estimate) feed a
+ * 16 -> 64 -> 64 -> 1 tanh network predicting y = log10(sigma_min/sigma_max);
+ * beta = 10^(y - margin) * sqrt(theta_1).  Parameters (float, in order):
+ * feature mean[16], feature scale[16], W1[64][16], b1[64], W2[64][64], b2[64],
+ * w3[64], b3, margin -- zolo_l0_param_count() values.  Training minimises the
+ * pinball (quantile) loss mean rho_tau(t - y) with Adam, full batch, on the
+ * device.  feat is count x 16 (row per matrix), all device pointers unless
+ * noted. */
+int zolo_l0_param_count(void);
+int zolo_l0_set_weights(const float* params /* host */, int nparams);
+int zolo_l0_features_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* feat,
+                            float* smax, void* stream);
+int zolo_l0_forward_device(const float* params, const float* feat, int N, float* y, void* stream);
+/* gradient of the loss w.r.t. the trained block (W1 .. b3), loss to a host float */
+int zolo_l0_grad_device(const float* params, const float* feat, const float* target, int N, float tau, float* grad,
+                        float* loss, void* stream);
+/* steps of Adam (lr, beta1 0.9, beta2 0.999) on params in place; loss_hist: host, steps values or NULL */
+int zolo_l0_train_device(float* params, const float* feat, const float* target, int N, int steps, float lr, float tau,
+                         float* loss_hist, void* stream);
 
 /* ---- measurement: GEMM launch ledger (no counterpart in the reference) ----
  * While enabled, every MFMA GEMM launch is bracketed by hipEvents on its stream

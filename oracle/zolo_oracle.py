@@ -421,6 +421,7 @@ def pzolopd1(A: np.ndarray, alpha: float, beta: float, rk: int, sym: str = "N",
 
 
 BATCH_CONVERGED = 6e-8   # CON - 1 at which an FP32 iterate has converged
+BATCH_F64_QR_CON = 1e8   # a first QR step above this CON runs in FP64 on the device
 
 
 def batched_plan(beta: float, rk: int = 0, max_it: int = 6) -> tuple[int, str]:
@@ -430,7 +431,8 @@ def batched_plan(beta: float, rk: int = 0, max_it: int = 6) -> tuple[int, str]:
     schedule is known in advance: r = choosem(1/beta), step k is a QR step when
     (k == 0 or CON > 1e3) and max c > 300, else a Cholesky step, and the matrix
     stops after the step whose CON - 1 is at most BATCH_CONVERGED.  Returns
-    (r, schedule string of 'Q' / 'C').
+    (r, schedule string of 'Q' / 'C').  (On the device a first QR step at
+    CON > BATCH_F64_QR_CON runs in FP64; the schedule is the same.)
     """
     con = 1.0 / min(beta, 1.0)
     r = rk if rk > 0 else choosem(con)

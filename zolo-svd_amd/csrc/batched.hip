@@ -16,11 +16,12 @@
 //  * condest_kernel      Higham's 1-norm estimate of ||(A_i/alpha_i)^{-1}||_1
 //                        from the LU (dlacn2 behind PDGECON at :186), one
 //                        workgroup per matrix; beta_i = rcond / sqrt(n_i) (:189).
-//  * pad_scale_kernel    X_i = diag(A_i / (alpha_i beta_i), I) in FP32.
+//  * pad_scale_kernel    X_i = diag(A_i / (alpha_i beta_i), I) in FP32 (or FP64).
 //  * build_stack_items / shift_copy_items: the per-item inputs of the QR and
 //                        Cholesky steps, from X or X^T X of the item's matrix.
 //  * formx2_items_kernel X_i <- (X_i + sum of its items' Z) / FONE_i and
-//                        ERR_i^2 (pzolopd1.f90:477-481, FormX2).
+//                        ERR_i^2 (pzolopd1.f90:477-481, FormX2); formx2_f64_kernel
+//                        the same for an FP64 step, rounding the result to FP32.
 //  * gram_err_kernel     ||X_i^T X_i - I||_F^2 over the unpadded block.
 //  * unpad_kernel        U_i = leading n_i x n_i block of X_i.
 #include <algorithm>
@@ -331,32 +332,34 @@ __global__ __launch_bounds__(kT) void condest_kernel(const BatchMat* mats, const
 // ---------------------------------------------------------------------------
 // Zolotarev iterations (FP32, bucket layout: matrix b at X + b * nb * nb).
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(kT) void pad_scale_kernel(const BatchMat* mats, const int* idx, const double* alpha,
-                                                       const double* beta, float* X, int64_t nb) {
+                                                       const double* beta, T* X, int64_t nb) {
   const int b = blockIdx.y;
   const int g = idx[b];
   const BatchMat m = mats[g];
-  float* Xb = X + (int64_t)b * nb * nb;
+  T* Xb = X + (int64_t)b * nb * nb;
   const double sc = 1.0 / (alpha[g] * beta[g]);
   for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < nb * nb; e += (int64_t)gridDim.x * kT) {
     const int64_t i = e % nb, j = e / nb;
-    float v;
-    if (i < m.n && j < m.n) v = (float)(sc * (double)m.A[i + j * m.lda]);
-    else v = (i == j) ? 1.0f : 0.0f;
+    T v;
+    if (i < m.n && j < m.n) v = (T)(sc * (double)m.A[i + j * m.lda]);
+    else v = (i == j) ? (T)1 : (T)0;
     Xb[e] = v;
   }
 }
 
 // S_k = [X_{map[k]}; sdiag_k I]  (2nb x nb)
-__global__ __launch_bounds__(kT) void build_stack_items_kernel(const float* X, int64_t nb, const int* map,
-                                                               const double* sdiag, float* S) {
+template <typename T>
+__global__ __launch_bounds__(kT) void build_stack_items_kernel(const T* X, int64_t nb, const int* map,
+                                                               const double* sdiag, T* S) {
   const int k = blockIdx.y;
-  const float* Xb = X + (int64_t)map[k] * nb * nb;
-  float* Sk = S + (int64_t)k * 2 * nb * nb;
-  const float sd = (float)sdiag[k];
+  const T* Xb = X + (int64_t)map[k] * nb * nb;
+  T* Sk = S + (int64_t)k * 2 * nb * nb;
+  const T sd = (T)sdiag[k];
   for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < 2 * nb * nb; e += (int64_t)gridDim.x * kT) {
     const int64_t i = e % (2 * nb), j = e / (2 * nb);
-    Sk[e] = (i < nb) ? Xb[i + j * nb] : ((i - nb == j) ? sd : 0.0f);
+    Sk[e] = (i < nb) ? Xb[i + j * nb] : ((i - nb == j) ? sd : (T)0);
   }
 }
 
@@ -399,6 +402,32 @@ __global__ __launch_bounds__(kT) void formx2_items_kernel(float* X, const float*
   }
   acc = block_sum256(acc, sh);
   if (threadIdx.x == 0) part[(int64_t)b * gridDim.x + blockIdx.x] = acc;
+}
+
+// FP64 step of slot c (matrix loc[c] of the bucket): X32_loc <- (X64_c + sum Z64[zoff[c] + k]) * ifone[c],
+// rounded to FP32; part[c][chunk] = sum (X+ - X)^2 in FP64
+__global__ __launch_bounds__(kT) void formx2_f64_kernel(const double* X64, const double* Z, float* X32, int64_t nn,
+                                                        const int* loc, const int* zoff, const int* zcnt,
+                                                        const double* ifone, double* part) {
+  __shared__ double sh[4];
+  const int c = blockIdx.y;
+  const int cnt = zcnt[c];
+  const double* Xc = X64 + (int64_t)c * nn;
+  const double* Zc = Z + (int64_t)zoff[c] * nn;
+  float* Xo = X32 + (int64_t)loc[c] * nn;
+  const double f = ifone[c];
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < nn; e += (int64_t)gridDim.x * kT) {
+    const double x = Xc[e];
+    double s = x;
+    for (int k = 0; k < cnt; ++k) s += Zc[(int64_t)k * nn + e];
+    const double xp = s * f;
+    const double d = xp - x;
+    acc += d * d;
+    Xo[e] = (float)xp;
+  }
+  acc = block_sum256(acc, sh);
+  if (threadIdx.x == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = acc;
 }
 
 // out[gid[b]] = sum of the chunk partials of matrix b (matrices with zcnt = 0 untouched)
@@ -497,17 +526,36 @@ int kb_condest(const BatchMat* mats, const int* idx, const double* W, int64_t nb
   return kOk;
 }
 
-int kb_pad_scale(const BatchMat* mats, const int* idx, const double* alpha, const double* beta, float* X, int64_t nb,
+template <typename T>
+int kb_pad_scale(const BatchMat* mats, const int* idx, const double* alpha, const double* beta, T* X, int64_t nb,
                  int count, hipStream_t s) {
-  hipLaunchKernelGGL(pad_scale_kernel, dim3(gfor(nb * nb, 64), count), dim3(kT), 0, s, mats, idx, alpha, beta, X, nb);
+  hipLaunchKernelGGL((pad_scale_kernel<T>), dim3(gfor(nb * nb, 64), count), dim3(kT), 0, s, mats, idx, alpha, beta, X,
+                     nb);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }
+template int kb_pad_scale<float>(const BatchMat*, const int*, const double*, const double*, float*, int64_t, int,
+                                 hipStream_t);
+template int kb_pad_scale<double>(const BatchMat*, const int*, const double*, const double*, double*, int64_t, int,
+                                  hipStream_t);
 
-int kb_build_stack(const float* X, int64_t nb, const int* map, const double* sdiag, float* S, int items,
-                   hipStream_t s) {
-  hipLaunchKernelGGL(build_stack_items_kernel, dim3(gfor(2 * nb * nb, 64), items), dim3(kT), 0, s, X, nb, map, sdiag,
-                     S);
+template <typename T>
+int kb_build_stack(const T* X, int64_t nb, const int* map, const double* sdiag, T* S, int items, hipStream_t s) {
+  hipLaunchKernelGGL((build_stack_items_kernel<T>), dim3(gfor(2 * nb * nb, 64), items), dim3(kT), 0, s, X, nb, map,
+                     sdiag, S);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+template int kb_build_stack<float>(const float*, int64_t, const int*, const double*, float*, int, hipStream_t);
+template int kb_build_stack<double>(const double*, int64_t, const int*, const double*, double*, int, hipStream_t);
+
+int kb_formx2_f64(const double* X64, const double* Z, float* X32, int64_t nb, const int* loc, const int* zoff,
+                  const int* zcnt, const double* ifone, double* part, int nchunk, const int* gid, double* err2,
+                  int count, hipStream_t s) {
+  hipLaunchKernelGGL(formx2_f64_kernel, dim3(nchunk, count), dim3(kT), 0, s, X64, Z, X32, nb * nb, loc, zoff, zcnt,
+                     ifone, part);
+  hipLaunchKernelGGL(sum_parts_kernel, dim3((count + kT - 1) / kT), dim3(kT), 0, s, part, nchunk, count, zcnt, gid,
+                     err2);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

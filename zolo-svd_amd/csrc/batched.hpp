@@ -28,10 +28,14 @@ int kb_copy_block(const double* S, int64_t lds, int64_t sS, double* D, int64_t l
                   int count, hipStream_t s);
 int kb_condest(const BatchMat* mats, const int* idx, const double* W, int64_t nb, const int* info, double* beta,
                int count, hipStream_t s);
-int kb_pad_scale(const BatchMat* mats, const int* idx, const double* alpha, const double* beta, float* X, int64_t nb,
+template <typename T>
+int kb_pad_scale(const BatchMat* mats, const int* idx, const double* alpha, const double* beta, T* X, int64_t nb,
                  int count, hipStream_t s);
-int kb_build_stack(const float* X, int64_t nb, const int* map, const double* sdiag, float* S, int items,
-                   hipStream_t s);
+template <typename T>
+int kb_build_stack(const T* X, int64_t nb, const int* map, const double* sdiag, T* S, int items, hipStream_t s);
+int kb_formx2_f64(const double* X64, const double* Z, float* X32, int64_t nb, const int* loc, const int* zoff,
+                  const int* zcnt, const double* ifone, double* part, int nchunk, const int* gid, double* err2,
+                  int count, hipStream_t s);
 int kb_shift_copy(const float* W, int64_t nb, const int* map, const double* shift, float* Wq, int items,
                   hipStream_t s);
 int kb_formx2(float* X, const float* Z, int64_t nb, const int* zoff, const int* zcnt, const double* ifone,
@@ -44,7 +48,7 @@ int kb_fill_int(int* p, int count, int value, hipStream_t s);
 
 // Driver (batched_pd.cpp).
 struct BatchOptions {
-  int precision = 1;               // 1: FP32 iterations, 2: BF16x3 on MFMA
+  int precision = 1;               // 1: FP32 iterations (the only one accepted)
   int l0_source = 0;               // 0: FP64 LU + 1-norm estimate; 2: beta_in (LU estimate as fallback)
   const double* beta_in = nullptr; // host, one per matrix
   double orth_tol = 1e-6;          // ||U^T U - I||_F / sqrt(n) accepted by the check

@@ -5,8 +5,12 @@
 // FP64 LU and the 1-norm estimator (or a caller / learned estimate, with the
 // LU estimate as the fallback), X0 = A / (alpha beta), r = choosem(1/beta),
 // a QR step when (IT == 0 or CON > 1e3) and max c > 300, else a Cholesky step,
-// X <- (X + sum_j Z_j) / FONE.  The iterations run in FP32 (or BF16x3 on
-// MFMA).  CON follows a scalar recurrence that only depends on beta, so the
+// X <- (X + sum_j Z_j) / FONE.  The iterations run in FP32 on MFMA, except a
+// first QR step at CON > 1e8, which runs in FP64 from the FP32 data (in FP32
+// its r shifted contributions cancel down to errors of order u * CON, which
+// leaves singular values far from 1); its result is rounded to FP32.  BF16x3
+// products (about 2^-16 relative) would leave the orthogonality near 1e-5,
+// above the 1e-6 target of this path.  CON follows a scalar recurrence that only depends on beta, so the
 // whole schedule of every matrix is known once beta is: the host plans it,
 // and the iterations of all matrices are captured into one hipGraph and
 // launched once.  A matrix stops after the step whose CON - 1 is below the
@@ -29,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "coeff.hpp"
@@ -40,6 +45,7 @@ namespace {
 
 constexpr int kMaxSteps = 6;
 constexpr double kConverged = 6e-8;      // CON - 1 below the FP32 unit roundoff
+constexpr double kF64QrCon = 1e8;        // a first QR step above this CON runs in FP64
 constexpr int kMaxPerBucket = 2048;      // matrices per bucket chunk
 constexpr int kFormChunks = 16;          // workgroups per matrix in FormX2
 
@@ -50,6 +56,7 @@ struct Plan {
   int r = 0;
   int steps = 0;
   char type[kMaxSteps] = {};       // 'Q' or 'C'
+  bool f64[kMaxSteps] = {};        // step run in FP64 (a first QR step at large CON)
   double shift[kMaxSteps][8];      // QR: sqrt(c_{2j-1}); Cholesky: c_{2j-1}
   double alpha[kMaxSteps][8];      // QR: -a_j / sqrt(c_{2j-1}); Cholesky: -a_j
   double inv_fone[kMaxSteps];
@@ -70,6 +77,7 @@ bool make_plan(double beta, int rk_override, Plan& p) {
     for (int i = 0; i < 2 * rk; ++i) maxc = std::max(maxc, coeff[i]);
     const bool qr = (it == 0 || con > 1e3) && maxc > 3e2;
     p.type[it] = qr ? 'Q' : 'C';
+    p.f64[it] = qr && it == 0 && con > kF64QrCon;
     for (int q = 0; q < rk; ++q) {
       if (qr) {
         p.shift[it][q] = std::sqrt(coeff[2 * q]);
@@ -100,6 +108,9 @@ std::map<std::pair<int64_t, int>, int> g_tile_cache;
 
 // Host-side tables of one (bucket, step), offsets into the device table blob.
 struct StepTables {
+  int n64 = 0, b64 = 0;                  // FP64 QR items, matrices doing them
+  size_t q64_slot, q64_sd, q64_al;       // per FP64 QR item: X64 slot, sqrt(c), alpha
+  size_t m64_loc, m64_gid, m64_zoff, m64_zcnt, m64_ifone;  // per FP64 slot
   int nq = 0, nc = 0, bc = 0;            // QR items, Cholesky items, matrices doing Cholesky
   size_t q_map, q_sd, q_al;              // per QR item: local matrix, sqrt(c), alpha
   size_t c_mat;                          // per Cholesky matrix: local matrix index
@@ -112,7 +123,7 @@ struct Bucket {
   std::vector<int> mats;                 // global matrix indices
   size_t idx;                            // device table: global index per local slot
   std::vector<StepTables> steps;
-  int max_items = 0, max_q = 0, max_c = 0, max_bc = 0;
+  int max_items = 0, max_q = 0, max_c = 0, max_bc = 0, max_n64 = 0, max_b64 = 0;
 };
 
 class Blob {
@@ -149,8 +160,13 @@ size_t bucket_bytes(const Bucket& b) {
   size_t ch = sizeof(float) * ((size_t)b.max_bc * nn + 2 * (size_t)b.max_c * nn + (size_t)b.max_c * h * h) +
               sizeof(int) * 64;
   size_t chk = 2 * sizeof(double) * B * nn;                                // FP64 copy + Gram
+  size_t q64 = 0;
+  if (b.max_n64 > 0)
+    q64 = sizeof(double) * ((size_t)b.max_b64 * nn + (size_t)b.max_n64 * nn + 4 * (size_t)b.max_n64 * nn +
+                            (size_t)b.max_b64 * kFormChunks) +
+          stacked_qr_bytes<double>(nb, b.max_n64) + 8 * 256;
   size_t part = sizeof(double) * B * kFormChunks;
-  return x + z + std::max(std::max(qr, ch), chk) + part + 16 * 256;
+  return x + z + std::max(std::max(std::max(qr, ch), chk), q64) + part + 16 * 256;
 }
 
 // Bytes of the LU-based estimate for one bucket.
@@ -243,15 +259,41 @@ int bucket_iterate(const Ctx& c, const BatchMat* d_mats, const Bucket& bk, char*
   const size_t mark = ar.used();
   ZOLO_ARENA_OK(ar);
   const int* idx = at<int>(tab, bk.idx);
-  ZOLO_CHECK(kb_pad_scale(d_mats, idx, d_alpha, d_beta, X, nb, B, c.s));
+  ZOLO_CHECK(kb_pad_scale<float>(d_mats, idx, d_alpha, d_beta, X, nb, B, c.s));
   for (const StepTables& st : bk.steps) {
+    if (st.n64 > 0) {
+      ar.reset();
+      ar.take<char>(mark);
+      double* X64 = ar.take<double>((size_t)st.b64 * nn);
+      double* Z64 = ar.take<double>((size_t)st.n64 * nn);
+      double* S64 = ar.take<double>((size_t)st.n64 * 2 * nn);
+      double* Q64 = ar.take<double>((size_t)st.n64 * 2 * nn);
+      double* part64 = ar.take<double>((size_t)st.b64 * kFormChunks);
+      ZOLO_ARENA_OK(ar);
+      const int* gid64 = at<int>(tab, st.m64_gid);
+      ZOLO_CHECK(kb_pad_scale<double>(d_mats, gid64, d_alpha, d_beta, X64, nb, st.b64, c.s));
+      ZOLO_CHECK(kb_build_stack<double>(X64, nb, at<int>(tab, st.q64_slot), at<double>(tab, st.q64_sd), S64, st.n64,
+                                        c.s));
+      ZOLO_CHECK(stacked_qr<double>(c, S64, 2 * nb, 2 * nn, Q64, 2 * nb, 2 * nn, nb, st.n64, nullptr, ar));
+      GemmDesc d;  // Z_k = alpha_k Q1_k Q2_k^T (FP64)
+      d.batch = st.n64;
+      d.transA = 0; d.transB = 1; d.M = nb; d.N = nb; d.K = nb;
+      d.A = Q64; d.lda = 2 * nb; d.strideA = 2 * nn;
+      d.B = Q64 + nb; d.ldb = 2 * nb; d.strideB = 2 * nn;
+      d.C = Z64; d.ldc = nb; d.strideC = nn;
+      d.alphaVec = at<double>(tab, st.q64_al); d.bLo = 256;
+      ZOLO_CHECK(gemm(Prec::F64, with_tile(c, d), c.s));
+      ZOLO_CHECK(kb_formx2_f64(X64, Z64, X, nb, at<int>(tab, st.m64_loc), at<int>(tab, st.m64_zoff),
+                               at<int>(tab, st.m64_zcnt), at<double>(tab, st.m64_ifone), part64, kFormChunks, gid64,
+                               d_err2, st.b64, c.s));
+    }
     if (st.nq > 0) {
       ar.reset();
       ar.take<char>(mark);
       float* S = ar.take<float>((size_t)st.nq * 2 * nn);
       float* Q = ar.take<float>((size_t)st.nq * 2 * nn);
       ZOLO_ARENA_OK(ar);
-      ZOLO_CHECK(kb_build_stack(X, nb, at<int>(tab, st.q_map), at<double>(tab, st.q_sd), S, st.nq, c.s));
+      ZOLO_CHECK(kb_build_stack<float>(X, nb, at<int>(tab, st.q_map), at<double>(tab, st.q_sd), S, st.nq, c.s));
       ZOLO_CHECK(stacked_qr<float>(c, S, 2 * nb, 2 * nn, Q, 2 * nb, 2 * nn, nb, st.nq, nullptr, ar));
       GemmDesc d;  // Z_k = alpha_k Q1_k Q2_k^T
       d.batch = st.nq;
@@ -398,7 +440,7 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
     if (!U[i]) return -5;
     if (ldu[i] < n[i]) return -6;
   }
-  if (opt.precision != 1 && opt.precision != 2) return -7;
+  if (opt.precision != 1) return -7;
   if (opt.l0_source < 0 || opt.l0_source > 2) return -8;
   if (opt.l0_source == 2 && !opt.beta_in) return -9;
   res.beta.assign(count, 0.0);
@@ -505,9 +547,18 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
   const auto t2 = clk::now();
   std::vector<Plan> plans(count);
   std::vector<char> ok(count, 1);
+  {  // the elliptic-function coefficients of every step: host threads over matrices
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int per = (count + nt - 1) / nt;
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nt && w * per < count; ++w)
+      pool.emplace_back([&, w]() {
+        for (int i = w * per; i < std::min(count, (w + 1) * per); ++i) ok[i] = make_plan(res.beta[i], opt.rk, plans[i]);
+      });
+    for (auto& th : pool) th.join();
+  }
   for (int i = 0; i < count; ++i) {
-    if (!make_plan(res.beta[i], opt.rk, plans[i])) {
-      ok[i] = 0;
+    if (!ok[i]) {
       res.failed[i] = 1;
     } else {
       res.iters[i] = plans[i].steps;
@@ -530,9 +581,26 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
       StepTables st;
       std::vector<int> q_map, c_mat, c_map, c_slot, zoff(B, 0), zcnt(B, 0);
       std::vector<double> q_sd, q_al, c_sh, c_al, ifone(B, 1.0);
+      std::vector<int> q64_slot, m64_loc, m64_gid, m64_zoff, m64_zcnt;
+      std::vector<double> q64_sd, q64_al, m64_ifone;
       for (int m = 0; m < B; ++m) {
         const Plan& p = plans[b.mats[m]];
-        if (k >= p.steps || p.type[k] != 'Q') continue;
+        if (k >= p.steps || p.type[k] != 'Q' || !p.f64[k]) continue;
+        const int slot = (int)m64_loc.size();
+        m64_loc.push_back(m);
+        m64_gid.push_back(b.mats[m]);
+        m64_zoff.push_back((int)q64_slot.size());
+        m64_zcnt.push_back(p.r);
+        m64_ifone.push_back(p.inv_fone[k]);
+        for (int q = 0; q < p.r; ++q) {
+          q64_slot.push_back(slot);
+          q64_sd.push_back(p.shift[k][q]);
+          q64_al.push_back(p.alpha[k][q]);
+        }
+      }
+      for (int m = 0; m < B; ++m) {
+        const Plan& p = plans[b.mats[m]];
+        if (k >= p.steps || p.type[k] != 'Q' || p.f64[k]) continue;
         zoff[m] = (int)q_map.size();
         zcnt[m] = p.r;
         ifone[m] = p.inv_fone[k];
@@ -558,6 +626,13 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
           c_al.push_back(p.alpha[k][q]);
         }
       }
+      st.n64 = (int)q64_slot.size();
+      st.b64 = (int)m64_loc.size();
+      st.q64_slot = blob.put(q64_slot); st.q64_sd = blob.put(q64_sd); st.q64_al = blob.put(q64_al);
+      st.m64_loc = blob.put(m64_loc); st.m64_gid = blob.put(m64_gid); st.m64_zoff = blob.put(m64_zoff);
+      st.m64_zcnt = blob.put(m64_zcnt); st.m64_ifone = blob.put(m64_ifone);
+      b.max_n64 = std::max(b.max_n64, st.n64);
+      b.max_b64 = std::max(b.max_b64, st.b64);
       st.nq = nq;
       st.nc = (int)c_map.size();
       st.bc = (int)c_mat.size();

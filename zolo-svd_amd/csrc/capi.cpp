@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "batched.hpp"
+#include "mlp.hpp"
 #include "coeff.hpp"
 #include "driver.hpp"
 
@@ -66,6 +67,7 @@ int zolo_init(int device) {
 
 int zolo_finalize(void) {
   workspace_release_all();
+  l0_release();
   return kOk;
 }
 
@@ -427,7 +429,10 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
                            int* flags_out, zolo_batch_info_t* info, void* stream) {
   if (count < 0) return -1;
   if (count > 0 && (!n || !A || !lda || !U || !ldu)) return -2;
-  if (precision != 1 && precision != 2) return -7;
+  if (precision != 1) {
+    set_error("batched PD: only FP32 iterations (precision 1) reach the 1e-6 orthogonality target");
+    return -7;
+  }
   if (l0_source < 0 || l0_source > 2) return -8;
   if (l0_source == 2 && !beta_in) return -9;
   if (!(orth_tol > 0.0)) return -10;
@@ -483,5 +488,71 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
 }
 
 int zolo_batched_tile(int64_t n, int precision) { return batched_tile_of(n, precision); }
+
+int zolo_l0_param_count(void) { return kParamCount; }
+
+int zolo_l0_set_weights(const float* params, int nparams) {
+  if (!params) return -1;
+  if (nparams != kParamCount) {
+    set_error("zolo_l0_set_weights: expected " + std::to_string(kParamCount) + " parameters");
+    return -2;
+  }
+  for (int i = 0; i < kFeat; ++i)
+    if (!(params[kFeat + i] > 0.0f)) {
+      set_error("zolo_l0_set_weights: feature scales must be positive");
+      return -1;
+    }
+  ZOLO_CHECK(ensure_device());
+  return l0_set_weights(params);
+}
+
+int zolo_l0_features_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* feat,
+                            float* smax, void* stream) {
+  if (count < 0) return -1;
+  if (count > 0 && (!n || !A || !lda || !feat || !smax)) return -2;
+  for (int i = 0; i < count; ++i) {
+    if (n[i] < 1 || n[i] > kMaxBatchN) return -2;
+    if (lda[i] < n[i]) return -4;
+  }
+  if (count == 0) return kOk;
+  ZOLO_CHECK(ensure_device());
+  return l0_features(count, n, A, lda, feat, smax, (hipStream_t)stream);
+}
+
+int zolo_l0_forward_device(const float* params, const float* feat, int N, float* y, void* stream) {
+  if (!params) return -1;
+  if (!feat) return -2;
+  if (N < 0) return -3;
+  if (!y) return -4;
+  ZOLO_CHECK(ensure_device());
+  ZOLO_CHECK(kl_forward(params, feat, N, nullptr, nullptr, y, (hipStream_t)stream));
+  ZOLO_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return kOk;
+}
+
+int zolo_l0_grad_device(const float* params, const float* feat, const float* target, int N, float tau, float* grad,
+                        float* loss, void* stream) {
+  if (!params) return -1;
+  if (!feat) return -2;
+  if (!target) return -3;
+  if (N < 1) return -4;
+  if (!(tau > 0.0f && tau < 1.0f)) return -5;
+  if (!grad) return -6;
+  ZOLO_CHECK(ensure_device());
+  return l0_grad(params, feat, target, N, tau, grad, loss, (hipStream_t)stream);
+}
+
+int zolo_l0_train_device(float* params, const float* feat, const float* target, int N, int steps, float lr, float tau,
+                         float* loss_hist, void* stream) {
+  if (!params) return -1;
+  if (!feat) return -2;
+  if (!target) return -3;
+  if (N < 1) return -4;
+  if (steps < 0) return -5;
+  if (!(lr > 0.0f)) return -6;
+  if (!(tau > 0.0f && tau < 1.0f)) return -7;
+  ZOLO_CHECK(ensure_device());
+  return l0_train(params, feat, target, N, steps, lr, tau, loss_hist, (hipStream_t)stream);
+}
 
 }  // extern "C"

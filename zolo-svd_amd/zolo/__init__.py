@@ -128,6 +128,12 @@ _sig("zolo_chol_inv_device", C.c_int, [C.c_int, _I64, _P, _I64, _P, _I64, _P])
 _sig("zolo_pd_batched_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_double, C.c_int,
                                          C.c_int, C.c_int, _P, _P, _P, _P, _P, _P])
 _sig("zolo_batched_tile", C.c_int, [_I64, C.c_int])
+_sig("zolo_l0_param_count", C.c_int, [])
+_sig("zolo_l0_set_weights", C.c_int, [_P, C.c_int])
+_sig("zolo_l0_features_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P])
+_sig("zolo_l0_forward_device", C.c_int, [_P, _P, C.c_int, _P, _P])
+_sig("zolo_l0_grad_device", C.c_int, [_P, _P, _P, C.c_int, C.c_float, _P, C.POINTER(C.c_float), _P])
+_sig("zolo_l0_train_device", C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_float, C.c_float, _P, _P])
 _sig("zolo_profile_enable", C.c_int, [C.c_int])
 _sig("zolo_profile_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), _D, _D])
 
@@ -425,6 +431,80 @@ def pd_batched_device(mats, precision: str = "f32", l0: str = "lu", beta=None, o
 def batched_tile(n: int, precision: str = "f32") -> int:
     """GEMM tile chosen by the autotuner for matrices of order n (-1: not tuned yet)."""
     return int(_lib.zolo_batched_tile(n, PREC[precision]))
+
+
+# ---------------------------------------------------------------------------
+# Learned lower bound (features, network, training on the device)
+# ---------------------------------------------------------------------------
+L0_WEIGHTS = os.path.join(_HERE, "l0_weights.json")
+
+
+def l0_param_count() -> int:
+    return int(_lib.zolo_l0_param_count())
+
+
+def l0_set_weights(params) -> None:
+    """Install the network parameters (zolo_l0_param_count() floats) for ``l0="learned"``."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    _check("zolo_l0_set_weights", _lib.zolo_l0_set_weights(p.ctypes.data_as(C.c_void_p), int(p.size)))
+
+
+def l0_load_weights(path: str = L0_WEIGHTS) -> dict:
+    """Load a weights file written by ``zolo.train_l0`` and install it; returns its metadata."""
+    import json
+    with open(path) as f:
+        meta = json.load(f)
+    l0_set_weights(np.array(meta["params"], dtype=np.float32))
+    return meta
+
+
+def _mat_arrays(mats):
+    n = np.array([m.shape[0] for m in mats], dtype=np.int64)
+    pa = np.array([m.data_ptr() for m in mats], dtype=np.uint64)
+    return n, pa, n.copy()
+
+
+def l0_features_device(mats):
+    """Features (count x 16 float32 CUDA tensor) and smax (count) of column-major float32 matrices."""
+    torch = _torch()
+    k = len(mats)
+    feat = torch.empty((k, 16), dtype=torch.float32, device="cuda")
+    smax = torch.empty((k,), dtype=torch.float32, device="cuda")
+    n, pa, ld = _mat_arrays(mats)
+    _check("zolo_l0_features_device", _lib.zolo_l0_features_device(
+        k, n.ctypes.data_as(C.c_void_p), pa.ctypes.data_as(C.c_void_p), ld.ctypes.data_as(C.c_void_p),
+        C.c_void_p(feat.data_ptr()), C.c_void_p(smax.data_ptr()), _stream()))
+    return feat, smax
+
+
+def l0_forward_device(params, feat):
+    """Network output y = predicted log10(sigma_min/sigma_max) for each feature row."""
+    torch = _torch()
+    y = torch.empty((feat.shape[0],), dtype=torch.float32, device="cuda")
+    _check("zolo_l0_forward_device", _lib.zolo_l0_forward_device(
+        C.c_void_p(params.data_ptr()), C.c_void_p(feat.data_ptr()), feat.shape[0], C.c_void_p(y.data_ptr()),
+        _stream()))
+    return y
+
+
+def l0_grad_device(params, feat, target, tau: float):
+    """(gradient of the pinball loss w.r.t. the trained block, loss) computed by the device kernels."""
+    torch = _torch()
+    g = torch.empty((l0_param_count() - 33,), dtype=torch.float32, device="cuda")
+    loss = C.c_float()
+    _check("zolo_l0_grad_device", _lib.zolo_l0_grad_device(
+        C.c_void_p(params.data_ptr()), C.c_void_p(feat.data_ptr()), C.c_void_p(target.data_ptr()), feat.shape[0],
+        tau, C.c_void_p(g.data_ptr()), C.byref(loss), _stream()))
+    return g, loss.value
+
+
+def l0_train_device(params, feat, target, steps: int, lr: float = 1e-3, tau: float = 0.05):
+    """Full-batch Adam on the device, params (float32 CUDA tensor) updated in place; returns the loss history."""
+    hist = np.zeros(max(steps, 1), dtype=np.float32)
+    _check("zolo_l0_train_device", _lib.zolo_l0_train_device(
+        C.c_void_p(params.data_ptr()), C.c_void_p(feat.data_ptr()), C.c_void_p(target.data_ptr()), feat.shape[0],
+        steps, lr, tau, hist.ctypes.data_as(C.c_void_p), _stream()))
+    return hist[:steps]
 
 
 def finalize():

@@ -1,0 +1,53 @@
+"""Synthetic matrices of the batched workload, generated on the GPU with torch.
+
+Batched polar decomposition workload: independent square FP32 matrices,
+n drawn uniformly from {nmin..nmax}, A = U diag(s) V^T with U, V Haar
+orthogonal (QR of a Gaussian matrix with the sign of diag(R) folded in),
+log10(kappa) uniform in [lo, hi] and s geometric from 1 to 1/kappa.  The
+matrices are formed in FP64 and rounded to FP32, and returned in the
+column-major layout of ``zolo.to_device`` (row-major storage of A^T).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def batch(count: int, seed: int, nmin: int = 32, nmax: int = 512, logk=(1.0, 12.0)):
+    """Returns (list of float32 CUDA tensors, n per matrix, log10 kappa per matrix)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, size=count)
+    lk = rng.uniform(logk[0], logk[1], size=count)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    mats = [None] * count
+    groups = {}
+    for i, n in enumerate(ns):
+        groups.setdefault(int(n), []).append(i)
+    for n in sorted(groups):
+        idx = groups[n]
+        G = torch.randn(2, len(idx), n, n, dtype=torch.float64, device="cuda", generator=g)
+        Q, R = torch.linalg.qr(G)
+        Q = Q * torch.sign(torch.diagonal(R, dim1=-2, dim2=-1)).unsqueeze(-2)
+        kap = torch.tensor(lk[idx], dtype=torch.float64, device="cuda")
+        t = torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1)
+        s = torch.pow(10.0, -kap[:, None] * t[None, :])
+        A = (Q[0] * s[:, None, :]) @ Q[1].transpose(-1, -2).contiguous()
+        At = A.transpose(-1, -2).contiguous().to(torch.float32)
+        for j, i in enumerate(idx):
+            mats[i] = At[j]
+    return mats, ns, lk
+
+
+def log_cond(mats) -> np.ndarray:
+    """log10(sigma_min / sigma_max) of each FP32 matrix from an FP64 SVD (torch, on the GPU)."""
+    import torch
+    out = np.zeros(len(mats))
+    groups = {}
+    for i, m in enumerate(mats):
+        groups.setdefault(m.shape[0], []).append(i)
+    for n, idx in groups.items():
+        S = torch.linalg.svdvals(torch.stack([mats[i] for i in idx]).double())
+        r = torch.log10(S[:, -1] / S[:, 0]).cpu().numpy()
+        out[idx] = r
+    return out
