@@ -16,7 +16,14 @@ libzolo.so, on the stream the kernels run on), and a CPU baseline: the FP64
 CPU restatement of one shifted Cholesky solve timed on the host, extrapolated
 to the executed schedule.
 
+Batched small PDs (the "small PDs/sec" part of the metric): 4096 FP32
+matrices, n uniform in {32..512}, log10 kappa uniform in [1, 12] (zolo.workloads),
+one captured hipGraph per call, bound from the FP64 LU estimate and from the
+learned network (when zolo/l0_weights.json exists), PDs per second of the
+whole call with the inputs in HBM.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--precision bf16x3|f32|f64]
+                       [--small-count 4096] [--no-small] [--svd N]
 """
 import argparse
 import json
@@ -24,6 +31,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -140,6 +149,69 @@ def roofline(zl):
             "launches": cnt, "avg_launch_ms": ms / cnt, "flops_per_launch": fl / cnt}
 
 
+def small_pds(torch, zl, count, seed):
+    """Batched small PDs: PDs per second of whole calls, LU bound and learned bound."""
+    from zolo import workloads
+    log(f"generating {count} small matrices")
+    mats, ns, lk = workloads.batch(count, seed)
+    torch.cuda.synchronize()
+    res = {"count": count, "n_range": [int(ns.min()), int(ns.max())], "workload":
+           "FP32, n ~ U{32..512}, log10 kappa ~ U[1,12], Haar U V, s geometric"}
+    sources = ["lu"]
+    if os.path.exists(zl.L0_WEIGHTS):
+        meta = zl.l0_load_weights()
+        sources.append("learned")
+        res["l0_margin"] = meta.get("margin")
+    for src in sources:
+        zl.pd_batched_device(mats, l0=src)          # warm-up: tile autotune, code paths
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, info = zl.pd_batched_device(mats, l0=src)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[src] = {"pds_per_sec": count / dt, "s": dt, "orth_max": float(info.orth.max()),
+                    "failed": int(info.failed.sum()), "fallback_rate": float(info.fallback.mean()),
+                    "iters_mean": float(info.iters.mean()), "iters_hist": np.bincount(info.iters).tolist(),
+                    "graph_nodes": info.graph_nodes, "phase_ms": {k: round(v, 1) for k, v in info.phase_ms.items()}}
+        log(f"small PDs [{src}]: {count / dt:.0f} PD/s, orth max {info.orth.max():.2e}, "
+            f"fallback {info.fallback.mean():.3f}, phases {res[src]['phase_ms']}")
+    del mats
+    torch.cuda.empty_cache()
+    return res
+
+
+def svd_tail(torch, zl, n, seed):
+    """Zolo-SVD (polar + eigensolver + U = U_p V) of the clustered kappa~3 workload at order n."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed + 4)
+
+    def haar():
+        Q, R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+        return Q * torch.sign(torch.diagonal(R))
+
+    U0, V0 = haar(), haar()
+    k = torch.arange(1, n + 1, dtype=torch.float64, device="cuda")
+    s = 1.0 + 0.5 * torch.sin(k) + 1e-3 * torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    A = (U0 * s) @ V0.T
+    dA = A.T.contiguous()
+    zl.svd_device(dA)                                 # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S, dU, dV, info, ph = zl.svd_device(dA)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    U, V = dU.T, dV.T
+    eye = torch.eye(n, dtype=torch.float64, device="cuda")
+    bwd = (torch.linalg.norm(A - (U * S) @ V.T) / torch.linalg.norm(A)).item()
+    sig_err = (torch.max(torch.abs(S - torch.sort(s).values))).item()
+    out = {"n": n, "s": dt, "schedule": info.schedule, "backward": bwd, "sigma_max_abs_err": sig_err,
+           "orth_U": (torch.linalg.norm(U.T @ U - eye) / math.sqrt(n)).item(),
+           "orth_V": (torch.linalg.norm(V.T @ V - eye) / math.sqrt(n)).item(),
+           "phase_ms": {k: round(v, 1) for k, v in ph.items()}}
+    log(f"zolo-svd n={n}: {dt:.3f} s, {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +222,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20260103)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 / FP32 comparison runs")
+    ap.add_argument("--small-count", type=int, default=4096, help="matrices of the batched small-PD workload")
+    ap.add_argument("--no-small", action="store_true", help="skip the batched small-PD workload")
+    ap.add_argument("--svd", type=int, default=0, help="also time the Zolo-SVD tail at this n (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the polar-decomposition path is built and measured on a single "
@@ -211,6 +286,13 @@ def main():
         cpu = cpu_baseline(n, ref_sched.upper(), info.r)
         log(f"cpu baseline {cpu['value']:.1f} s ({cpu['sample']})")
 
+    small = None
+    if not args.no_small and args.small_count > 0:
+        small = small_pds(torch, zl, args.small_count, args.seed)
+    svd = None
+    if args.svd > 0:
+        svd = svd_tail(torch, zl, args.svd, args.seed)
+
     out = {
         "metric": METRIC,
         "value": sec,
@@ -233,6 +315,10 @@ def main():
         "roofline": rl,
         "cpu_baseline": cpu,
     }
+    if small:
+        out["small_pds"] = small
+    if svd:
+        out["zolo_svd"] = svd
     if "f64" in compare:
         out["speedup_vs_fp64_gpu"] = compare["f64"]["s"] / sec
     if cpu:

@@ -159,6 +159,28 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
 /* GEMM tile the autotuner chose for matrices of order n (-1 if not tuned yet). */
 int zolo_batched_tile(int64_t n, int precision);
 
+/* ---- Zolo-SVD tail (device pointers, FP64) ----------------------------------
+ * zolo_syevd_device replaces ELPA_PDSYEVD2 (elpa_pdsyevd2.F90:1, called at
+ * ZoloCHK1.f90:86): H (n x n symmetric, overwritten) = Z diag(w) Z^T, w
+ * ascending.  Householder tridiagonalisation (MFMA rank-2k updates), Cuppen
+ * divide and conquer on the tridiagonal (Gu-Eisenstat eigenvectors, MFMA
+ * eigenvector updates), blocked back-transformation.  phase_ms (3 values,
+ * may be NULL): tridiagonalisation, divide and conquer, back-transformation.
+ * zolo_svd_device is the flow of ZoloCHK1.f90:80-91 on one GPU: bounds
+ * (test_zolodrv1.f90:179-189), polar factor U_p by Zolo-PD (precision as
+ * zolo_pd_device; rk = 0 picks choosem), H = (U_p^T A + A^T U_p)/2,
+ * H = V diag(S) V^T (S ascending), U = U_p V; A = U diag(S) V^T.
+ * phase_ms (5 values, may be NULL): polar + H, tridiagonalisation, divide and
+ * conquer, back-transformation, eigensolve + U. */
+int zolo_syevd_device(int64_t n, double* H, int64_t ldh, double* w, double* Z, int64_t ldz, double* phase_ms,
+                      void* stream);
+int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U, int64_t ldu,
+                    double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream);
+/* FormX2 (FormX2.f90:2) on host arrays: A <- (A + Z)/fone, Z <- the same, err = ||A_new - A_old||_F. */
+int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err);
+/* Host-array form of zolo_syevd_device (the Fortran ELPA_PDSYEVD2 drop-in calls it). */
+int zolo_syevd_host(int64_t n, double* H, int64_t ldh, double* w, double* Z, int64_t ldz);
+
 /* ---- learned lower bound (replaces the condition estimate of
  * test_zolodrv1.f90:185-189 for the batched path, with it as the fallback) ----
  * Features of X = A/||A||_F (16 per matrix: log2 n, log10 of 8 Lanczos Ritz

@@ -23,6 +23,8 @@ How each function is pinned (checked by ``tests/test_oracle.py``):
   by a direct LAPACK solve, and against each other.
 * ``pzolopd1`` / ``pqdwhfacs`` -- against the closed form U V^T of the
   generator A = U diag(s) V^T and against scipy.linalg.polar.
+* ``zolo_svd`` -- A = U diag(S) V^T with S the generator's singular values
+  (closed form), U and V orthogonal.
 * ``batched_plan`` -- by applying the planned steps to diagonal matrices whose
   singular values fill [beta, 1]: the result is orthogonal to the FP32 unit
   roundoff after the last step and not before it.
@@ -519,6 +521,21 @@ def polar_h(U: np.ndarray, A: np.ndarray) -> np.ndarray:
     """H = (U^T A + A^T U)/2 (pqdwhfacs.f90:460-466, ZoloCHK1.f90:80-83)."""
     T = U.T @ A
     return 0.5 * (T + T.T)
+
+
+def zolo_svd(A: np.ndarray, rk: int = 0):
+    """Zolo-SVD (ZoloCHK1.f90:80-91): polar factor, H, eigendecomposition, U = U_p V.
+
+    The eigensolver step (ELPA_PDSYEVD2, ZoloCHK1.f90:86) is LAPACK's dsyevd
+    here (numpy.linalg.eigh); eigenvalues ascending as ELPA returns them.
+    Returns (S ascending, U, V) with A = U diag(S) V^T.
+    """
+    alpha, beta = bounds_estimate(A)
+    r = rk if rk > 0 else choosem(1.0 / beta)
+    Up, _ = pzolopd1(A, alpha, beta, r)
+    H = polar_h(Up, A)
+    w, V = np.linalg.eigh(H)
+    return w, Up @ V, V
 
 
 # ---------------------------------------------------------------------------

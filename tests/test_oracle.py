@@ -335,3 +335,16 @@ def test_batched_plan_converges_exactly_at_last_step(beta):
     assert np.max(np.abs(x - 1.0)) <= 2 * zo.BATCH_CONVERGED
     x_short = _scalar_steps(beta, r, len(sched) - 1)
     assert np.max(np.abs(x_short - 1.0)) > zo.BATCH_CONVERGED
+
+
+def test_zolo_svd_closed_form():
+    n = 60
+    rng = np.random.default_rng(17)
+    U0 = zo.haar(n, rng)
+    V0 = zo.haar(n, rng)
+    s = 1.0 + 0.5 * np.sin(np.arange(1, n + 1)) + 1e-3 * rng.standard_normal(n)
+    A = (U0 * s) @ V0.T
+    S, U, V = zo.zolo_svd(A)
+    assert np.allclose(S, np.sort(s), atol=1e-13)
+    assert np.linalg.norm(A - (U * S) @ V.T) / np.linalg.norm(A) < 1e-14
+    assert np.linalg.norm(U.T @ U - np.eye(n)) < 1e-13 and np.linalg.norm(V.T @ V - np.eye(n)) < 1e-13

@@ -1,5 +1,6 @@
 // extern "C" entry points of libzolo.so (declared in include/zolo.h).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -9,6 +10,7 @@
 #include "mlp.hpp"
 #include "coeff.hpp"
 #include "driver.hpp"
+#include "eig.hpp"
 
 namespace zolo {
 namespace {
@@ -488,6 +490,130 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
 }
 
 int zolo_batched_tile(int64_t n, int precision) { return batched_tile_of(n, precision); }
+
+int zolo_syevd_device(int64_t n, double* H, int64_t ldh, double* w, double* Z, int64_t ldz, double* phase_ms,
+                      void* stream) {
+  if (n < 1 || n > 32768) return -1;
+  if (!H) return -2;
+  if (ldh < n) return -3;
+  if (!w) return -4;
+  if (!Z) return -5;
+  if (ldz < n) return -6;
+  ZOLO_CHECK(ensure_device());
+  return syevd_run(n, H, ldh, w, Z, ldz, (hipStream_t)stream, phase_ms);
+}
+
+int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err) {
+  if (n < 1) return -1;
+  if (!(fone != 0.0)) return -2;
+  if (!A) return -3;
+  if (lda < n) return -4;
+  if (!Z) return -5;
+  if (ldz < n) return -6;
+  if (!err) return -7;
+  ZOLO_CHECK(ensure_device());
+  DevMat dA, dZ;
+  ZOLO_CHECK(dA.alloc(n));
+  ZOLO_CHECK(dZ.alloc(n));
+  ZOLO_CHECK(dA.upload(A, lda, n));
+  ZOLO_CHECK(dZ.upload(Z, ldz, n));
+  constexpr int kParts = 1024;
+  const size_t bytes = sizeof(double) * (kParts + 8) + 512;
+  char* base = nullptr;
+  ZOLO_CHECK(workspace_acquire(bytes, &base));
+  Arena ar(base, bytes);
+  double* part = ar.take<double>(kParts);
+  double* out = ar.take<double>(8);
+  // X <- (X + Z) / FONE with the error, on the device (FormX2.f90:73-89)
+  ZOLO_CHECK(k_formx2<double>(dA.p, n, dZ.p, n, n * n, 1, n, fone, part, kParts, out, nullptr));
+  double e2 = 0.0;
+  ZOLO_HIP(hipMemcpy(&e2, out, sizeof(double), hipMemcpyDeviceToHost));
+  *err = std::sqrt(e2);
+  ZOLO_CHECK(dA.download(A, lda, n));
+  ZOLO_CHECK(dA.download(Z, ldz, n));  // FormX2 leaves the new iterate in both arrays
+  return kOk;
+}
+
+int zolo_syevd_host(int64_t n, double* H, int64_t ldh, double* w, double* Z, int64_t ldz) {
+  if (n < 1 || n > 32768) return -1;
+  if (!H) return -2;
+  if (ldh < n) return -3;
+  if (!w) return -4;
+  if (!Z) return -5;
+  if (ldz < n) return -6;
+  ZOLO_CHECK(ensure_device());
+  DevMat dH, dZ;
+  ZOLO_CHECK(dH.alloc(n));
+  ZOLO_CHECK(dZ.alloc(n));
+  double* dw = nullptr;
+  ZOLO_HIP(hipMalloc(&dw, sizeof(double) * n));
+  struct Free {
+    double* p;
+    ~Free() { (void)hipFree(p); }
+  } fw{dw};
+  ZOLO_CHECK(dH.upload(H, ldh, n));
+  ZOLO_CHECK(syevd_run(n, dH.p, n, dw, dZ.p, n, nullptr));
+  ZOLO_HIP(hipDeviceSynchronize());
+  ZOLO_HIP(hipMemcpy(w, dw, sizeof(double) * n, hipMemcpyDeviceToHost));
+  ZOLO_CHECK(dZ.download(Z, ldz, n));
+  return kOk;
+}
+
+int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U, int64_t ldu,
+                    double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream) {
+  if (precision < 0 || precision > 2) return -1;
+  if (rk < 0 || rk > 8) return -2;
+  if (n < 1 || n > 32768) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!S) return -6;
+  if (!U) return -7;
+  if (ldu < n) return -8;
+  if (!V) return -9;
+  if (ldv < n) return -10;
+  ZOLO_CHECK(ensure_device());
+  hipStream_t s = (hipStream_t)stream;
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  double alpha = 0.0, beta = 0.0;
+  {
+    const size_t bytes = bounds_bytes(n);
+    char* base = nullptr;
+    ZOLO_CHECK(workspace_acquire(bytes, &base));
+    Arena ar(base, bytes);
+    ZOLO_CHECK(bounds_run(0, n, A, lda, &alpha, &beta, ar, s));
+  }
+  const int r = rk > 0 ? rk : choosem(1.0 / beta);
+  DevMat Ht;  // H, then U_p V
+  ZOLO_CHECK(Ht.alloc(n));
+  zolo_info_t local{};
+  zolo_info_t* inf = info ? info : &local;
+  // polar factor U_p into U, H = (U_p^T A + A^T U_p)/2 into Ht (ZoloCHK1.f90:80-83)
+  const int st = zolo_pd_run(precision, 'N', r, n, alpha, beta, A, lda, U, ldu, Ht.p, n, inf, s);
+  if (st) return st;
+  ZOLO_HIP(hipStreamSynchronize(s));
+  const auto t1 = clk::now();
+  // H = V diag(S) V^T (ZoloCHK1.f90:86), S ascending as ELPA returns it
+  double ph[3] = {0, 0, 0};
+  ZOLO_CHECK(syevd_run(n, Ht.p, n, S, V, ldv, s, ph));
+  // U = U_p V (ZoloCHK1.f90:90)
+  GemmDesc g;
+  g.M = n; g.N = n; g.K = n;
+  g.A = U; g.lda = ldu; g.B = V; g.ldb = ldv; g.C = Ht.p; g.ldc = n;
+  ZOLO_CHECK(gemm(Prec::F64, g, s));
+  ZOLO_HIP(hipMemcpy2DAsync(U, sizeof(double) * ldu, Ht.p, sizeof(double) * n, sizeof(double) * n, n,
+                            hipMemcpyDeviceToDevice, s));
+  ZOLO_HIP(hipStreamSynchronize(s));
+  const auto t2 = clk::now();
+  if (phase_ms) {
+    phase_ms[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();  // bounds + polar + H
+    phase_ms[1] = ph[0];                                                           // tridiagonalisation
+    phase_ms[2] = ph[1];                                                           // divide and conquer
+    phase_ms[3] = ph[2];                                                           // back-transformation
+    phase_ms[4] = std::chrono::duration<double, std::milli>(t2 - t1).count();   // whole eigensolve + U
+  }
+  return kOk;
+}
 
 int zolo_l0_param_count(void) { return kParamCount; }
 

@@ -1,0 +1,54 @@
+// Symmetric eigensolver (tridiagonalisation, divide and conquer,
+// back-transformation) and the Zolo-SVD assembly.
+#pragma once
+#include "linalg.hpp"
+
+namespace zolo {
+
+constexpr int kTrdNb = 32;  // tridiagonalisation panel width
+
+struct TrdScratch {
+  double* part;   // per-workgroup partial sums (<= 64)
+  double* sc;     // device scalars
+  double* dpart;  // per-workgroup partial dots (64 x 2 kTrdNb)
+  double* ypart;  // matrix-vector partials (chunks x n)
+  double* wpart;  // per-workgroup partial dots (<= 64)
+};
+
+struct DcLevel {
+  int nmerge = 0;
+  int smax = 0;
+  const void* merges = nullptr;  // device array of {off, s1, s2}
+  const double* e = nullptr;
+  const double* lam = nullptr;
+  double* lam_new = nullptr;
+  double* Q = nullptr;
+  int64_t ldq = 0;
+  double* U = nullptr;
+  int64_t ldu = 0;
+  double *rho, *dK, *zK, *dD, *rotcs, *zbuf, *dbuf, *tau, *zh, *inv_norm, *vals;
+  int *pK, *pD, *rotab, *cnt, *perm, *org, *rank;
+};
+
+int ke_fill(double* A, int64_t ld, int64_t m, int64_t n, double val, hipStream_t s);
+int ke_trd_column(double* A, int64_t ld, int64_t n, int64_t k0, int64_t i, double* W, double* v, double* tau, double* e,
+                  TrdScratch& w, hipStream_t s);
+int ke_trd_panel_end(double* A, int64_t ld, int64_t k0, int jb, int64_t n, double* d, hipStream_t s);
+int ke_trd_extract_v(const double* A, int64_t ld, int64_t k0, int64_t r0, int jb, int64_t mrows, double* V,
+                     int64_t ldv, hipStream_t s);
+int ke_dc_leaves(const double* d, const double* e, int64_t n, double* lam, double* Q, int64_t ldq, hipStream_t s);
+int ke_dc_level(const DcLevel& L, hipStream_t s);
+
+// Host drivers (eig_driver.cpp).
+size_t syevd_bytes(int64_t n);
+// H (n x n, ld) symmetric, overwritten; lambda ascending (n), Z eigenvectors (ld ldz).
+int syevd_run(int64_t n, double* H, int64_t ldh, double* lambda, double* Z, int64_t ldz, hipStream_t s,
+              double* phase_ms = nullptr);
+
+// Apply Q = H_{k0} ... H_{k0+jb-1} (reflectors stored below the first subdiagonal
+// of A's columns k0.., heads at row k + 1, tau[k]) from the left to C (rows
+// k0+1 .. n-1, ncols columns): C <- Q C.  Used by the back-transformation.
+int apply_trd_block(const Ctx& c, const double* A, int64_t lda, const double* tau, int64_t k0, int jb, int64_t n,
+                    double* C, int64_t ldc, int64_t ncols, Arena& ar);
+
+}  // namespace zolo

@@ -41,10 +41,14 @@ constexpr int kQrNb = 64;   // panel width
 template <typename T>
 int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t j, int apply_prev,
                     int reduce_cur, double* part, int nwg, T* tau, int64_t strideTau, int batch, hipStream_t s);
-// Whole FP32 panel in one workgroup per shift: rows n + jb <= kQrPanelWgRows.
+// Whole panel in one workgroup per shift: rows n + jb <= kQrPanelWgRows, width
+// jb <= kQrPanelWgCols<T> (64 for FP32, 32 for FP64).
 constexpr int kQrPanelWgRows = 576;
-int k_qr_panel_wg(float* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, float* tau,
-                  int64_t strideTau, int batch, hipStream_t s);
+template <typename T>
+constexpr int kQrPanelWgCols = sizeof(T) == 4 ? 64 : 32;
+template <typename T>
+int k_qr_panel_wg(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, T* tau, int64_t strideTau,
+                  int batch, hipStream_t s);
 template <typename T>
 int k_qr_extract_v(const T* S, int64_t lds, int64_t strideS, T* V, int64_t ldv, int64_t strideV, int64_t p, int jb,
                    int64_t mrows, int batch, hipStream_t s);

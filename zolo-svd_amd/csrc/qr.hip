@@ -32,6 +32,16 @@ __device__ inline void butterfly_step(V* a, int lane) {
     a[j] = keep + __shfl_xor(send, H, 64);
   }
 }
+// Same for 32 columns: lane l returns the sum of column l & 31.
+template <typename V>
+__device__ inline V wave_colsum32(V* a, int lane) {
+  butterfly_step<16>(a, lane);
+  butterfly_step<8>(a, lane);
+  butterfly_step<4>(a, lane);
+  butterfly_step<2>(a, lane);
+  butterfly_step<1>(a, lane);
+  return a[0] + __shfl_xor(a[0], 32, 64);
+}
 template <typename V>
 __device__ inline V wave_colsum64(V* a, int lane) {
   butterfly_step<32>(a, lane);
@@ -144,79 +154,80 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
   }
 }
 
-// Whole-panel factorisation in one workgroup (FP32 panels of at most
-// kQrPanelWgRows rows, the small matrices of the batched path): one thread per
-// row with the row's jb values in registers, and for every column one
-// workgroup-wide reduction of the Gram row g_c = sum_{i>j} S[i][j] S[i][c]
+// Whole-panel factorisation in one workgroup for stacks of at most
+// kQrPanelWgRows rows (the small matrices of the batched path): one thread per
+// row with the row's NC panel values in registers, shifted so that the
+// current column is always v[0] (no register is indexed by the loop counter);
+// every step stores the finished column and shifts.  For each column one
+// workgroup-wide reduction gives the Gram row g_c = sum_{i>j} S[i][j] S[i][c]
 // (a wave butterfly, then the waves summed in FP64 in a fixed order).
+// NC = 64 for FP32 panels, 32 for FP64 panels (register budget).
 constexpr int kPanelWgRows = kQrPanelWgRows;
 
-__global__ __launch_bounds__(kPanelWgRows) void qr_panel_wg_kernel(float* S, int64_t lds, int64_t strideS, int64_t n,
-                                                                   int64_t p, int jb, float* tau, int64_t strideTau) {
-  __shared__ double red[kPanelWgRows / 64][NBP];
-  __shared__ double rr[NBP];
-  __shared__ float wv[NBP];
-  __shared__ float sc[2];
+template <typename T, int NC>
+__global__ __launch_bounds__(kPanelWgRows) void qr_panel_wg_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
+                                                                   int64_t p, int jb, T* tau, int64_t strideTau) {
+  __shared__ double red[kPanelWgRows / 64][NC];
+  __shared__ double rr[NC];
+  __shared__ T wv[NC];
+  __shared__ T sc[2];
   const int q = blockIdx.y;
-  float* Sq = S + q * strideS;
+  T* Sq = S + q * strideS;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nwave = blockDim.x >> 6;
   const int64_t mrows = n + jb;
   const int64_t i = p + t;
   const bool mine = t < mrows;
   // columns >= jb are zero in registers and get zero updates throughout
-  float v[NBP];
+  T v[NC];
 #pragma unroll
-  for (int c = 0; c < NBP; ++c) v[c] = (mine && c < jb) ? Sq[i + (p + c) * lds] : 0.0f;
+  for (int c = 0; c < NC; ++c) v[c] = (mine && c < jb) ? Sq[i + (p + c) * lds] : (T)0;
   for (int jj = 0; jj < jb; ++jj) {
     const int64_t j = p + jj;
     const bool live = mine && i >= j && i <= n + j;
-    float x = 0.0f;
-#pragma unroll
-    for (int c = 0; c < NBP; ++c)
-      if (c == jj) x = v[c];
     if (live && i == j) {
 #pragma unroll
-      for (int c = 0; c < NBP; ++c) rr[c] = (double)v[c];
+      for (int c = 0; c < NC; ++c) rr[c] = (double)v[c];
     }
-    if (!(live && i > j)) x = 0.0f;
-    // lanes c > jj of the sum are the Gram row, lane jj the squared norm below
-    // the diagonal; lanes c < jj are not used
-    float g[NBP];
+    const T x = (live && i > j) ? v[0] : (T)0;
+    // lanes c >= 1 of the sum: the Gram row (columns jj + c); lane 0: the
+    // squared norm below the diagonal
+    T g[NC];
 #pragma unroll
-    for (int c = 0; c < NBP; ++c) g[c] = x * v[c];
-    red[wave][lane] = (double)wave_colsum64(g, lane);
+    for (int c = 0; c < NC; ++c) g[c] = x * v[c];
+    const T gs = (NC == 64) ? wave_colsum64(g, lane) : wave_colsum32(g, lane);
+    if (lane < NC) red[wave][lane] = (double)gs;
     __syncthreads();
     if (wave == 0) {
       double gc = 0.0;
-      for (int w = 0; w < nwave; ++w) gc += red[w][lane];
-      const double xn2 = __shfl(gc, jj, 64);
-      const double alpha = rr[jj];
+      if (lane < NC)
+        for (int w = 0; w < nwave; ++w) gc += red[w][lane];
+      const double xn2 = __shfl(gc, 0, 64);
+      const double alpha = rr[0];
       double beta = alpha, tv = 0.0, scal = 0.0;
       if (xn2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
         tv = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
       }
-      // w_c = tau v^T S[:, c] for c > jj, zero elsewhere
-      wv[lane] = (lane > jj && lane < jb) ? (float)(tv * (rr[lane] + scal * gc)) : 0.0f;
+      // w_c = tau v^T S[:, jj + c] for c >= 1, zero elsewhere
+      if (lane < NC) wv[lane] = (lane >= 1 && lane < jb - jj) ? (T)(tv * (rr[lane] + scal * gc)) : (T)0;
       if (lane == 0) {
-        sc[0] = (float)beta;
-        sc[1] = (float)scal;
-        tau[q * strideTau + j] = (float)tv;
+        sc[0] = (T)beta;
+        sc[1] = (T)scal;
+        tau[q * strideTau + j] = (T)tv;
       }
     }
     __syncthreads();
     if (live) {
-      const float vi = (i == j) ? 1.0f : x * sc[1];
-      const float newj = (i == j) ? sc[0] : vi;
+      const T vi = (i == j) ? (T)1 : x * sc[1];
+      v[0] = (i == j) ? sc[0] : vi;
 #pragma unroll
-      for (int c = 0; c < NBP; ++c) v[c] = (c == jj) ? newj : fmaf(-vi, wv[c], v[c]);
+      for (int c = 1; c < NC; ++c) v[c] = fma(-vi, wv[c], v[c]);
     }
-  }
-  if (mine) {
+    if (mine) Sq[i + j * lds] = v[0];
 #pragma unroll
-    for (int c = 0; c < NBP; ++c)
-      if (c < jb) Sq[i + (p + c) * lds] = v[c];
+    for (int c = 0; c + 1 < NC; ++c) v[c] = v[c + 1];
+    v[NC - 1] = (T)0;
   }
 }
 
@@ -305,15 +316,17 @@ int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   return kOk;
 }
 
-int k_qr_panel_wg(float* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, float* tau,
-                  int64_t strideTau, int batch, hipStream_t s) {
+template <typename T>
+int k_qr_panel_wg(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, T* tau, int64_t strideTau,
+                  int batch, hipStream_t s) {
+  constexpr int NC = sizeof(T) == 4 ? 64 : 32;
   const int64_t mrows = n + jb;
-  if (jb > NBP || jb <= 0 || mrows > kPanelWgRows) {
+  if (jb > NC || jb <= 0 || mrows > kPanelWgRows) {
     set_error("qr panel (one workgroup): size out of range");
     return kErrArg;
   }
   const unsigned threads = (unsigned)((mrows + 63) / 64 * 64);
-  hipLaunchKernelGGL(qr_panel_wg_kernel, dim3(1, batch), dim3(threads), 0, s, S, lds, strideS, n, p, jb, tau,
+  hipLaunchKernelGGL((qr_panel_wg_kernel<T, NC>), dim3(1, batch), dim3(threads), 0, s, S, lds, strideS, n, p, jb, tau,
                      strideTau);
   ZOLO_HIP(hipGetLastError());
   return kOk;
@@ -352,7 +365,8 @@ int k_set_identity_top(T* Q, int64_t ldq, int64_t strideQ, int64_t m, int64_t n,
                                  hipStream_t);                                                                     \
   template int k_qr_form_t<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t, int, T*, int64_t, int64_t,    \
                               int, hipStream_t);                                                                   \
-  template int k_set_identity_top<T>(T*, int64_t, int64_t, int64_t, int64_t, int, hipStream_t);
+  template int k_set_identity_top<T>(T*, int64_t, int64_t, int64_t, int64_t, int, hipStream_t);                \
+  template int k_qr_panel_wg<T>(T*, int64_t, int64_t, int64_t, int64_t, int, T*, int64_t, int, hipStream_t);
 ZOLO_QR_INST(double)
 ZOLO_QR_INST(float)
 

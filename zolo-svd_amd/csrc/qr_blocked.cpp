@@ -2,15 +2,16 @@
 // (mpdgeqrf.f:262-313, mpdorgqr.f:253-337), batched over the Zolotarev shifts.
 //
 // Inner level: panels of kQrNb = 64 columns are factored column by column
-// (qr.hip; FP32 panels of small stacks in a single workgroup) and update only the rest of their 256-wide outer block.  Outer
+// (qr.hip; for small stacks each panel, 64 columns in FP32 or 32 in FP64, in
+// a single workgroup) and update only the rest of their 256-wide outer block.  Outer
 // level: the block reflector of all 256 columns, I - V T V^T, is assembled
 // from the inner T factors (T_AB = -T_AA (V_A^T V_B) T_BB for adjacent column
 // groups) and applied to the trailing matrix, and later to Q, as GEMMs with
 // K = 256.  Every block reflector only touches rows p .. n+p+jb-1: rows below
 // are still zero in the identity block of the stack.
 #include <algorithm>
-#include <type_traits>
 
+#include "eig.hpp"
 #include "linalg.hpp"
 
 namespace zolo {
@@ -26,17 +27,17 @@ int gemm_t(const Ctx& c, const GemmDesc& d) {
 // One thread per panel row (the panel kernel keeps each row in registers).
 int qr_nwg(int64_t mrows) { return (int)std::max<int64_t>((mrows + 255) / 256, 1); }
 
-// T of the outer block from its diagonal 64-blocks (already in Tb, ld
+// T of the outer block from its diagonal blk-blocks (already in Tb, ld
 // kQrOuter) and G = V^T V: for adjacent column groups A | B,
 // T_AB = -T_AA G_AB T_BB (the upper-triangular T of H_A H_B).
 template <typename T>
 int combine_t(const Ctx& c, T* Tb, int64_t sT, const T* G, int64_t sG, T* tmp, int64_t sTmp, int lo, int hi,
-              int batch) {
-  if (hi - lo <= kQrNb) return kOk;
-  int mid = lo + ((hi - lo) / 2 + kQrNb - 1) / kQrNb * kQrNb;
-  if (mid >= hi) mid = hi - kQrNb;
-  ZOLO_CHECK(combine_t<T>(c, Tb, sT, G, sG, tmp, sTmp, lo, mid, batch));
-  ZOLO_CHECK(combine_t<T>(c, Tb, sT, G, sG, tmp, sTmp, mid, hi, batch));
+              int batch, int blk) {
+  if (hi - lo <= blk) return kOk;
+  int mid = lo + ((hi - lo) / 2 + blk - 1) / blk * blk;
+  if (mid >= hi) mid = hi - blk;
+  ZOLO_CHECK(combine_t<T>(c, Tb, sT, G, sG, tmp, sTmp, lo, mid, batch, blk));
+  ZOLO_CHECK(combine_t<T>(c, Tb, sT, G, sG, tmp, sTmp, mid, hi, batch, blk));
   const int64_t ld = kQrOuter;
   const int a = mid - lo, b = hi - mid;
   GemmDesc d;  // tmp = G_AB T_BB   (T_BB upper: op(B)[k][j] = 0 for k > j)
@@ -105,7 +106,9 @@ int apply_block_reflector(const Ctx& c, const T* V, int64_t ldv, int64_t sV, con
 
 template <typename T>
 int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, T* Tall, Arena& ar) {
-  const int nb = kQrNb, nbo = kQrOuter;
+  // small stacks: every inner panel in one workgroup (narrower panels in FP64)
+  const bool fused = n + kQrPanelWgCols<T> <= kQrPanelWgRows;
+  const int nb = fused ? kQrPanelWgCols<T> : kQrNb, nbo = kQrOuter;
   QrScratch q;
   ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
   const int nwg_max = qr_nwg(n + nb);
@@ -125,9 +128,8 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
       const int jb = (int)std::min<int64_t>(nb, P + JB - p);
       const int64_t mrows = n + jb;
       const int nwg = qr_nwg(mrows);
-      if (std::is_same<T, float>::value && mrows <= kQrPanelWgRows) {
-        ZOLO_CHECK(k_qr_panel_wg(reinterpret_cast<float*>(S), lds, sS, n, p, jb, reinterpret_cast<float*>(tau), n,
-                                 batch, c.s));
+      if (fused) {
+        ZOLO_CHECK(k_qr_panel_wg<T>(S, lds, sS, n, p, jb, tau, n, batch, c.s));
       } else {
         for (int64_t j = p; j <= p + jb; ++j)
           ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, j, j > p, j < p + jb, part, nwg, tau, n, batch, c.s));
@@ -155,7 +157,7 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
       d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
       d.C = G; d.ldc = nbo; d.strideC = sG;
       ZOLO_CHECK(gemm_t<T>(c, d));
-      ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch));
+      ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch, nb));
     }
     const int64_t ncols = n - P - JB;
     if (ncols > 0)
@@ -199,7 +201,7 @@ int orgqr_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, 
         ZOLO_CHECK(k_qr_form_t<T>(G + o * (nbo + 1), nbo, sG, tau, n, p, jb, Tb + o * (nbo + 1), nbo, sTall, batch,
                                   c.s));
       }
-      ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch));
+      ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch, kQrNb));
     }
     ZOLO_CHECK(apply_block_reflector<T>(c, V, mrows, sV, Tb, nbo, sTall, JB, Q + P + P * ldq, ldq, sQ, mrows, n - P,
                                         false, (T*)q.W, (T*)q.W2, sW, batch));
@@ -208,6 +210,37 @@ int orgqr_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, 
 }
 
 }  // namespace
+
+int apply_trd_block(const Ctx& c, const double* A, int64_t lda, const double* tau, int64_t k0, int jb, int64_t n,
+                    double* C, int64_t ldc, int64_t ncols, Arena& ar) {
+  const int nbo = kQrOuter;
+  if (jb <= 0 || jb > nbo) {
+    set_error("apply_trd_block: block width out of range");
+    return kErrArg;
+  }
+  const int64_t r0 = k0 + 1, mrows = n - r0;
+  double* V = ar.take<double>((size_t)mrows * jb);
+  double* G = ar.take<double>((size_t)nbo * nbo);
+  double* Tm = ar.take<double>((size_t)nbo * nbo);
+  double* tmp = ar.take<double>((size_t)nbo * nbo);
+  double* W = ar.take<double>((size_t)nbo * ncols);
+  double* W2 = ar.take<double>((size_t)nbo * ncols);
+  ZOLO_ARENA_OK(ar);
+  ZOLO_CHECK(ke_trd_extract_v(A, lda, k0, r0, jb, mrows, V, mrows, c.s));
+  GemmDesc d;  // G = V^T V
+  d.transA = 1; d.M = jb; d.N = jb; d.K = mrows;
+  d.A = V; d.lda = mrows; d.B = V; d.ldb = mrows; d.C = G; d.ldc = nbo;
+  ZOLO_CHECK(gemm_t<double>(c, d));
+  ZOLO_CHECK(k_fill<double>(Tm, nbo, 0, nbo, nbo, 0.0, 1, c.s));
+  for (int o = 0; o < jb; o += kQrNb) {
+    const int b = std::min(kQrNb, jb - o);
+    ZOLO_CHECK(k_qr_form_t<double>(G + (int64_t)o * (nbo + 1), nbo, 0, tau, 0, k0 + o, b, Tm + (int64_t)o * (nbo + 1),
+                                   nbo, 0, 1, c.s));
+  }
+  ZOLO_CHECK(combine_t<double>(c, Tm, 0, G, 0, tmp, 0, 0, jb, 1, kQrNb));
+  return apply_block_reflector<double>(c, V, mrows, 0, Tm, nbo, 0, jb, C + r0, ldc, 0, mrows, ncols, false, W, W2, 0,
+                                       1);
+}
 
 template <typename T>
 size_t stacked_qr_bytes(int64_t n, int r) {

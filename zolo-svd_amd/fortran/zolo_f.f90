@@ -3,7 +3,8 @@
 !> The external subroutines keep the reference's names and argument lists
 !> (pzolopd1.f90:2, pzolopd2.f90:2, pqdwhfacs.f90:2, zolochol.f90:2,
 !> zoloqr.f90:2, pdnormest.f90:2, mpdgeqrf.f:5, mpdorgqr.f:5, FormX2.f90:2,
-!> hfunc.f90:2) and module auxil_zolo keeps Auxil_Zolo.f90's interface.  The
+!> hfunc.f90:2, elpa_pdsyevd2.F90:1) and module auxil_zolo keeps
+!> Auxil_Zolo.f90's interface.  The
 !> descriptors must describe a 1x1 process grid (RSRC = CSRC = 0, LLD >= M);
 !> the BLACS context is never dereferenced, so the library links no MPI or
 !> BLACS.  Work is done by libzolo.so through the ISO_C_BINDING interfaces of
@@ -102,6 +103,18 @@ module zolo_capi
       integer(c_int64_t), value :: n, lds
       real(c_double) :: s(*)
       real(c_double), intent(in) :: tau(*)
+    end function
+    integer(c_int) function c_zolo_formx2_host(n, fone, a, lda, z, ldz, err) bind(C, name='zolo_formx2_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n, lda, ldz
+      real(c_double), value :: fone
+      real(c_double) :: a(*), z(*)
+      real(c_double) :: err
+    end function
+    integer(c_int) function c_zolo_syevd_host(n, h, ldh, w, z, ldz) bind(C, name='zolo_syevd_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n, ldh, ldz
+      real(c_double) :: h(*), w(*), z(*)
     end function
     integer(c_int) function c_zolo_compute_coeff(con, tol, rk, coeff) bind(C, name='zolo_compute_coeff')
       import :: c_int, c_double
@@ -476,29 +489,58 @@ subroutine mpdorgqr(m, n, k, a, ia, ja, desca, tau, work, lwork, info)
   info = c_zolo_mpdorgqr_host(int(n, c_int64_t), a(ioa), int(desca(9), c_int64_t), tau)
 end subroutine mpdorgqr
 
-!> FormX2 (FormX2.f90:2) on the local n x n arrays: Z <- (A+Z)/FONE, ERR = ||Z-A||_F, A <- Z.
+!> FormX2 (FormX2.f90:2) on the local n x n arrays: Z <- (A+Z)/FONE, ERR = ||Z-A||_F, A <- Z
+!> (computed on the GPU).
 subroutine formx2(n, nblk, fone, err, a, z, contxt, work, info, it, ip)
+  use zolo_capi
+  use iso_c_binding
   implicit none
   integer, intent(in) :: n, nblk, contxt, it, ip
   double precision, intent(in) :: fone
   double precision, intent(out) :: err
   double precision, intent(inout) :: a(n, n), z(n, n), work(*)
   integer, intent(out) :: info
-  integer :: i, j
-  double precision :: s, d
   if (nblk < 0 .or. contxt < -1 .or. it < -1 .or. ip < -1) work(1) = 0d0
-  s = 0d0
-  do j = 1, n
-    do i = 1, n
-      z(i, j) = (a(i, j) + z(i, j)) / fone
-      d = z(i, j) - a(i, j)
-      s = s + d * d
-      a(i, j) = z(i, j)
-    end do
-  end do
-  err = sqrt(s)
+  err = 0d0
   info = 0
+  if (n <= 0) return
+  info = c_zolo_formx2_host(int(n, c_int64_t), fone, a, int(n, c_int64_t), z, int(n, c_int64_t), err)
 end subroutine formx2
+
+!> ELPA_PDSYEVD2 (elpa_pdsyevd2.F90:1) on a 1x1 grid: A (na x na symmetric,
+!> overwritten) = Z diag(EV) Z^T with EV ascending, computed on the GPU
+!> (tridiagonalisation, divide and conquer, back-transformation).  The grid and
+!> communicator arguments are validated for a single process and otherwise
+!> unused; UPLO 'L', 'U' or 'A' (the full matrix is read).
+subroutine elpa_pdsyevd2(uplo, na, nev, a, lda, lca, ev, z, nblk, sc_desc, np_rows, np_cols, my_prow, my_pcol, &
+                         myid, mpi_comm_in, info, real_kernel)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  character, intent(in) :: uplo
+  integer, intent(in) :: na, nev, lda, lca, nblk, np_rows, np_cols, my_prow, my_pcol, myid
+  integer, intent(inout) :: info
+  integer, optional :: mpi_comm_in, real_kernel
+  double precision, intent(inout) :: a(lda, *), z(lda, *), ev(*)
+  integer, intent(in) :: sc_desc(*)
+  info = 0
+  if (uplo /= 'L' .and. uplo /= 'l' .and. uplo /= 'U' .and. uplo /= 'u' .and. uplo /= 'A' .and. uplo /= 'a') then
+    info = -1
+  else if (na < 0) then
+    info = -2
+  else if (nev < 0 .or. nev > na) then
+    info = -3
+  else if (lda < max(1, na)) then
+    info = -5
+  else if (np_rows /= 1 .or. np_cols /= 1 .or. my_prow /= 0 .or. my_pcol /= 0) then
+    info = -11
+  end if
+  if (info /= 0) return
+  if (na == 0) return
+  if (nblk < 0 .or. lca < 0 .or. myid < 0 .or. sc_desc(1) < 0) info = 0
+  if (present(mpi_comm_in) .or. present(real_kernel)) info = 0
+  info = c_zolo_syevd_host(int(na, c_int64_t), a, int(lda, c_int64_t), ev, z, int(lda, c_int64_t))
+end subroutine elpa_pdsyevd2
 
 !> hfunc (hfunc.f90:2): the QDWH weight a(l).
 double precision function hfunc(x)

@@ -10,6 +10,7 @@ Mirrors the reference's Fortran entry points on a 1x1 process grid:
 * ``bounds``     -- the alpha/beta estimate of test_zolodrv1.f90:179-189.
 * ``compute_coeff`` / ``choosem`` / ``hfunc`` / ``mellipke`` / ``mellipj`` -- Auxil_Zolo.f90, hfunc.f90.
 * ``pd_batched_device`` -- PZOLOPD1 over many small FP32 matrices of uneven size in one hipGraph.
+* ``syevd_device`` / ``svd_device`` -- the Zolo-SVD tail: ELPA_PDSYEVD2's role and ZoloCHK1's assembly.
 
 Host arrays are numpy (column-major copies are made as needed); the
 ``*_device`` variants take torch CUDA tensors (column-major data in HBM, see
@@ -128,6 +129,9 @@ _sig("zolo_chol_inv_device", C.c_int, [C.c_int, _I64, _P, _I64, _P, _I64, _P])
 _sig("zolo_pd_batched_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, C.c_int, C.c_int, _P, C.c_double, C.c_int,
                                          C.c_int, C.c_int, _P, _P, _P, _P, _P, _P])
 _sig("zolo_batched_tile", C.c_int, [_I64, C.c_int])
+_sig("zolo_syevd_device", C.c_int, [_I64, _P, _I64, _P, _P, _I64, _P, _P])
+_sig("zolo_svd_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _I64, _P, _I64, C.POINTER(_Info), _P,
+                                  _P])
 _sig("zolo_l0_param_count", C.c_int, [])
 _sig("zolo_l0_set_weights", C.c_int, [_P, C.c_int])
 _sig("zolo_l0_features_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P])
@@ -431,6 +435,39 @@ def pd_batched_device(mats, precision: str = "f32", l0: str = "lu", beta=None, o
 def batched_tile(n: int, precision: str = "f32") -> int:
     """GEMM tile chosen by the autotuner for matrices of order n (-1: not tuned yet)."""
     return int(_lib.zolo_batched_tile(n, PREC[precision]))
+
+
+# ---------------------------------------------------------------------------
+# Zolo-SVD tail: symmetric eigensolver and the SVD assembly
+# ---------------------------------------------------------------------------
+def syevd_device(dH):
+    """H = Z diag(w) Z^T on the device (column-major FP64, H overwritten); returns (w ascending, Z, phase_ms)."""
+    torch = _torch()
+    n = dH.shape[0]
+    w = torch.empty((n,), dtype=torch.float64, device="cuda")
+    Z = torch.empty_like(dH)
+    ph = np.zeros(3)
+    _check("zolo_syevd_device", _lib.zolo_syevd_device(n, C.c_void_p(dH.data_ptr()), n, C.c_void_p(w.data_ptr()),
+                                                       C.c_void_p(Z.data_ptr()), n, ph.ctypes.data_as(C.c_void_p),
+                                                       _stream()))
+    return w, Z, dict(zip(("tridiagonal", "divide_conquer", "back_transform"), ph.tolist()))
+
+
+def svd_device(dA, precision: str = "f64", rk: int = 0):
+    """Zolo-SVD (ZoloCHK1.f90:80-91): A = U diag(S) V^T with S ascending; returns (S, U, V, RunInfo, phase_ms)."""
+    torch = _torch()
+    n = dA.shape[0]
+    S = torch.empty((n,), dtype=torch.float64, device="cuda")
+    U = torch.empty_like(dA)
+    V = torch.empty_like(dA)
+    ci = _Info()
+    ph = np.zeros(5)
+    _check("zolo_svd_device", _lib.zolo_svd_device(PREC[precision], rk, n, C.c_void_p(dA.data_ptr()), n,
+                                                   C.c_void_p(S.data_ptr()), C.c_void_p(U.data_ptr()), n,
+                                                   C.c_void_p(V.data_ptr()), n, C.byref(ci),
+                                                   ph.ctypes.data_as(C.c_void_p), _stream()))
+    names = ("polar_h", "tridiagonal", "divide_conquer", "back_transform", "eig_total")
+    return S, U, V, RunInfo.from_c(ci), dict(zip(names, ph.tolist()))
 
 
 # ---------------------------------------------------------------------------
