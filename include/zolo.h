@@ -140,7 +140,12 @@ int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* L
  * per process), 0 automatic, 1..4 fixed (128x128, 64x128, 128x64, 64x64).
  * use_graph = 0 launches eagerly.  Per-matrix outputs (host arrays, each may be
  * NULL): beta_out, orth_out, iters_out, flags_out (bit 0: fallback used,
- * bit 1: check failed or no positive bound). */
+ * bit 1: check failed or no positive bound, bit 2: restarted).  A matrix
+ * still off the orthogonality target after its planned steps (the CON
+ * recurrence assumes exact arithmetic; near CON ~ 1/u the first step is not
+ * that accurate) restarts from its current iterate, which has the same polar
+ * factor, with the LU bound of that iterate (as the restart of
+ * pzolopd2.f90:397-405), at most twice. */
 typedef struct zolo_batch_info {
   int count;             /* matrices                                              */
   int buckets;           /* size buckets (n rounded up to a multiple of 32)       */
@@ -148,9 +153,10 @@ typedef struct zolo_batch_info {
   int failures;          /* matrices failing the orthogonality check at the end   */
   int max_iters;         /* largest iteration count                               */
   int graph_nodes;       /* nodes of the captured graphs                          */
+  int continued;         /* matrices restarted from their iterate (see below)     */
   double time_ms;        /* wall time of the call                                 */
   double phase_ms[8];    /* 0 alpha + beta, 1 plan, 2 tile choice, 3 iterations,
-                            4 fallback, 5 learned features + network              */
+                            4 fallback and restarts, 5 learned features + network */
 } zolo_batch_info_t;
 int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* const* U,
                            const int64_t* ldu, int precision, int l0_source, const double* beta_in, double orth_tol,

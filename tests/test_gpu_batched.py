@@ -113,6 +113,27 @@ def test_batched_fallback_on_overestimated_bound(zl, torch_gpu):
             assert info.beta[k] == pytest.approx(beta[k], rel=1e-12)
 
 
+def test_batched_restart_near_singular(zl, torch_gpu):
+    """A numerically singular FP32 matrix: CON ~ 6e15, where the plan is one
+    FP64 QR step (the CON recurrence reports convergence after it) but that
+    step leaves singular values in [0.12, 2.5] (the FP64 oracle shows the same);
+    the matrix restarts from its iterate and meets the target."""
+    rng = np.random.default_rng(23)
+    n = 200
+    A = (zo.haar(n, rng) * np.geomspace(1.0, 1e-3, n)) @ zo.haar(n, rng).T
+    A[:, 1] = A[:, 0]
+    A[7, 0] = 0.0
+    A[7, 1] = 2e-13
+    A32 = A.astype(np.float32)
+    mats = [A32, _batch(seed=29, sizes=[64])[0]]
+    Us, info, _, _ = _run(zl, mats)
+    assert 1.0 / info.beta[0] > 1e15
+    assert zo.batched_plan(info.beta[0])[1] == "Q"
+    assert info.continued[0] and not info.continued[1]
+    assert not info.failed.any()
+    _check_polar(A32, Us[0])
+
+
 def test_batched_argument_errors(zl, torch_gpu):
     import torch
     good = zl.to_device(np.eye(8, dtype=np.float32))

@@ -55,11 +55,12 @@ struct BatchOptions {
   int rk = 0;                      // Zolotarev degree override (0: choosem per matrix)
   int tile = -1;                   // GEMM tile: -1 autotune per bucket, 0 automatic, 1..4 fixed
   bool use_graph = true;           // capture each phase into one hipGraph
+  int continue_rounds = 2;         // restarts of matrices still off after their planned steps
 };
 
 struct BatchResult {
   std::vector<double> alpha, beta, orth, err;
-  std::vector<int> iters, rk, fallback, failed;
+  std::vector<int> iters, rk, fallback, failed, continued;
   size_t graph_nodes = 0;
   int buckets = 0;
   double phase_ms[8] = {};         // 0 alpha + beta, 1 plan, 2 tile choice, 3 iterations, 4 fallback

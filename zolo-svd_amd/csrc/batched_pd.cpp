@@ -450,6 +450,7 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
   res.rk.assign(count, 0);
   res.fallback.assign(count, 0);
   res.failed.assign(count, 0);
+  res.continued.assign(count, 0);
 
   // A graph cannot be captured on the legacy default stream: work on the
   // library's own stream after the caller's default-stream work has drained.
@@ -696,6 +697,7 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
 
   // ---- phase 4: fallback to the LU estimate ------------------------------------
   const auto t5 = clk::now();
+  // (a fallback call runs its own continuation; the merge keeps its counts)
   if (opt.l0_source != 0) {
     std::vector<int> redo;
     for (int i = 0; i < count; ++i)
@@ -722,12 +724,45 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
         res.iters[i] += r2.iters[k];
         res.rk[i] = r2.rk[k];
         res.failed[i] = r2.failed[k];
+        res.continued[i] = r2.continued[k];
       }
       res.graph_nodes += r2.graph_nodes;
     }
   }
-  for (int i = 0; i < count; ++i)
-    if (!(res.orth[i] <= opt.orth_tol)) res.failed[i] = 1;
+  // ---- phase 5: continuation --------------------------------------------------
+  // The CON recurrence assumes exact arithmetic; at CON near 1/u the first step
+  // leaves singular values far from 1 although the recurrence says CON = 1 (the
+  // reference's ERR test then simply runs more steps).  A matrix still off after
+  // its planned steps restarts from its current iterate, which has the same
+  // polar factor, with the LU bound of that iterate (the restart of
+  // pzolopd2.f90:397-405).
+  for (int round = 0; round < opt.continue_rounds; ++round) {
+    std::vector<int> redo;
+    for (int i = 0; i < count; ++i)
+      if (!(res.orth[i] <= opt.orth_tol) && std::isfinite(res.orth[i]) && res.iters[i] > 0) redo.push_back(i);
+    if (redo.empty()) break;
+    std::vector<int64_t> rn, rld;
+    std::vector<const float*> rA;
+    std::vector<float*> rU;
+    for (int i : redo) {
+      rn.push_back(n[i]); rA.push_back(U[i]); rU.push_back(U[i]); rld.push_back(ldu[i]);
+    }
+    BatchOptions oc = opt;
+    oc.l0_source = 0;
+    oc.beta_in = nullptr;
+    oc.continue_rounds = 0;
+    BatchResult r3;
+    ZOLO_CHECK(pd_batched_run((int)redo.size(), rn.data(), rA.data(), rld.data(), rU.data(), rld.data(), oc, r3, s));
+    for (size_t k = 0; k < redo.size(); ++k) {
+      const int i = redo[k];
+      res.continued[i] += 1;
+      res.orth[i] = r3.orth[k];
+      res.err[i] = r3.err[k];
+      res.iters[i] += r3.iters[k];
+    }
+    res.graph_nodes += r3.graph_nodes;
+  }
+  for (int i = 0; i < count; ++i) res.failed[i] = (res.orth[i] <= opt.orth_tol) ? 0 : 1;
   res.phase_ms[4] = ms_since(t5);
   res.time_ms = ms_since(t_start);
   return kOk;

@@ -464,12 +464,13 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
     return st;
   }
   if (st) return st;
-  int fb = 0, fail = 0, mx = 0;
+  int fb = 0, fail = 0, mx = 0, cont = 0;
   for (int i = 0; i < count; ++i) {
+    cont += res.continued[i] ? 1 : 0;
     if (beta_out) beta_out[i] = res.beta[i];
     if (orth_out) orth_out[i] = res.orth[i];
     if (iters_out) iters_out[i] = res.iters[i];
-    if (flags_out) flags_out[i] = (res.fallback[i] ? 1 : 0) | (res.failed[i] ? 2 : 0);
+    if (flags_out) flags_out[i] = (res.fallback[i] ? 1 : 0) | (res.failed[i] ? 2 : 0) | (res.continued[i] ? 4 : 0);
     fb += res.fallback[i] ? 1 : 0;
     fail += res.failed[i] ? 1 : 0;
     mx = std::max(mx, res.iters[i]);
@@ -479,6 +480,7 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
     info->count = count;
     info->buckets = res.buckets;
     info->fallbacks = fb;
+    info->continued = cont;
     info->failures = fail;
     info->max_iters = mx;
     info->graph_nodes = (int)res.graph_nodes;

@@ -363,7 +363,8 @@ def chol_inv_device(dA, dLi, precision: str = "f64"):
 class _BatchInfo(C.Structure):
     _fields_ = [
         ("count", C.c_int), ("buckets", C.c_int), ("fallbacks", C.c_int), ("failures", C.c_int),
-        ("max_iters", C.c_int), ("graph_nodes", C.c_int), ("time_ms", C.c_double), ("phase_ms", C.c_double * 8),
+        ("max_iters", C.c_int), ("graph_nodes", C.c_int), ("continued", C.c_int), ("time_ms", C.c_double),
+        ("phase_ms", C.c_double * 8),
     ]
 
 
@@ -379,6 +380,7 @@ class BatchRunInfo:
     iters: np.ndarray
     fallback: np.ndarray
     failed: np.ndarray
+    continued: np.ndarray
     buckets: int
     graph_nodes: int
     time_ms: float
@@ -427,6 +429,7 @@ def pd_batched_device(mats, precision: str = "f32", l0: str = "lu", beta=None, o
                                      ptr(it_out), ptr(fl_out), C.byref(ci), _stream())
     _check("zolo_pd_batched_device", st)
     info = BatchRunInfo(beta=b_out, orth=o_out, iters=it_out, fallback=(fl_out & 1) != 0, failed=(fl_out & 2) != 0,
+                        continued=(fl_out & 4) != 0,
                         buckets=ci.buckets, graph_nodes=ci.graph_nodes, time_ms=ci.time_ms,
                         phase_ms={p: ci.phase_ms[i] for i, p in enumerate(BATCH_PHASES)})
     return outs, info
