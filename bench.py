@@ -23,7 +23,7 @@ learned network (when zolo/l0_weights.json exists), PDs per second of the
 whole call with the inputs in HBM.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--precision bf16x3|f32|f64]
-                       [--small-count 4096] [--no-small] [--svd N]
+                       [--small-count 4096] [--no-small] [--svd 8192 (0 = skip)]
 """
 import argparse
 import json
@@ -224,7 +224,11 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 / FP32 comparison runs")
     ap.add_argument("--small-count", type=int, default=4096, help="matrices of the batched small-PD workload")
     ap.add_argument("--no-small", action="store_true", help="skip the batched small-PD workload")
-    ap.add_argument("--svd", type=int, default=0, help="also time the Zolo-SVD tail at this n (0: skip)")
+    ap.add_argument("--svd", type=int, default=8192, help="also time the Zolo-SVD tail at this n (0: skip)")
+    ap.add_argument("--part", default="all", choices=["all", "small", "svd"],
+                    help="run only the batched small-PD or the Zolo-SVD section (for profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="start with one 1 GiB device-to-device copy (a known-bytes kernel for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the polar-decomposition path is built and measured on a single "
@@ -236,6 +240,21 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").normal_()
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.part != "all":
+        out = {"part": args.part}
+        if args.part == "small":
+            out["small_pds"] = small_pds(torch, zl, args.small_count, args.seed)
+        else:
+            out["zolo_svd"] = svd_tail(torch, zl, args.svd, args.seed)
+        print(json.dumps(out), flush=True)
+        return
     n = args.n
     log(f"generating workload n={n}")
     A, dA, Ug, Vg = make_workload(torch, n, args.seed)

@@ -38,6 +38,8 @@ typedef struct zolo_info {
   int r;                  /* Zolotarev degree (number of shifts)                       */
   int info;               /* first non-positive pivot column + 1 of a shifted Cholesky */
   int polish_passes;      /* FP64 polish passes executed (mixed precision only)         */
+  int restarts;           /* PZOLOPD2 restarts: bounds re-estimated when CON reached 1
+                             before ERR met TOL2 (pzolopd2.f90:397-405)                  */
   char schedule[16];      /* per iteration: 'Q' QR step, 'C' FP64 Cholesky, 'c' low prec*/
   double err[16];         /* ERR = ||X+ - X||_F after each FormX2                      */
   double con[16];         /* CON at the start of each iteration                        */

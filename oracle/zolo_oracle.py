@@ -23,6 +23,8 @@ How each function is pinned (checked by ``tests/test_oracle.py``):
   by a direct LAPACK solve, and against each other.
 * ``pzolopd1`` / ``pqdwhfacs`` -- against the closed form U V^T of the
   generator A = U diag(s) V^T and against scipy.linalg.polar.
+* ``pzolopd2`` -- the same closed form, on a numerically singular matrix
+  where CON collapses to 1 after the first step and the restart is taken.
 * ``zolo_svd`` -- A = U diag(S) V^T with S the generator's singular values
   (closed form), U and V orthogonal.
 * ``batched_plan`` -- by applying the planned steps to diagonal matrices whose
@@ -384,6 +386,7 @@ class ZoloLog:
     maxc: list = field(default_factory=list)
     tol2: float = 0.0
     iters: int = 0
+    restarts: int = 0
 
 
 def pzolopd1(A: np.ndarray, alpha: float, beta: float, rk: int, sym: str = "N",
@@ -449,6 +452,51 @@ def batched_plan(beta: float, rk: int = 0, max_it: int = 6) -> tuple[int, str]:
         if con - 1.0 <= BATCH_CONVERGED:
             break
     return r, sched
+
+
+def pzolopd2(A: np.ndarray, alpha: float, beta: float, rk: int, sym: str = "N",
+             max_it: int = 12) -> tuple[np.ndarray, ZoloLog]:
+    """PZOLOPD2 (pzolopd2.f90:386-487): PZOLOPD1 with IT < 12 and a restart.
+
+    When the CON recurrence reaches exactly 1 while ERR is still above TOL2 the
+    reference calls FormXTwo (pzolopd2.f90:480-486), which it never defines;
+    restated here as: re-estimate alpha, beta of the current iterate
+    (test_zolodrv1.f90:179-189), X <- X/(alpha beta), CON <- 1/beta, continue.
+    log.restarts counts them.
+    """
+    n = A.shape[0]
+    X = A / (alpha * beta)
+    con = 1.0 / beta
+    tol1 = 10.0 * DLAMCH_P / 2.0
+    tol2 = math.exp(math.log(tol1) / (2.0 * rk + 1.0))
+    log = ZoloLog(tol2=tol2)
+    it, err = 0, 1.0
+    while err > tol2 and it < max_it:
+        coeff = compute_coeff(con, DLAMCH_P, rk)
+        maxc = float(coeff[:2 * rk].max())
+        log.con.append(con)
+        log.maxc.append(maxc)
+        Z = np.zeros_like(X)
+        if (it == 0 or con > 1e3) and maxc > 3e2:
+            log.schedule.append("QR")
+            for ip in range(1, rk + 1):
+                Z += zoloqr_step(X, coeff, rk, ip)
+        else:
+            log.schedule.append("CHOL")
+            for ip in range(1, rk + 1):
+                Z += zolochol_step(X, coeff, rk, ip, sym)
+        fone, con = zolo_fone_con(con, coeff, rk)
+        X, err = formx2(X, Z, fone)
+        log.err.append(err)
+        it += 1
+        if con == 1.0 and err > tol2 and it < max_it:
+            ax, bx = bounds_estimate(X)
+            if 0.0 < bx < 1.0:
+                X = X / (ax * bx)
+                con = 1.0 / bx
+                log.restarts += 1
+    log.iters = it
+    return X, log
 
 
 @dataclass

@@ -117,3 +117,24 @@ def test_argument_errors(zl):
     with pytest.raises(zl.ZoloError) as e:
         zl.pzolopd1(A, 1.0, 2.0, 2)
     assert e.value.status == -6
+
+
+def test_pzolopd2_restart_matches_oracle(zl):
+    """PZOLOPD2 restart (pzolopd2.f90:397-405): CON collapses to 1 after the first
+    step of a numerically singular matrix; both the GPU and the oracle restart
+    from the iterate with re-estimated bounds and reach the SVD polar factor."""
+    rng = np.random.default_rng(23)
+    n = 120
+    A = (zo.haar(n, rng) * np.geomspace(1.0, 1e-3, n)) @ zo.haar(n, rng).T
+    A[:, 1] = A[:, 0]
+    A[7, 0] = 0.0
+    A[7, 1] = 2e-14
+    al, be = zo.bounds_estimate(A)
+    rk = zo.choosem(1.0 / be)
+    Uo, lo = zo.pzolopd2(A, al, be, rk)
+    U, H, info = zl.pzolopd2(A, al, be, rk)
+    assert lo.restarts >= 1 and info.restarts == lo.restarts
+    assert info.schedule == "".join("Q" if x == "QR" else "C" for x in lo.schedule)
+    W, s, Vt = np.linalg.svd(A)
+    assert np.linalg.norm(U - W @ Vt) / np.sqrt(n) < 1e-12
+    assert zo.orthogonality(U)[0] < 1e-13

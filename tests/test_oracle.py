@@ -348,3 +348,26 @@ def test_zolo_svd_closed_form():
     assert np.allclose(S, np.sort(s), atol=1e-13)
     assert np.linalg.norm(A - (U * S) @ V.T) / np.linalg.norm(A) < 1e-14
     assert np.linalg.norm(U.T @ U - np.eye(n)) < 1e-13 and np.linalg.norm(V.T @ V - np.eye(n)) < 1e-13
+
+
+def _near_singular(n=120, delta=2e-14, seed=23):
+    rng = np.random.default_rng(seed)
+    A = (zo.haar(n, rng) * np.geomspace(1.0, 1e-3, n)) @ zo.haar(n, rng).T
+    A[:, 1] = A[:, 0]
+    A[7, 0] = 0.0
+    A[7, 1] = delta
+    return A
+
+
+def test_pzolopd2_restart_closed_form():
+    """CON collapses to 1 after the first step (beta ~ 1e-16) while ERR is large:
+    PZOLOPD2 restarts from the iterate and reaches the SVD polar factor."""
+    A = _near_singular()
+    al, be = zo.bounds_estimate(A)
+    assert 1.0 / be > 5e15
+    rk = zo.choosem(1.0 / be)
+    U2, log2 = zo.pzolopd2(A, al, be, rk)
+    assert log2.restarts >= 1
+    W, s, Vt = np.linalg.svd(A)
+    assert np.linalg.norm(U2 - W @ Vt) / np.sqrt(A.shape[0]) < 1e-12
+    assert zo.orthogonality(U2)[0] < 1e-14

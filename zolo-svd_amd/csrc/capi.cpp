@@ -130,7 +130,7 @@ int zolo_pd2_host(int precision, char sym, int rk, int64_t n, double alpha, doub
   if (H) ZOLO_CHECK(dH.alloc(n));
   ZOLO_CHECK(dA.upload(A, lda, n));
   const int st = zolo_pd_run(precision, sym, rk, n, alpha, beta, dA.p, n, dU.p, n, H ? dH.p : nullptr, n, info,
-                             nullptr, 12);
+                             nullptr, 12, true);
   if (st < 0) return st;
   ZOLO_HIP(hipDeviceSynchronize());
   ZOLO_CHECK(dU.download(U, ldu, n));

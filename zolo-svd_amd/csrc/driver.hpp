@@ -7,7 +7,7 @@ namespace zolo {
 
 int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A, int64_t lda,
                 double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s,
-                int max_it = 6);
+                int max_it = 6, bool restart = false);
 int qdwh_run(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu, double* H,
              int64_t ldh, zolo_info_t* info, hipStream_t s);
 int form_h(const Ctx& c, const double* U, int64_t ldu, const double* A, int64_t lda, int64_t n, double* H,

@@ -92,6 +92,7 @@ __global__ __launch_bounds__(kT) void features_kernel(const BatchMat* mats, cons
   __shared__ double u[kMaxBatchN];
   __shared__ double w[kMaxBatchN];
   __shared__ double Y[kSketch][kMaxBatchN];
+  __shared__ double Om[kSketch][kMaxBatchN];
   __shared__ double ta[kLanczos], tb[kLanczos], theta[kLanczos];
   __shared__ double sh[4];
   const int g = blockIdx.x;
@@ -110,21 +111,29 @@ __global__ __launch_bounds__(kT) void features_kernel(const BatchMat* mats, cons
     for (int j = 0; j < n; ++j) s += fabs(x(i, j));
     rmax = fmax(rmax, s);
   }
-  for (int j = t; j < n; j += kT) {
+  // column sums: one wave per column, lanes over rows (coalesced)
+  const int lane = t & 63, wave = t >> 6;
+  for (int j = wave; j < n; j += kT / 64) {
     double s = 0.0;
-    for (int i = 0; i < n; ++i) s += fabs(x(i, j));
+    for (int i = lane; i < n; i += 64) s += fabs(x(i, j));
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     cmax = fmax(cmax, s);
   }
   const double ninf = block_max(rmax, sh);
   const double n1 = block_max(cmax, sh);
 
-  // sketch Y = X Omega and the diagonal of its R factor (modified Gram-Schmidt)
+  // sketch Y = X Omega and the diagonal of its R factor (modified Gram-Schmidt);
+  // Omega is generated once per matrix into LDS
+  for (int j = t; j < n; j += kT)
+#pragma unroll
+    for (int k = 0; k < kSketch; ++k) Om[k][j] = gauss((uint32_t)(j * kSketch + k));
+  __syncthreads();
   for (int i = t; i < n; i += kT) {
     double acc[kSketch] = {0.0, 0.0, 0.0, 0.0};
     for (int j = 0; j < n; ++j) {
       const double xv = x(i, j);
 #pragma unroll
-      for (int k = 0; k < kSketch; ++k) acc[k] += xv * gauss((uint32_t)(j * kSketch + k));
+      for (int k = 0; k < kSketch; ++k) acc[k] += xv * Om[k][j];
     }
 #pragma unroll
     for (int k = 0; k < kSketch; ++k) Y[k][i] = acc[k];
@@ -159,10 +168,11 @@ __global__ __launch_bounds__(kT) void features_kernel(const BatchMat* mats, cons
       u[i] = s;
     }
     __syncthreads();
-    for (int j = t; j < n; j += kT) {  // w = X^T u
+    for (int j = wave; j < n; j += kT / 64) {  // w = X^T u, one wave per column
       double s = 0.0;
-      for (int i = 0; i < n; ++i) s += x(i, j) * u[i];
-      w[j] = s;
+      for (int i = lane; i < n; i += 64) s += x(i, j) * u[i];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) w[j] = s;
     }
     __syncthreads();
     double a = 0.0;

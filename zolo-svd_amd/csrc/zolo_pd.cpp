@@ -97,7 +97,8 @@ int form_h(const Ctx& c, const double* U, int64_t ldu, const double* A, int64_t 
 // Zolo-PD
 // ---------------------------------------------------------------------------
 int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A, int64_t lda,
-                double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s, int max_it) {
+                double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s, int max_it,
+                bool restart) {
   if (precision < 0 || precision > 2) return -1;
   if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -2;
   if (rk < 1 || rk > 8) return -3;
@@ -119,6 +120,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
   const int64_t nn = n * n;
   // Workspace: scalars | X64 | Z64 (r) | [mixed: X32, Z32 (r), Xref64, polish] | step scratch
   size_t step64 = std::max(qr_step_bytes<double>(n, rk), chol_step_bytes<double>(n, rk));
+  if (restart) step64 = std::max(step64, bounds_bytes(n));
   size_t bytes = kScalarBytes + sizeof(double) * (size_t)(1 + rk) * nn + step64 + 4096;
   if (mixed) bytes += sizeof(float) * (size_t)(1 + rk) * nn + sizeof(double) * nn + polish_bytes(n) +
                       chol_step_bytes<float>(n, rk);
@@ -221,6 +223,23 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
     err = std::sqrt(e2);
     if (it < 16) info->err[it] = err;
     ++it;
+    // PZOLOPD2's restart (pzolopd2.f90:397-405, 480-486): CON has reached 1
+    // while ERR is still above TOL2, i.e. the scalar bound no longer describes
+    // the iterate (first steps at CON near 1/u).  The reference hands this to
+    // FormXTwo, which it never defines; here the bounds of the current iterate
+    // are re-estimated (alpha, beta as test_zolodrv1.f90:179-189) and the
+    // iteration continues from X / (alpha beta) with CON = 1/beta.
+    if (restart && !low_active && con == 1.0 && err > tol2 && it < max_it) {
+      ar.reset();
+      ar.take<char>(mark);
+      double ax = 0.0, bx = 0.0;
+      ZOLO_CHECK(bounds_run(0, n, X, n, &ax, &bx, ar, s));
+      if (bx > 0.0 && bx < 1.0) {
+        ZOLO_CHECK(k_scale_copy<double, double>(X, n, X, n, n, n, 1.0 / (ax * bx), s));
+        con = 1.0 / bx;
+        ++info->restarts;
+      }
+    }
     // Mixed precision: once the Zolotarev bound says the low-precision iterate
     // has converged to its working precision (CON - 1 below the FP32 unit
     // roundoff), a further low-precision step cannot improve it; the FP64

@@ -15,7 +15,7 @@ module zolo_capi
   private :: c_int, c_double, c_int64_t, c_char, c_ptr
 
   type, bind(C) :: zolo_info_t
-    integer(c_int) :: iters, r, info, polish_passes
+    integer(c_int) :: iters, r, info, polish_passes, restarts
     character(kind=c_char) :: schedule(16)
     real(c_double) :: err(16), con(16), maxc(16)
     real(c_double) :: alpha, beta, time_ms

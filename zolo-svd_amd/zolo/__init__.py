@@ -55,7 +55,7 @@ class ZoloError(RuntimeError):
 
 class _Info(C.Structure):
     _fields_ = [
-        ("iters", C.c_int), ("r", C.c_int), ("info", C.c_int), ("polish_passes", C.c_int),
+        ("iters", C.c_int), ("r", C.c_int), ("info", C.c_int), ("polish_passes", C.c_int), ("restarts", C.c_int),
         ("schedule", C.c_char * 16),
         ("err", C.c_double * 16), ("con", C.c_double * 16), ("maxc", C.c_double * 16),
         ("alpha", C.c_double), ("beta", C.c_double), ("time_ms", C.c_double),
@@ -81,6 +81,7 @@ class RunInfo:
     phase_ms: dict = field(default_factory=dict)
     polish_passes: int = 0
     info: int = 0
+    restarts: int = 0
 
     @staticmethod
     def from_c(ci: _Info) -> "RunInfo":
@@ -88,7 +89,7 @@ class RunInfo:
         return RunInfo(iters=k, r=ci.r, schedule=ci.schedule.decode()[:k], err=list(ci.err[:k]),
                        con=list(ci.con[:k]), maxc=list(ci.maxc[:k]), alpha=ci.alpha, beta=ci.beta,
                        time_ms=ci.time_ms, phase_ms={p: ci.phase_ms[i] for i, p in enumerate(PHASES)},
-                       polish_passes=ci.polish_passes, info=ci.info)
+                       polish_passes=ci.polish_passes, info=ci.info, restarts=ci.restarts)
 
 
 _P = C.c_void_p
@@ -116,6 +117,8 @@ _sig("zolo_pd_device", C.c_int, [C.c_int, C.c_char, C.c_int, _I64, C.c_double, C
                                  _I64, C.POINTER(_Info), _P])
 _sig("zolo_pd_host", C.c_int, [C.c_int, C.c_char, C.c_int, _I64, C.c_double, C.c_double, _D, _I64, _D, _I64, _D,
                                _I64, C.POINTER(_Info)])
+_sig("zolo_pd2_host", C.c_int, [C.c_int, C.c_char, C.c_int, _I64, C.c_double, C.c_double, _D, _I64, _D, _I64, _D,
+                                _I64, C.POINTER(_Info)])
 _sig("zolo_bounds_device", C.c_int, [C.c_int, _I64, _P, _I64, _D, _D, _P])
 _sig("zolo_qdwh_device", C.c_int, [C.c_char, _I64, C.c_double, _P, _I64, _P, _I64, _P, _I64, C.POINTER(_Info), _P])
 _sig("zolo_qdwh_host", C.c_int, [C.c_char, _I64, C.c_double, _D, _I64, _D, _I64, _D, _I64, C.POINTER(_Info)])
@@ -227,6 +230,22 @@ def pzolopd1(A, alpha: float, beta: float, rk: int, sym: str = "N", polar: bool 
     st = _lib.zolo_pd_host(PREC[precision], sym.encode()[:1], rk, n, alpha, beta, _dptr(A), n, _dptr(U), n,
                            _dptr(H) if polar else None, n, C.byref(ci))
     _check("zolo_pd_host", st)
+    return U, H, RunInfo.from_c(ci)
+
+
+def pzolopd2(A, alpha: float, beta: float, rk: int, sym: str = "N", polar: bool = True,
+             precision: str = "f64"):
+    """PZOLOPD2 on the GPU: PZOLOPD1 with the iteration cap of 12 and the restart
+    of pzolopd2.f90:397-405 (bounds of the iterate re-estimated when CON reaches
+    1 before ERR meets TOL2).  Returns (U, H or None, RunInfo)."""
+    A = _fortran(A)
+    n = A.shape[0]
+    U = np.zeros((n, n), order="F")
+    H = np.zeros((n, n), order="F") if polar else None
+    ci = _Info()
+    st = _lib.zolo_pd2_host(PREC[precision], sym.encode()[:1], rk, n, alpha, beta, _dptr(A), n, _dptr(U), n,
+                            _dptr(H) if polar else None, n, C.byref(ci))
+    _check("zolo_pd2_host", st)
     return U, H, RunInfo.from_c(ci)
 
 
