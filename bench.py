@@ -47,7 +47,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(torch, n, seed):
+def make_workload(torch, n, seed, factors="haar"):
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
 
@@ -56,8 +56,19 @@ def make_workload(torch, n, seed):
         Q, R = torch.linalg.qr(G)
         return Q * torch.sign(torch.diagonal(R))
 
-    U = haar()
-    V = haar()
+    def reflector():
+        # product of two reflections I - 2 Y (Y^T Y)^{-1} Y^T across random 256-dimensional
+        # subspaces: dense and orthogonal, built from GEMMs and a 256 x 256 host inverse only
+        Q = None
+        for _ in range(2):
+            Y = torch.randn(n, 256, dtype=torch.float64, device="cuda", generator=g)
+            Minv = torch.linalg.inv((Y.T @ Y).cpu()).to("cuda")
+            P = torch.eye(n, dtype=torch.float64, device="cuda") - 2.0 * (Y @ Minv) @ Y.T
+            Q = P if Q is None else Q @ P
+        return Q
+
+    U = haar() if factors == "haar" else reflector()
+    V = haar() if factors == "haar" else reflector()
     m = n - n // 20
     s = torch.full((n,), 1e-8, dtype=torch.float64, device="cuda")
     s[:m] = 10.0 ** (-8.0 * torch.arange(m, dtype=torch.float64, device="cuda") / max(m - 1, 1))
@@ -227,6 +238,9 @@ def main():
     ap.add_argument("--svd", type=int, default=8192, help="also time the Zolo-SVD tail at this n (0: skip)")
     ap.add_argument("--part", default="all", choices=["all", "small", "svd"],
                     help="run only the batched small-PD or the Zolo-SVD section (for profiling)")
+    ap.add_argument("--factors", default="haar", choices=["haar", "reflector"],
+                    help="singular-vector factors of the n=16384 workload: Haar (torch QR) or products of two "
+                         "random reflections (GEMMs only, for counter runs where the QR library is unavailable)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="start with one 1 GiB device-to-device copy (a known-bytes kernel for counter scaling)")
     args = ap.parse_args()
@@ -257,7 +271,7 @@ def main():
         return
     n = args.n
     log(f"generating workload n={n}")
-    A, dA, Ug, Vg = make_workload(torch, n, args.seed)
+    A, dA, Ug, Vg = make_workload(torch, n, args.seed, args.factors)
     dU = torch.empty_like(dA)
     dH = torch.empty_like(dA)
     torch.cuda.synchronize()
@@ -324,7 +338,8 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "f64" else f"f64+{args.precision}",
-        "data": "synthetic (Haar U, V seeded on the GPU; s geometric 1..1e-8 with 5% at 1e-8)",
+        "data": f"synthetic ({'Haar' if args.factors == 'haar' else 'reflection-product'} U, V seeded on the GPU; "
+                "s geometric 1..1e-8 with 5% at 1e-8)",
         "config": {"workload": f"Zolo-PD {args.precision} n={n}, kappa=1e8 with 5% cluster at 1e-8",
                    "n": n, "r": info.r, "precision": args.precision, "schedule": info.schedule,
                    "polish_passes": info.polish_passes, "parallelism": "single GPU"},
