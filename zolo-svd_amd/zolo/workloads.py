@@ -1,4 +1,7 @@
-"""Synthetic matrices of the batched workload, generated on the GPU with torch.
+"""Synthetic matrices of the workloads, generated on the GPU with torch.
+
+``square``: one FP64 matrix A = U diag(s) V^T with Haar U, V and the spectra
+of the full-size workloads (geometric, log-uniform, clustered, sine).
 
 Batched polar decomposition workload: independent square FP32 matrices,
 n drawn uniformly from {nmin..nmax}, A = U diag(s) V^T with U, V Haar
@@ -37,6 +40,54 @@ def batch(count: int, seed: int, nmin: int = 32, nmax: int = 512, logk=(1.0, 12.
         for j, i in enumerate(idx):
             mats[i] = At[j]
     return mats, ns, lk
+
+
+def spectrum(kind: str, n: int, kappa: float, g):
+    """Singular values (descending, FP64 CUDA tensor) of the square workloads.
+
+    geometric: 1 -> 1/kappa; loguniform: s_1 = 1, s_n = 1/kappa pinned, the
+    others 10^(-log10(kappa) u), u ~ U(0,1); cluster: geometric 1 -> 1/kappa
+    over the first n - n//20 values, the last n//20 exactly 1/kappa; sine:
+    1 + 0.5 sin(k) + 1e-3 N(0,1), k = 1..n.
+    """
+    import torch
+    dev = "cuda"
+    t = torch.arange(n, dtype=torch.float64, device=dev)
+    if kind == "geometric":
+        return torch.pow(kappa, -t / max(n - 1, 1))
+    if kind == "loguniform":
+        s = torch.pow(kappa, -torch.rand(n, dtype=torch.float64, device=dev, generator=g))
+        s[0], s[-1] = 1.0, 1.0 / kappa
+        return torch.sort(s, descending=True).values
+    if kind == "cluster":
+        m = n - n // 20
+        s = torch.full((n,), 1.0 / kappa, dtype=torch.float64, device=dev)
+        s[:m] = torch.pow(kappa, -t[:m] / max(m - 1, 1))
+        return s
+    if kind == "sine":
+        return 1.0 + 0.5 * torch.sin(t + 1.0) + 1e-3 * torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    raise ValueError(kind)
+
+
+def square(n: int, kind: str, kappa: float, seed: int):
+    """A = U diag(s) V^T (FP64, on the GPU) with Haar U, V.
+
+    Returns (A, dA, U, s, V): A row-major as torch holds it, dA the
+    column-major buffer the library takes (the row-major buffer of A^T).
+    """
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+
+    def haar():
+        Q, R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+        return Q * torch.sign(torch.diagonal(R))
+
+    U = haar()
+    V = haar()
+    s = spectrum(kind, n, kappa, g)
+    A = (U * s) @ V.T.contiguous()
+    return A, A.T.contiguous(), U, s, V
 
 
 def log_cond(mats, workers: int = 16) -> np.ndarray:
