@@ -110,6 +110,11 @@ __device__ inline f4 mfma16(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Workgroup barrier that first waits for this wave's LDS operations only:
+// global loads in flight stay in flight across it.  The memory clobber keeps
+// the compiler from moving LDS accesses across the barrier.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 struct TileCoord { int64_t i0, j0; bool skip; };
 
 // Map the linear workgroup id to a (row, col) tile, XCD-aware and grouped.
@@ -274,50 +279,62 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
     }
   };
 
+  auto compute = [&](int buf, int ks) {
+    const int kr = ks + (lane >> 4);
+    T av[FM], bv[FN];
+#pragma unroll
+    for (int m = 0; m < FM; ++m) av[m] = As[buf][kr][wm * (FM * 16) + m * 16 + (lane & 15)];
+#pragma unroll
+    for (int n = 0; n < FN; ++n) bv[n] = Bs[buf][kr][wn * (FN * 16) + n * 16 + (lane & 15)];
+#pragma unroll
+    for (int m = 0; m < FM; ++m)
+#pragma unroll
+      for (int n = 0; n < FN; ++n) acc[m][n] = mfma16(bv[n], av[m], acc[m][n]);
+  };
+
   if (kb < ke) {
-    // Two LDS stages: tile k+1 is staged into the other buffer while tile k
-    // feeds the MFMAs, so each K step needs a single barrier.
+    // Two LDS stages and a register stage: while slab k feeds the MFMAs from
+    // one LDS buffer, slab k+1 (loaded into registers during slab k-1) is
+    // written to the other buffer right after the first k-step, and the global
+    // loads of slab k+2 are issued behind it.  The loads thus have a whole slab
+    // of MFMAs to land, and the single barrier per slab waits for LDS traffic
+    // only, not for the loads in flight.
     load_tiles(kb);
     store_tiles(0);
-    __syncthreads();
+    if (kb + BK < ke) load_tiles(kb + BK);
+    lds_barrier();
     int buf = 0;
     for (int64_t k0 = kb; k0 < ke; k0 += BK) {
-      const bool more = (k0 + BK) < ke;
-      if (more) load_tiles(k0 + BK);
-#pragma unroll
-      for (int ks = 0; ks < BK; ks += 4) {
-        const int kr = ks + (lane >> 4);
-        T av[FM], bv[FN];
-#pragma unroll
-        for (int m = 0; m < FM; ++m) av[m] = As[buf][kr][wm * (FM * 16) + m * 16 + (lane & 15)];
-#pragma unroll
-        for (int n = 0; n < FN; ++n) bv[n] = Bs[buf][kr][wn * (FN * 16) + n * 16 + (lane & 15)];
-#pragma unroll
-        for (int m = 0; m < FM; ++m)
-#pragma unroll
-          for (int n = 0; n < FN; ++n) acc[m][n] = mfma16(av[m], bv[n], acc[m][n]);
+      compute(buf, 0);
+      if (k0 + BK < ke) {
+        store_tiles(buf ^ 1);
+        if (k0 + 2 * BK < ke) load_tiles(k0 + 2 * BK);
       }
-      if (more) store_tiles(buf ^ 1);
-      __syncthreads();
+#pragma unroll
+      for (int ks = 4; ks < BK; ks += 4) compute(buf, ks);
+      lds_barrier();
       buf ^= 1;
     }
   }
 
+  // The MFMAs compute C^T tiles (B fragment as the row operand), so each lane
+  // holds one row of C and the 16 lanes of a row group cover 16 consecutive
+  // rows of a column: stores and the beta loads are 128-byte (FP64) or
+  // 64-byte (FP32) contiguous segments.  Lane layout of the 16x16x4 outputs:
+  // column 4*v + lane/16 (f64) or 4*(lane/16) + v (f32), row lane%16.
   const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
   const T al = T(alpha), be = T(d.beta);
 #pragma unroll
   for (int m = 0; m < FM; ++m) {
+    const int64_t row = i0 + wm * (FM * 16) + m * 16 + (lane & 15);
+    if (row >= d.M) continue;
 #pragma unroll
     for (int n = 0; n < FN; ++n) {
-      const int64_t col = j0 + wn * (FN * 16) + n * 16 + (lane & 15);
-      if (col >= d.N) continue;
-      const int64_t rowb = i0 + wm * (FM * 16) + m * 16;
+      const int64_t colb = j0 + wn * (FN * 16) + n * 16;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        // Output lane layout: f32 16x16x4 gives lane rows 4*(lane/16)+v,
-        // f64 16x16x4 interleaves them as 4*v + lane/16 (column = lane%16).
-        const int64_t row = rowb + ((sizeof(T) == 8) ? (4 * v + (lane >> 4)) : ((lane >> 4) * 4 + v));
-        if (row < d.M) {
+        const int64_t col = colb + ((sizeof(T) == 8) ? (4 * v + (lane >> 4)) : ((lane >> 4) * 4 + v));
+        if (col < d.N) {
           T* p = C + row + col * d.ldc;
           const T val = al * acc[m][n][v];
           *p = (d.beta == 0.0) ? val : val + be * (*p);
