@@ -15,8 +15,10 @@ def _errs(A, U, H):
     return zo.backward_error(A, U, H), zo.orthogonality(U)[0]
 
 
+# n > 544 takes the factored QR step (X = Q_X R once, then [R; s I] per shift)
 @pytest.mark.parametrize("n,kind,kappa,seed", [(64, "geometric", 1e6, 1), (200, "loguniform", 1e12, 2),
-                                               (150, "cluster", 1e8, 3), (128, "sine", 0, 4)])
+                                               (150, "cluster", 1e8, 3), (128, "sine", 0, 4),
+                                               (700, "loguniform", 1e12, 11), (600, "cluster", 1e8, 12)])
 def test_pzolopd1_f64_matches_oracle(zl, n, kind, kappa, seed):
     A, Ug, s, Vg = zo.make_matrix(n, kind, kappa, seed)
     alpha, beta = zo.bounds_estimate(A)
@@ -41,7 +43,8 @@ def test_pzolopd1_f64_matches_oracle(zl, n, kind, kappa, seed):
 
 
 @pytest.mark.parametrize("prec", ["f32", "bf16x3"])
-@pytest.mark.parametrize("n,kind,kappa,seed", [(256, "cluster", 1e8, 5), (200, "loguniform", 1e12, 6)])
+@pytest.mark.parametrize("n,kind,kappa,seed", [(256, "cluster", 1e8, 5), (200, "loguniform", 1e12, 6),
+                                               (650, "cluster", 1e8, 13)])
 def test_pzolopd1_mixed(zl, prec, n, kind, kappa, seed):
     A, Ug, s, Vg = zo.make_matrix(n, kind, kappa, seed)
     alpha, beta = zo.bounds_estimate(A)

@@ -331,7 +331,8 @@ int zolo_qr_step_host(int64_t n, const double* X, int64_t ldx, double* Z, int64_
   const double hs[2] = {std::sqrt(c), -coeff[2 * rk + ip - 1] / std::sqrt(c)};
   ZOLO_HIP(hipMemcpy(dsc, hs, sizeof(hs), hipMemcpyHostToDevice));
   Ctx cx;
-  ZOLO_CHECK(qr_step<double>(cx, dX.p, n, n, 1, dsc, dsc + 1, dZ.p, n, n * n, ar));
+  int nz = 1;
+  ZOLO_CHECK(qr_step<double>(cx, dX.p, n, n, 1, dsc, dsc + 1, dZ.p, n, n * n, ar, &nz));
   ZOLO_HIP(hipDeviceSynchronize());
   ZOLO_CHECK(dZ.download(Z, ldz, n));
   return kOk;

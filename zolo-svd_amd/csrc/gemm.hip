@@ -7,9 +7,10 @@
 //
 //  * F64    : v_mfma_f64_16x16x4f64, 128x128x16 tile, 4 waves of 64x64.
 //  * F32    : v_mfma_f32_16x16x4f32, same tiling.
-//  * BF16X3 : fp32 operands split at LDS staging into hi + lo bfloat16 halves,
-//             three v_mfma_f32_16x16x32_bf16 products (hi*hi + hi*lo + lo*hi)
-//             accumulated in fp32; 128x128x32 tile.
+//  * BF16X3 : fp32 operands split once per call into hi + lo bfloat16 halves
+//             (a tiled layout whose K slabs are contiguous), three
+//             v_mfma_f32_16x16x32_bf16 products (lo*hi + hi*lo + hi*hi)
+//             accumulated in fp32; 256x128x32 tile of 8 waves.
 //
 // Work distribution: tiles are numbered so that the 8 XCDs (round-robin
 // workgroup dispatch) each own a contiguous block of tiles, and inside a block
@@ -92,7 +93,7 @@ int ledger_query(int prec, int ta, int tb, int tm, int tn, int64_t* count, doubl
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int BM = 128, BN = 128;   // bf16x3 tile
+constexpr int BM = 128, BN = 128;   // smallest tile of the grid-size check
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -354,22 +355,92 @@ __device__ inline unsigned short bf16_rn(float x) {
 }
 __device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-template <int TA, int TB>
-__global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmDesc d) {
-  constexpr int BK = 32;
-  constexpr int LDK = BK + 8;  // row pitch in bf16 elements (80 bytes)
-  __shared__ unsigned short Ah[BM][LDK], Al[BM][LDK], Bh[BN][LDK], Bl[BN][LDK];
+constexpr int kSplitTile = 64;
+constexpr int kB3K = 32;            // K depth of one bf16x3 slab and of a layout tile
+constexpr int kB3Rows = 128;        // rows of a layout tile
+constexpr int kB3Pitch = kB3K + 8;  // LDS row pitch in bf16 elements (80 bytes: conflict-free 16-byte reads)
+constexpr int kB3M = 256, kB3N = 128, kB3Threads = 512;  // workgroup tile: 8 waves of 64 x 64
 
-  const TileCoord tc = tile_of<BM, BN>(d.M, d.N, d.lowerC);
+// Offset of element (r, k) of a split operand in its tiled layout: tiles of
+// 128 rows x 32 k (8 KB of bf16, row-major inside), ordered k-fastest within a
+// 128-row block, so the K slabs of one row block are consecutive in memory.
+__device__ inline int64_t b3_offset(int64_t r, int64_t k, int64_t Kp) {
+  return ((r / kB3Rows) * (Kp / kB3K) + k / kB3K) * (kB3Rows * kB3K) + (r % kB3Rows) * kB3K + (k % kB3K);
+}
+
+// Split an fp32 operand into bf16 hi and lo halves (x = hi + lo to 16 bits) in
+// the tiled layout, zero outside R x K (rows padded to Rp, depth to Kp).
+// Source element (r, k) is src[k + r * ld] when KMAJ, else src[r + k * ld].
+// Batch item z reads operand map ? map[z] : z (stride 0: one shared operand).
+template <int KMAJ>
+__global__ __launch_bounds__(kThreads) void bf16_split_kernel(const float* __restrict__ src, int64_t ld,
+                                                              int64_t stride, const int* map, int64_t R, int64_t K,
+                                                              int64_t Kp, unsigned short* __restrict__ hi,
+                                                              unsigned short* __restrict__ lo, int64_t ostride) {
+  __shared__ float tile[kSplitTile][kSplitTile + 1];
+  const int64_t z = blockIdx.z;
+  const float* S = src + (map ? (int64_t)map[z] : z) * stride;
+  const int64_t k0 = (int64_t)blockIdx.x * kSplitTile, r0 = (int64_t)blockIdx.y * kSplitTile;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int it = 0; it < kSplitTile * kSplitTile / kThreads; ++it) {
+    const int idx = t + it * kThreads;
+    const int rr = KMAJ ? idx / kSplitTile : idx % kSplitTile;
+    const int kk = KMAJ ? idx % kSplitTile : idx / kSplitTile;
+    const int64_t r = r0 + rr, k = k0 + kk;
+    float v = 0.f;
+    if (r < R && k < K) v = KMAJ ? S[k + r * ld] : S[r + k * ld];
+    tile[rr][kk] = v;
+  }
+  __syncthreads();
+  // thread: one row, 16 consecutive k (inside one 32-wide layout tile) -> two 16-byte stores per half
+  const int rr = t >> 2, kk = (t & 3) * 16;
+  unsigned int ph[8], pl[8];
+#pragma unroll
+  for (int e = 0; e < 16; e += 2) {
+    const float x0 = tile[rr][kk + e], x1 = tile[rr][kk + e + 1];
+    const unsigned short h0 = bf16_rn(x0), h1 = bf16_rn(x1);
+    const unsigned short l0 = bf16_rn(x0 - bf16_to_f(h0)), l1 = bf16_rn(x1 - bf16_to_f(h1));
+    ph[e / 2] = (unsigned int)h0 | ((unsigned int)h1 << 16);
+    pl[e / 2] = (unsigned int)l0 | ((unsigned int)l1 << 16);
+  }
+  const int64_t o = z * ostride + b3_offset(r0 + rr, k0 + kk, Kp);
+  uint4* H = reinterpret_cast<uint4*>(hi + o);
+  uint4* L = reinterpret_cast<uint4*>(lo + o);
+  H[0] = make_uint4(ph[0], ph[1], ph[2], ph[3]);
+  H[1] = make_uint4(ph[4], ph[5], ph[6], ph[7]);
+  L[0] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+  L[1] = make_uint4(pl[4], pl[5], pl[6], pl[7]);
+}
+
+// BF16x3 GEMM on pre-split operands in the tiled layout: op(A) rows (hi/lo,
+// padded to 256-row multiples) and op(B) columns (padded to 128).  256 x 128
+// workgroup tile of 8 waves (each 64 x 64), K slabs of 32 double-buffered in
+// LDS with a register stage (slab k+2 loads in flight while slab k computes),
+// one LDS-only barrier per slab.  Per 16x16 block: three
+// v_mfma_f32_16x16x32_bf16 (lo*hi + hi*lo + hi*hi), fp32 accumulation.
+__global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, const unsigned short* __restrict__ Ahi,
+                                                                 const unsigned short* __restrict__ Alo, int64_t sAs,
+                                                                 const unsigned short* __restrict__ Bhi,
+                                                                 const unsigned short* __restrict__ Blo, int64_t sBs,
+                                                                 int64_t Kp) {
+  __shared__ __attribute__((aligned(16))) unsigned short LA[2][2][kB3M][kB3Pitch];  // [stage][hi lo][row][k]
+  __shared__ __attribute__((aligned(16))) unsigned short LB[2][2][kB3N][kB3Pitch];
+
+  const TileCoord tc = tile_of<kB3M, kB3N>(d.M, d.N, d.lowerC);
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;
-  const float* __restrict__ A = reinterpret_cast<const float*>(d.A) + (d.mapA ? (int64_t)d.mapA[z] : z) * d.strideA;
-  const float* __restrict__ B = reinterpret_cast<const float*>(d.B) + (d.mapB ? (int64_t)d.mapB[z] : z) * d.strideB;
   float* __restrict__ C = reinterpret_cast<float*>(d.C) + (d.mapC ? (int64_t)d.mapC[z] : z) * d.strideC;
+  const unsigned short* __restrict__ Ah = Ahi + z * sAs;
+  const unsigned short* __restrict__ Al = Alo + z * sAs;
+  const unsigned short* __restrict__ Bh = Bhi + z * sBs;
+  const unsigned short* __restrict__ Bl = Blo + z * sBs;
 
   int64_t kb, ke;
-  k_range<BM, BN>(d, i0, j0, kb, ke);
+  k_range<kB3M, kB3N>(d, i0, j0, kb, ke);
+  kb = (kb / kB3K) * kB3K;
+  ke = min((ke + kB3K - 1) / kB3K * kB3K, Kp);
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -381,146 +452,91 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmDesc d) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
 
-  float ra[16], rb[16];
-  const bool interior = (i0 + BM <= d.M) && (j0 + BN <= d.N);
-  auto load_fast = [&](int64_t k0) {
-    if (TA == 0) {
-      const float* src = A + (i0 + (t & 7) * 16) + (k0 + (t >> 3)) * d.lda;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) ra[e] = src[e];
-    } else {
-      const float* src = A + (k0 + (t & 1) * 16) + (i0 + (t >> 1)) * d.lda;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) ra[e] = src[e];
-    }
-    if (TB == 0) {
-      const float* src = B + (k0 + (t & 1) * 16) + (j0 + (t >> 1)) * d.ldb;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) rb[e] = src[e];
-    } else {
-      const float* src = B + (j0 + (t & 7) * 16) + (k0 + (t >> 3)) * d.ldb;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) rb[e] = src[e];
-    }
-  };
+  // A slab: 2 layout tiles (1024 chunks of 16 B per half), B slab: 1 tile (512
+  // chunks); thread t takes A chunks t and t + 512 and B chunk t.  Chunk e of a
+  // tile is row e / 4, k (e % 4) * 8.
+  uint4 ra[2][2], rb[2];
+  const int64_t tiles_k = Kp / kB3K;
   auto load_tiles = [&](int64_t k0) {
-    if (interior && k0 + BK <= ke) {
-      load_fast(k0);
-      return;
+    const int64_t kt = k0 / kB3K;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int64_t o = ((i0 / kB3Rows + c) * tiles_k + kt) * (kB3Rows * kB3K) + (int64_t)t * 8;
+      ra[0][c] = *reinterpret_cast<const uint4*>(Ah + o);
+      ra[1][c] = *reinterpret_cast<const uint4*>(Al + o);
     }
-    if (TA == 0) {  // thread: k = t>>3, 16 consecutive i
-      const int kk = t >> 3, ii = (t & 7) * 16;
-      const int64_t k = k0 + kk;
-      const bool kin = k < ke;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int64_t i = i0 + ii + e;
-        ra[e] = (kin && i < d.M) ? A[i + k * d.lda] : 0.f;
-      }
-    } else {  // thread: i = t>>1, 16 consecutive k
-      const int ii = t >> 1, kk = (t & 1) * 16;
-      const int64_t i = i0 + ii;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int64_t k = k0 + kk + e;
-        ra[e] = (i < d.M && k < ke) ? A[k + i * d.lda] : 0.f;
-      }
-    }
-    if (TB == 0) {  // thread: j = t>>1, 16 consecutive k
-      const int jj = t >> 1, kk = (t & 1) * 16;
-      const int64_t j = j0 + jj;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int64_t k = k0 + kk + e;
-        rb[e] = (j < d.N && k < ke) ? B[k + j * d.ldb] : 0.f;
-      }
-    } else {  // thread: k = t>>3, 16 consecutive j
-      const int kk = t >> 3, jj = (t & 7) * 16;
-      const int64_t k = k0 + kk;
-      const bool kin = k < ke;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int64_t j = j0 + jj + e;
-        rb[e] = (kin && j < d.N) ? B[j + k * d.ldb] : 0.f;
-      }
-    }
+    const int64_t o = ((j0 / kB3Rows) * tiles_k + kt) * (kB3Rows * kB3K) + (int64_t)t * 8;
+    rb[0] = *reinterpret_cast<const uint4*>(Bh + o);
+    rb[1] = *reinterpret_cast<const uint4*>(Bl + o);
   };
-  auto split_store = [&](unsigned short (*H)[LDK], unsigned short (*L)[LDK], int r, int c, float x) {
-    const unsigned short h = bf16_rn(x);
-    H[r][c] = h;
-    L[r][c] = bf16_rn(x - bf16_to_f(h));
+  const int lrow = t >> 2, lk = (t & 3) * 8;
+  auto store_tiles = [&](int b) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      *reinterpret_cast<uint4*>(&LA[b][0][c * kB3Rows + lrow][lk]) = ra[0][c];
+      *reinterpret_cast<uint4*>(&LA[b][1][c * kB3Rows + lrow][lk]) = ra[1][c];
+    }
+    *reinterpret_cast<uint4*>(&LB[b][0][lrow][lk]) = rb[0];
+    *reinterpret_cast<uint4*>(&LB[b][1][lrow][lk]) = rb[1];
   };
-  auto store_tiles = [&]() {
-    if (TA == 0) {
-      const int kk = t >> 3, ii = (t & 7) * 16;
+  auto compute = [&](int b) {
+    const int kc = (lane >> 4) * 8;
+    bf8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) split_store(Ah, Al, ii + e, kk, ra[e]);
-    } else {
-      const int ii = t >> 1, kk = (t & 1) * 16;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) split_store(Ah, Al, ii, kk + e, ra[e]);
+    for (int m = 0; m < 4; ++m) {
+      const int r = wm * 64 + m * 16 + (lane & 15);
+      ah[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LA[b][0][r][kc]));
+      al[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LA[b][1][r][kc]));
     }
-    if (TB == 0) {
-      const int jj = t >> 1, kk = (t & 1) * 16;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) split_store(Bh, Bl, jj, kk + e, rb[e]);
-    } else {
-      const int kk = t >> 3, jj = (t & 7) * 16;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) split_store(Bh, Bl, jj + e, kk, rb[e]);
+    for (int n = 0; n < 4; ++n) {
+      const int r = wn * 64 + n * 16 + (lane & 15);
+      bh[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LB[b][0][r][kc]));
+      bl[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LB[b][1][r][kc]));
     }
+    // B fragment as the row operand: the accumulators hold C^T blocks, so the
+    // epilogue writes contiguous column segments
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[n], ah[m], acc[m][n], 0, 0, 0);
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[n], al[m], acc[m][n], 0, 0, 0);
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[n], ah[m], acc[m][n], 0, 0, 0);
+      }
   };
 
   if (kb < ke) {
     load_tiles(kb);
-    store_tiles();
-    __syncthreads();
-    for (int64_t k0 = kb; k0 < ke; k0 += BK) {
-      const bool more = (k0 + BK) < ke;
-      if (more) load_tiles(k0 + BK);
-      const int kc = (lane >> 4) * 8;
-      bf8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int r = wm * 64 + m * 16 + (lane & 15);
-        ah[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Ah[r][kc]));
-        al[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Al[r][kc]));
+    store_tiles(0);
+    if (kb + kB3K < ke) load_tiles(kb + kB3K);
+    lds_barrier();
+    int buf = 0;
+    for (int64_t k0 = kb; k0 < ke; k0 += kB3K) {
+      compute(buf);
+      if (k0 + kB3K < ke) {
+        store_tiles(buf ^ 1);
+        if (k0 + 2 * kB3K < ke) load_tiles(k0 + 2 * kB3K);
       }
-#pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        const int r = wn * 64 + n * 16 + (lane & 15);
-        bh[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Bh[r][kc]));
-        bl[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&Bl[r][kc]));
-      }
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < 4; ++n) {
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[m], bh[n], acc[m][n], 0, 0, 0);
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bl[n], acc[m][n], 0, 0, 0);
-          acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bh[n], acc[m][n], 0, 0, 0);
-        }
-      __syncthreads();
-      if (more) {
-        store_tiles();
-        __syncthreads();
-      }
+      lds_barrier();
+      buf ^= 1;
     }
   }
 
+  // lane layout of the 16x16x32 outputs (C^T blocks): row lane%16, column 4*(lane/16) + v
   const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
   const float alf = float(alpha), be = float(d.beta);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
+    const int64_t row = i0 + wm * 64 + m * 16 + (lane & 15);
+    if (row >= d.M) continue;
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      const int64_t col = j0 + wn * 64 + n * 16 + (lane & 15);
-      if (col >= d.N) continue;
-      const int64_t rowb = i0 + wm * 64 + m * 16 + (lane >> 4) * 4;
+      const int64_t colb = j0 + wn * 64 + n * 16 + (lane >> 4) * 4;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t row = rowb + v;
-        if (row < d.M) {
+        const int64_t col = colb + v;
+        if (col < d.N) {
           float* p = C + row + col * d.ldc;
           const float val = alf * acc[m][n][v];
           *p = (d.beta == 0.0) ? val : val + be * (*p);
@@ -648,14 +664,59 @@ int launch_typed(const GemmDesc& d, hipStream_t s) {
   return launch_tile<T, 128, 128>(d, s);
 }
 
+// Workspace of the split operands, grown on demand (never while capturing).
+void* g_b3_ws = nullptr;
+size_t g_b3_cap = 0;
+
 int launch_bf16x3(const GemmDesc& d, hipStream_t s) {
-  const int64_t tiles = cdiv(d.M, BM) * cdiv(d.N, BN);
+  const int64_t Mp = cdiv(d.M, kB3M) * kB3M, Np = cdiv(d.N, kB3N) * kB3N, Kp = cdiv(d.K, kSplitTile) * kSplitTile;
+  const int64_t nA = (d.strideA == 0 && !d.mapA) ? 1 : d.batch, nB = (d.strideB == 0 && !d.mapB) ? 1 : d.batch;
+  const int64_t sizeA = Mp * Kp, sizeB = Np * Kp;
+  const size_t bytes = sizeof(unsigned short) * 2 * (size_t)(nA * sizeA + nB * sizeB);
+  if (bytes > g_b3_cap) {
+    if (g_capture) {
+      set_error("gemm: the bf16x3 operand workspace cannot grow during graph capture");
+      return kErrArg;
+    }
+    if (g_b3_ws) {
+      ZOLO_HIP(hipDeviceSynchronize());
+      ZOLO_HIP(hipFree(g_b3_ws));
+    }
+    g_b3_ws = nullptr;
+    g_b3_cap = 0;
+    ZOLO_HIP(hipMalloc(&g_b3_ws, bytes));
+    g_b3_cap = bytes;
+  }
+  unsigned short* Ah = reinterpret_cast<unsigned short*>(g_b3_ws);
+  unsigned short* Al = Ah + nA * sizeA;
+  unsigned short* Bh = Al + nA * sizeA;
+  unsigned short* Bl = Bh + nB * sizeB;
+  if (Kp > 0) {
+    const float* A = reinterpret_cast<const float*>(d.A);
+    const float* B = reinterpret_cast<const float*>(d.B);
+    const dim3 ga((unsigned)(Kp / kSplitTile), (unsigned)(Mp / kSplitTile), (unsigned)nA);
+    const dim3 gb((unsigned)(Kp / kSplitTile), (unsigned)(Np / kSplitTile), (unsigned)nB);
+    // op(A) row i, depth k: A[k + i lda] when transposed (K-major), else A[i + k lda]
+    if (d.transA)
+      hipLaunchKernelGGL((bf16_split_kernel<1>), ga, dim3(kThreads), 0, s, A, d.lda, d.strideA, d.mapA, d.M, d.K, Kp,
+                         Ah, Al, sizeA);
+    else
+      hipLaunchKernelGGL((bf16_split_kernel<0>), ga, dim3(kThreads), 0, s, A, d.lda, d.strideA, d.mapA, d.M, d.K, Kp,
+                         Ah, Al, sizeA);
+    // op(B) column j, depth k: B[k + j ldb] when not transposed (K-major), else B[j + k ldb]
+    if (d.transB)
+      hipLaunchKernelGGL((bf16_split_kernel<0>), gb, dim3(kThreads), 0, s, B, d.ldb, d.strideB, d.mapB, d.N, d.K, Kp,
+                         Bh, Bl, sizeB);
+    else
+      hipLaunchKernelGGL((bf16_split_kernel<1>), gb, dim3(kThreads), 0, s, B, d.ldb, d.strideB, d.mapB, d.N, d.K, Kp,
+                         Bh, Bl, sizeB);
+    ZOLO_HIP(hipGetLastError());
+  }
+  const int64_t tiles = cdiv(d.M, kB3M) * cdiv(d.N, kB3N);
   dim3 grid((unsigned)tiles, 1, (unsigned)d.batch);
-  LedgerScope ls(d, 2, BM, BN, s);
-  if (d.transA == 0 && d.transB == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<0, 0>), grid, dim3(kThreads), 0, s, d);
-  else if (d.transA == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<0, 1>), grid, dim3(kThreads), 0, s, d);
-  else if (d.transB == 0) hipLaunchKernelGGL((gemm_bf16x3_kernel<1, 0>), grid, dim3(kThreads), 0, s, d);
-  else hipLaunchKernelGGL((gemm_bf16x3_kernel<1, 1>), grid, dim3(kThreads), 0, s, d);
+  LedgerScope ls(d, 2, kB3M, kB3N, s);
+  hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(kB3Threads), 0, s, d, Ah, Al, nA == 1 ? 0 : sizeA, Bh, Bl,
+                     nB == 1 ? 0 : sizeB, Kp);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

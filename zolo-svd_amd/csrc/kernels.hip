@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kT) void axpby_kernel(double a, const T* A, int64_t
 // Stacked matrix S_q = [X; s_q I] (2n x n, ld = lds), batched over q = blockIdx.y.
 template <typename T>
 __global__ __launch_bounds__(kT) void build_stack_kernel(const T* X, int64_t ldx, T* S, int64_t lds, int64_t strideS,
-                                                         int64_t n, const double* sdiag) {
+                                                         int64_t n, const double* sdiag, int upper) {
   const int q = blockIdx.y;
   T* Sq = S + q * strideS;
   const double s = sdiag[q];
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kT) void build_stack_kernel(const T* X, int64_t ldx
   for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
     const int64_t i = e % (2 * n), j = e / (2 * n);
     T v;
-    if (i < n) v = X[i + j * ldx];
+    if (i < n) v = (upper && i > j) ? (T)0 : X[i + j * ldx];
     else v = (i - n == j) ? (T)s : (T)0;
     Sq[i + j * lds] = v;
   }
@@ -309,9 +309,9 @@ int k_axpby(double a, const T* A, int64_t lda, double b, const T* B, int64_t ldb
 
 template <typename T>
 int k_build_stack(const T* X, int64_t ldx, T* S, int64_t lds, int64_t strideS, int64_t n, const double* sdiag,
-                  int batch, hipStream_t s) {
+                  int batch, hipStream_t s, bool upper) {
   hipLaunchKernelGGL((build_stack_kernel<T>), dim3(grid_for(2 * n * n), batch), dim3(kT), 0, s, X, ldx, S, lds, strideS,
-                     n, sdiag);
+                     n, sdiag, upper ? 1 : 0);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }
@@ -373,7 +373,7 @@ int k_chol_leaf(T* A, int64_t lda, int64_t strideA, T* Li, int64_t ldl, int64_t 
   template int k_norm1<T>(const T*, int64_t, int64_t, int64_t, double*, double*, hipStream_t);                           \
   template int k_axpby<T>(double, const T*, int64_t, double, const T*, int64_t, T*, int64_t, int64_t, int64_t,         \
                           hipStream_t);                                                                                 \
-  template int k_build_stack<T>(const T*, int64_t, T*, int64_t, int64_t, int64_t, const double*, int, hipStream_t);     \
+  template int k_build_stack<T>(const T*, int64_t, T*, int64_t, int64_t, int64_t, const double*, int, hipStream_t, bool);     \
   template int k_shift_copy<T>(const T*, int64_t, T*, int64_t, int64_t, int64_t, const double*, double, double, int,    \
                                hipStream_t);                                                                            \
   template int k_zero_upper<T>(T*, int64_t, int64_t, int64_t, int, hipStream_t);                                        \

@@ -19,8 +19,9 @@ template <typename T>
 int k_axpby(double a, const T* A, int64_t lda, double b, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m,
             int64_t n, hipStream_t s);
 template <typename T>
+// S_q = [X; sdiag_q I] (upper: only the upper triangle of X, zeros below).
 int k_build_stack(const T* X, int64_t ldx, T* S, int64_t lds, int64_t strideS, int64_t n, const double* sdiag,
-                  int batch, hipStream_t s);
+                  int batch, hipStream_t s, bool upper = false);
 template <typename T>
 int k_shift_copy(const T* W, int64_t ldw, T* Wq, int64_t ldq, int64_t strideQ, int64_t n, const double* shift,
                  double scale, double add, int batch, hipStream_t s);
@@ -39,8 +40,10 @@ int k_chol_leaf(T* A, int64_t lda, int64_t strideA, T* Li, int64_t ldl, int64_t 
 // Structured Householder QR (qr.hip).
 constexpr int kQrNb = 64;   // panel width
 template <typename T>
-int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t j, int apply_prev,
-                    int reduce_cur, double* part, int nwg, T* tau, int64_t strideTau, int batch, hipStream_t s);
+// Panel at column p of width jb over rows [p, r1_end) and [n, r2_end) of each 2n x n (or n x n) matrix.
+int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
+                    int64_t j, int apply_prev, int reduce_cur, double* part, int nwg, T* tau, int64_t strideTau,
+                    int batch, hipStream_t s);
 // Whole panel in one workgroup per shift: rows n + jb <= kQrPanelWgRows, width
 // jb <= kQrPanelWgCols<T> (64 for FP32, 32 for FP64).
 constexpr int kQrPanelWgRows = 576;
@@ -51,7 +54,7 @@ int k_qr_panel_wg(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int 
                   int batch, hipStream_t s);
 template <typename T>
 int k_qr_extract_v(const T* S, int64_t lds, int64_t strideS, T* V, int64_t ldv, int64_t strideV, int64_t p, int jb,
-                   int64_t mrows, int batch, hipStream_t s);
+                   int64_t n, int64_t r1_end, int64_t r2_end, int batch, hipStream_t s);
 template <typename T>
 int k_qr_form_t(const T* G, int64_t ldg, int64_t strideG, const T* tau, int64_t strideTau, int64_t p, int jb, T* Tm,
                 int64_t ldt, int64_t strideT, int batch, hipStream_t s);

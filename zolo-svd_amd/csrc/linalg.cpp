@@ -186,29 +186,73 @@ int chol_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const dou
 // ---------------------------------------------------------------------------
 // QR-based step (the structured QR itself is in qr_blocked.cpp).
 // ---------------------------------------------------------------------------
+namespace {
+// The factored form pays off with several shifts; small stacks keep the
+// one-workgroup panels of the plain stacked QR.
+template <typename T>
+bool qr_factored(int64_t n, int r) {
+  return r >= 2 && n + kQrPanelWgCols<T> > kQrPanelWgRows;
+}
+}  // namespace
+
 template <typename T>
 size_t qr_step_bytes(int64_t n, int r) {
-  return sizeof(T) * (size_t)4 * r * n * n + stacked_qr_bytes<T>(n, r) + 4096;
+  size_t b = sizeof(T) * (size_t)4 * r * n * n + stacked_qr_bytes<T>(n, r) + 4096;
+  if (qr_factored<T>(n, r)) b += sizeof(T) * (size_t)n * n + dense_qr_bytes<T>(n) + 4096;
+  return b;
 }
 
 template <typename T>
 int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const double* d_sdiag, const double* d_alpha,
-            T* Z, int64_t ldz, int64_t sZ, Arena& ar) {
+            T* Z, int64_t ldz, int64_t sZ, Arena& ar, int* nz) {
   const int64_t ls = 2 * n, ss = 2 * n * n;
   T* S = ar.take<T>((size_t)r * ss);
   T* Q = ar.take<T>((size_t)r * ss);
   ZOLO_ARENA_OK(ar);
-  ZOLO_CHECK(k_build_stack<T>(X, ldx, S, ls, ss, n, d_sdiag, r, c.s));
-  ZOLO_CHECK(stacked_qr<T>(c, S, ls, ss, Q, ls, ss, n, r, nullptr, ar));
-  // Z_q = alpha_q * Q1 Q2^T ; Q2 is exactly zero below its 255th subdiagonal
-  // (the block reflectors applied to Q are 256 columns wide).
-  GemmDesc d; d.batch = r;
-  d.transA = 0; d.transB = 1; d.M = n; d.N = n; d.K = n;
-  d.A = Q; d.lda = ls; d.strideA = ss;
-  d.B = Q + n; d.ldb = ls; d.strideB = ss;
-  d.C = Z; d.ldc = ldz; d.strideC = sZ;
-  d.alpha = 1.0; d.alphaVec = d_alpha; d.beta = 0.0; d.bLo = 256;
-  ZOLO_CHECK(gemm_t<T>(c, d));
+  if (!qr_factored<T>(n, r)) {
+    ZOLO_CHECK(k_build_stack<T>(X, ldx, S, ls, ss, n, d_sdiag, r, c.s));
+    ZOLO_CHECK(stacked_qr<T>(c, S, ls, ss, Q, ls, ss, n, r, nullptr, ar));
+    // Z_q = alpha_q * Q1 Q2^T ; Q2 is exactly zero below its 255th subdiagonal
+    // (the block reflectors applied to Q are 256 columns wide).
+    GemmDesc d; d.batch = r;
+    d.transA = 0; d.transB = 1; d.M = n; d.N = n; d.K = n;
+    d.A = Q; d.lda = ls; d.strideA = ss;
+    d.B = Q + n; d.ldb = ls; d.strideB = ss;
+    d.C = Z; d.ldc = ldz; d.strideC = sZ;
+    d.alpha = 1.0; d.alphaVec = d_alpha; d.beta = 0.0; d.bLo = 256;
+    ZOLO_CHECK(gemm_t<T>(c, d));
+    *nz = r;
+    return kOk;
+  }
+  // X = Q_X R (X itself is kept: it is the iterate FormX2 needs)
+  const int64_t nout = (n + 255) / 256;
+  T* Xw = ar.take<T>((size_t)n * n);
+  T* taux = ar.take<T>((size_t)n);
+  T* Tx = ar.take<T>((size_t)nout * 256 * 256);
+  ZOLO_ARENA_OK(ar);
+  ZOLO_CHECK(k_scale_copy<T, T>(X, ldx, Xw, n, n, n, 1.0, c.s));
+  ZOLO_CHECK(dense_geqrf<T>(c, Xw, n, n, taux, Tx, ar));
+  // [R; s_q I] = [Q~1_q; Q~2_q] R~_q for every shift; rows sorted large to small
+  // (R on top) keep the Householder QR row-wise accurate for tiny s_q
+  ZOLO_CHECK(k_build_stack<T>(Xw, n, S, ls, ss, n, d_sdiag, r, c.s, true));
+  const size_t mark = ar.used();
+  ZOLO_CHECK(stacked_qr<T>(c, S, ls, ss, Q, ls, ss, n, r, nullptr, ar, QrShape::Tri));
+  ar.reset();
+  ar.take<char>(mark);
+  // Z~ = sum_q alpha_q Q~1_q Q~2_q^T: both factors are zero below their 255th
+  // subdiagonal (256-wide outer blocks), so only k >= max(i, j) - 255 counts
+  for (int q = 0; q < r; ++q) {
+    GemmDesc d;
+    d.transA = 0; d.transB = 1; d.M = n; d.N = n; d.K = n;
+    d.A = Q + q * ss; d.lda = ls;
+    d.B = Q + q * ss + n; d.ldb = ls;
+    d.C = Z; d.ldc = ldz;
+    d.alpha = 1.0; d.alphaVec = d_alpha + q; d.beta = q ? 1.0 : 0.0; d.aLo = 256; d.bLo = 256;
+    ZOLO_CHECK(gemm_t<T>(c, d));
+  }
+  // Z = Q_X Z~
+  ZOLO_CHECK(dense_apply_q<T>(c, Xw, n, n, Tx, Z, ldz, ar));
+  *nz = 1;
   return kOk;
 }
 
@@ -220,7 +264,7 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
                             const double*, T*, int64_t, int64_t, Arena&, int*, bool);                                \
   template size_t qr_step_bytes<T>(int64_t, int);                                                                \
   template int qr_step<T>(const Ctx&, const T*, int64_t, int64_t, int, const double*, const double*, T*, int64_t, \
-                          int64_t, Arena&);
+                          int64_t, Arena&, int*);
 
 ZOLO_LA_INST(double)
 ZOLO_LA_INST(float)

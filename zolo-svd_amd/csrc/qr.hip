@@ -3,6 +3,13 @@
 // the reflector of column j only has support on rows j .. n+j, because the
 // identity block below X fills in one row per eliminated column.
 //
+// A panel at column p of width jb works on two row ranges, [p, r1_end) of the
+// top block and [n, r2_end) of the bottom block; column j of the panel only
+// involves the bottom rows up to n + j.  [X; s I]: r1_end = n, r2_end =
+// n + p + jb (one contiguous range).  A dense n x n matrix: r1_end = n, no
+// bottom rows.  [R; s I] with R upper triangular: r1_end = p + jb (the top
+// rows below the panel are zero and stay zero), r2_end = n + p + jb.
+//
 // Panel factorisation: one launch per column.  Launch j applies the reflector
 // of column j-1 to the rest of the panel and, in the same pass over the rows
 // each thread owns, reduces the Gram row g_c = sum_i S[i][j] S[i][c] that the
@@ -60,8 +67,9 @@ __device__ inline V wave_colsum64(V* a, int lane) {
 // products in registers, one batch of stores.
 template <typename T>
 __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb,
-                                                      int64_t j, int apply_prev, int reduce_cur, double* part,
-                                                      int nwg, T* tau, int64_t strideTau) {
+                                                      int64_t r1_end, int64_t r2_end, int64_t j, int apply_prev,
+                                                      int reduce_cur, double* part, int nwg, T* tau,
+                                                      int64_t strideTau) {
   __shared__ double gs[NBP], rr[NBP], wv[NBP];
   __shared__ double sc[3];  // beta, tau, scal
   __shared__ double red[4][NBP];
@@ -70,11 +78,13 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
   const int64_t rec = (int64_t)nwg * jb + jb;
   double* base = part + (int64_t)q * jb * rec;
   const int t = threadIdx.x, w = blockIdx.x;
-  const int64_t i = p + (int64_t)w * kT + t;           // the row this thread owns
+  // the row this thread owns: local index l over [p, r1_end) then [n, r2_end)
+  const int64_t l = (int64_t)w * kT + t, len1 = r1_end - p;
+  const int64_t i = l < len1 ? p + l : n + (l - len1);
   const int64_t jp = j - 1;
   const int jjp = (int)(jp - p), jj = (int)(j - p);
-  // rows beyond n + j take no part in either reflector of this launch
-  const bool live = i < p + n + jb && i <= n + j && i >= (apply_prev ? jp : j);
+  // bottom rows beyond n + j take no part in either reflector of this launch
+  const bool live = l < len1 + (r2_end - n) && (i < r1_end || i <= n + j) && i >= (apply_prev ? jp : j);
 
   // only the columns this launch updates or reduces: [jjp, jb) or [jj, jb)
   const int cmin = apply_prev ? jjp : jj;
@@ -231,20 +241,24 @@ __global__ __launch_bounds__(kPanelWgRows) void qr_panel_wg_kernel(T* S, int64_t
   }
 }
 
-// V (mrows x jb) with unit diagonal and explicit zeros, from the panel at (p, p).
+// V (mrows x jb, rows [p, r1_end) then [n, r2_end)) with unit diagonal and
+// explicit zeros, from the panel at (p, p).
 template <typename T>
 __global__ __launch_bounds__(kT) void qr_extract_v_kernel(const T* S, int64_t lds, int64_t strideS, T* V, int64_t ldv,
-                                                          int64_t strideV, int64_t p, int jb, int64_t mrows, int64_t n) {
+                                                          int64_t strideV, int64_t p, int jb, int64_t n,
+                                                          int64_t r1_end, int64_t r2_end) {
   const int q = blockIdx.y;
   const T* Sq = S + q * strideS;
   T* Vq = V + q * strideV;
+  const int64_t len1 = r1_end - p, mrows = len1 + (r2_end - n);
   const int64_t total = mrows * jb;
   for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
     const int64_t i = e % mrows;
     const int c = (int)(e / mrows);
+    const int64_t a = i < len1 ? p + i : n + (i - len1), pc = p + c;
     T v = (T)0;
-    if (i == c) v = (T)1;
-    else if (i > c && i <= n + c) v = Sq[(p + i) + (p + c) * lds];
+    if (a == pc) v = (T)1;
+    else if (a > pc && (a < r1_end || a <= n + pc)) v = Sq[a + pc * lds];
     Vq[i + (int64_t)c * ldv] = v;
   }
 }
@@ -304,14 +318,16 @@ inline unsigned grid_for(int64_t total) {
 }  // namespace
 
 template <typename T>
-int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t j, int apply_prev,
-                    int reduce_cur, double* part, int nwg, T* tau, int64_t strideTau, int batch, hipStream_t s) {
-  if (jb > NBP || jb <= 0) {
-    set_error("qr panel: width out of range");
+int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
+                    int64_t j, int apply_prev, int reduce_cur, double* part, int nwg, T* tau, int64_t strideTau,
+                    int batch, hipStream_t s) {
+  if (jb > NBP || jb <= 0 || r1_end < p + jb || r1_end > n || r2_end < n || r2_end > n + p + jb ||
+      (int64_t)nwg * kT < (r1_end - p) + (r2_end - n)) {
+    set_error("qr panel: width or row ranges out of range");
     return kErrArg;
   }
-  hipLaunchKernelGGL((qr_panel_kernel<T>), dim3(nwg, batch), dim3(kT), 0, s, S, lds, strideS, n, p, jb, j, apply_prev,
-                     reduce_cur, part, nwg, tau, strideTau);
+  hipLaunchKernelGGL((qr_panel_kernel<T>), dim3(nwg, batch), dim3(kT), 0, s, S, lds, strideS, n, p, jb, r1_end, r2_end,
+                     j, apply_prev, reduce_cur, part, nwg, tau, strideTau);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }
@@ -334,10 +350,14 @@ int k_qr_panel_wg(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int 
 
 template <typename T>
 int k_qr_extract_v(const T* S, int64_t lds, int64_t strideS, T* V, int64_t ldv, int64_t strideV, int64_t p, int jb,
-                   int64_t mrows, int batch, hipStream_t s) {
-  const int64_t n = mrows - jb;
+                   int64_t n, int64_t r1_end, int64_t r2_end, int batch, hipStream_t s) {
+  const int64_t mrows = (r1_end - p) + (r2_end - n);
+  if (r1_end < p + jb || r1_end > n || r2_end < n || r2_end > n + p + jb || ldv < mrows) {
+    set_error("qr extract v: row ranges out of range");
+    return kErrArg;
+  }
   hipLaunchKernelGGL((qr_extract_v_kernel<T>), dim3(grid_for(mrows * jb), batch), dim3(kT), 0, s, S, lds, strideS, V,
-                     ldv, strideV, p, jb, mrows, n);
+                     ldv, strideV, p, jb, n, r1_end, r2_end);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }
@@ -359,10 +379,10 @@ int k_set_identity_top(T* Q, int64_t ldq, int64_t strideQ, int64_t m, int64_t n,
 }
 
 #define ZOLO_QR_INST(T)                                                                                            \
-  template int k_qr_panel_step<T>(T*, int64_t, int64_t, int64_t, int64_t, int, int64_t, int, int, double*, int, T*, \
-                                  int64_t, int, hipStream_t);                                                      \
-  template int k_qr_extract_v<T>(const T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, int64_t, int,      \
-                                 hipStream_t);                                                                     \
+  template int k_qr_panel_step<T>(T*, int64_t, int64_t, int64_t, int64_t, int, int64_t, int64_t, int64_t, int, int, \
+                                  double*, int, T*, int64_t, int, hipStream_t);                                    \
+  template int k_qr_extract_v<T>(const T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, int64_t, int64_t,  \
+                                 int64_t, int, hipStream_t);                                                       \
   template int k_qr_form_t<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t, int, T*, int64_t, int64_t,    \
                               int, hipStream_t);                                                                   \
   template int k_set_identity_top<T>(T*, int64_t, int64_t, int64_t, int64_t, int, hipStream_t);                \

@@ -19,6 +19,21 @@ namespace {
 
 constexpr int kQrOuter = 256;
 
+// Row ranges of the block reflector of columns [p, p + jb): [p, r1) of the
+// top block and [n, r2) of the bottom block, stored contiguously in V.
+struct Rows {
+  int64_t r1, r2, len1, len2;
+  int64_t m() const { return len1 + len2; }
+};
+inline Rows rows_of(QrShape sh, int64_t n, int64_t p, int64_t jb) {
+  Rows r;
+  r.r1 = (sh == QrShape::Tri) ? p + jb : n;
+  r.r2 = (sh == QrShape::Dense) ? n : n + p + jb;
+  r.len1 = r.r1 - p;
+  r.len2 = r.r2 - n;
+  return r;
+}
+
 template <typename T>
 int gemm_t(const Ctx& c, const GemmDesc& d) {
   return gemm(prec_for<T>(c), with_tile(c, d), c.s);
@@ -77,17 +92,30 @@ int take_scratch(Arena& ar, int64_t n, int batch, QrScratch& q) {
   return kOk;
 }
 
-// Apply H^T = (I - V T V^T)^T (left) to the ncols columns at Cm, rows m.
+// Apply H^T = (I - V T V^T)^T (transpose) or H (left) to the ncols columns at
+// Cm (row p of the matrix).  The reflector's rows are [0, len1) at Cm and
+// [0, len2) at Cm + off2 (off2 = n - p); V holds them contiguously.  Its
+// bottom part is zero for bottom row k > p + column (the fill-in structure).
 template <typename T>
 int apply_block_reflector(const Ctx& c, const T* V, int64_t ldv, int64_t sV, const T* Tm, int64_t ldt, int64_t sT,
-                          int jb, T* Cm, int64_t ldc, int64_t sC, int64_t mrows, int64_t ncols, bool transpose,
-                          T* W, T* W2, int64_t sW, int batch) {
+                          int jb, T* Cm, int64_t ldc, int64_t sC, int64_t len1, int64_t off2, int64_t len2, int64_t p,
+                          int64_t ncols, bool transpose, T* W, T* W2, int64_t sW, int batch) {
+  const bool one = (off2 == len1) || len2 == 0;  // a single contiguous row range
+  const int64_t m1 = (off2 == len1) ? len1 + len2 : len1;
   GemmDesc d;  // W = V^T C
   d.batch = batch;
-  d.transA = 1; d.M = jb; d.N = ncols; d.K = mrows;
+  d.transA = 1; d.M = jb; d.N = ncols; d.K = m1;
   d.A = V; d.lda = ldv; d.strideA = sV; d.B = Cm; d.ldb = ldc; d.strideB = sC;
   d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
   ZOLO_CHECK(gemm_t<T>(c, d));
+  if (!one) {  // W += V2^T C2   (V2[k][i] = 0 for k > p + i)
+    d = GemmDesc();
+    d.batch = batch;
+    d.transA = 1; d.M = jb; d.N = ncols; d.K = len2;
+    d.A = V + len1; d.lda = ldv; d.strideA = sV; d.B = Cm + off2; d.ldb = ldc; d.strideB = sC;
+    d.C = W; d.ldc = jb; d.strideC = sW; d.beta = 1.0; d.aHi = (int)p;
+    ZOLO_CHECK(gemm_t<T>(c, d));
+  }
   d = GemmDesc();  // W2 = T^T W (GEQRF) or T W (ORGQR)
   d.batch = batch;
   d.transA = transpose ? 1 : 0; d.M = jb; d.N = ncols; d.K = jb;
@@ -98,16 +126,26 @@ int apply_block_reflector(const Ctx& c, const T* V, int64_t ldv, int64_t sV, con
   ZOLO_CHECK(gemm_t<T>(c, d));
   d = GemmDesc();  // C -= V W2
   d.batch = batch;
-  d.M = mrows; d.N = ncols; d.K = jb;
+  d.M = m1; d.N = ncols; d.K = jb;
   d.A = V; d.lda = ldv; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
   d.C = Cm; d.ldc = ldc; d.strideC = sC; d.alpha = -1.0; d.beta = 1.0;
-  return gemm_t<T>(c, d);
+  ZOLO_CHECK(gemm_t<T>(c, d));
+  if (!one) {  // C2 -= V2 W2
+    d = GemmDesc();
+    d.batch = batch;
+    d.M = len2; d.N = ncols; d.K = jb;
+    d.A = V + len1; d.lda = ldv; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
+    d.C = Cm + off2; d.ldc = ldc; d.strideC = sC; d.alpha = -1.0; d.beta = 1.0; d.aLo = (int)p;
+    ZOLO_CHECK(gemm_t<T>(c, d));
+  }
+  return kOk;
 }
 
 template <typename T>
-int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, T* Tall, Arena& ar) {
+int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, T* Tall, Arena& ar,
+               QrShape sh) {
   // small stacks: every inner panel in one workgroup (narrower panels in FP64)
-  const bool fused = n + kQrPanelWgCols<T> <= kQrPanelWgRows;
+  const bool fused = sh == QrShape::Stack && n + kQrPanelWgCols<T> <= kQrPanelWgRows;
   const int nb = fused ? kQrPanelWgCols<T> : kQrNb, nbo = kQrOuter;
   QrScratch q;
   ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
@@ -126,50 +164,51 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
     ZOLO_CHECK(k_fill<T>(Tb, nbo, sTall, nbo, nbo, 0.0, batch, c.s));
     for (int64_t p = P; p < P + JB; p += nb) {
       const int jb = (int)std::min<int64_t>(nb, P + JB - p);
-      const int64_t mrows = n + jb;
-      const int nwg = qr_nwg(mrows);
+      const Rows ri = rows_of(sh, n, p, jb);
+      const int nwg = qr_nwg(ri.m());
       if (fused) {
         ZOLO_CHECK(k_qr_panel_wg<T>(S, lds, sS, n, p, jb, tau, n, batch, c.s));
       } else {
         for (int64_t j = p; j <= p + jb; ++j)
-          ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, j, j > p, j < p + jb, part, nwg, tau, n, batch, c.s));
+          ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, ri.r1, ri.r2, j, j > p, j < p + jb, part, nwg, tau, n,
+                                        batch, c.s));
       }
-      ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, p, jb, mrows, batch, c.s));
+      ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, ri.m(), sV, p, jb, n, ri.r1, ri.r2, batch, c.s));
       GemmDesc d;  // G = V^T V -> T of the panel, stored on the diagonal of the outer T
       d.batch = batch;
-      d.transA = 1; d.M = jb; d.N = jb; d.K = mrows;
-      d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
+      d.transA = 1; d.M = jb; d.N = jb; d.K = ri.m();
+      d.A = V; d.lda = ri.m(); d.strideA = sV; d.B = V; d.ldb = ri.m(); d.strideB = sV;
       d.C = G; d.ldc = jb; d.strideC = sG;
       ZOLO_CHECK(gemm_t<T>(c, d));
       T* Tp = Tb + (p - P) * (nbo + 1);
       ZOLO_CHECK(k_qr_form_t<T>(G, jb, sG, tau, n, p, jb, Tp, nbo, sTall, batch, c.s));
       const int64_t ncols = P + JB - p - jb;  // rest of the outer block only
       if (ncols > 0)
-        ZOLO_CHECK(apply_block_reflector<T>(c, V, mrows, sV, Tp, nbo, sTall, jb, S + p + (p + jb) * lds, lds, sS, mrows,
-                                            ncols, true, (T*)q.W, (T*)q.W2, sW, batch));
+        ZOLO_CHECK(apply_block_reflector<T>(c, V, ri.m(), sV, Tp, nbo, sTall, jb, S + p + (p + jb) * lds, lds, sS,
+                                            ri.len1, n - p, ri.len2, p, ncols, true, (T*)q.W, (T*)q.W2, sW, batch));
     }
-    const int64_t mrows = n + JB;
-    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, P, JB, mrows, batch, c.s));
+    const Rows ro = rows_of(sh, n, P, JB);
+    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, ro.m(), sV, P, JB, n, ro.r1, ro.r2, batch, c.s));
     if (JB > nb) {
       GemmDesc d;
       d.batch = batch;
-      d.transA = 1; d.M = JB; d.N = JB; d.K = mrows;
-      d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
+      d.transA = 1; d.M = JB; d.N = JB; d.K = ro.m();
+      d.A = V; d.lda = ro.m(); d.strideA = sV; d.B = V; d.ldb = ro.m(); d.strideB = sV;
       d.C = G; d.ldc = nbo; d.strideC = sG;
       ZOLO_CHECK(gemm_t<T>(c, d));
       ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch, nb));
     }
     const int64_t ncols = n - P - JB;
     if (ncols > 0)
-      ZOLO_CHECK(apply_block_reflector<T>(c, V, mrows, sV, Tb, nbo, sTall, JB, S + P + (P + JB) * lds, lds, sS, mrows,
-                                          ncols, true, (T*)q.W, (T*)q.W2, sW, batch));
+      ZOLO_CHECK(apply_block_reflector<T>(c, V, ro.m(), sV, Tb, nbo, sTall, JB, S + P + (P + JB) * lds, lds, sS,
+                                          ro.len1, n - P, ro.len2, P, ncols, true, (T*)q.W, (T*)q.W2, sW, batch));
   }
   return kOk;
 }
 
 template <typename T>
 int orgqr_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, T* Q, int64_t ldq, int64_t sQ,
-               int64_t n, int batch, T* Tall, bool have_t, Arena& ar) {
+               int64_t n, int batch, T* Tall, bool have_t, Arena& ar, QrShape sh) {
   const int nb = kQrNb, nbo = kQrOuter;
   QrScratch q;
   ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
@@ -184,15 +223,15 @@ int orgqr_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, 
   for (int64_t ob = nout - 1; ob >= 0; --ob) {
     const int64_t P = ob * nbo;
     const int JB = (int)std::min<int64_t>(nbo, n - P);
-    const int64_t mrows = n + JB;
+    const Rows ro = rows_of(sh, n, P, JB);
     T* Tb = Tall + ob * sG;
-    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, mrows, sV, P, JB, mrows, batch, c.s));
+    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, ro.m(), sV, P, JB, n, ro.r1, ro.r2, batch, c.s));
     if (!have_t) {  // re-form T of the outer block from V and tau
       ZOLO_CHECK(k_fill<T>(Tb, nbo, sTall, nbo, nbo, 0.0, batch, c.s));
       GemmDesc d;
       d.batch = batch;
-      d.transA = 1; d.M = JB; d.N = JB; d.K = mrows;
-      d.A = V; d.lda = mrows; d.strideA = sV; d.B = V; d.ldb = mrows; d.strideB = sV;
+      d.transA = 1; d.M = JB; d.N = JB; d.K = ro.m();
+      d.A = V; d.lda = ro.m(); d.strideA = sV; d.B = V; d.ldb = ro.m(); d.strideB = sV;
       d.C = G; d.ldc = nbo; d.strideC = sG;
       ZOLO_CHECK(gemm_t<T>(c, d));
       for (int64_t p = P; p < P + JB; p += nb) {
@@ -203,8 +242,29 @@ int orgqr_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, 
       }
       ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch, kQrNb));
     }
-    ZOLO_CHECK(apply_block_reflector<T>(c, V, mrows, sV, Tb, nbo, sTall, JB, Q + P + P * ldq, ldq, sQ, mrows, n - P,
-                                        false, (T*)q.W, (T*)q.W2, sW, batch));
+    ZOLO_CHECK(apply_block_reflector<T>(c, V, ro.m(), sV, Tb, nbo, sTall, JB, Q + P + P * ldq, ldq, sQ, ro.len1, n - P,
+                                        ro.len2, P, n - P, false, (T*)q.W, (T*)q.W2, sW, batch));
+  }
+  return kOk;
+}
+
+// M <- Q M for the Q of a dense n x n GEQRF (reflectors and outer-block T
+// factors from geqrf_impl), M n x n: the outer blocks from last to first.
+template <typename T>
+int apply_q_dense_impl(const Ctx& c, const T* S, int64_t lds, const T* Tall, T* M, int64_t ldm, int64_t n, Arena& ar) {
+  const int nbo = kQrOuter;
+  QrScratch q;
+  ZOLO_CHECK(take_scratch<T>(ar, n, 1, q));
+  T* V = (T*)q.V;
+  const int64_t sG = (int64_t)nbo * nbo;
+  const int64_t nout = (n + nbo - 1) / nbo;
+  for (int64_t ob = nout - 1; ob >= 0; --ob) {
+    const int64_t P = ob * nbo;
+    const int JB = (int)std::min<int64_t>(nbo, n - P);
+    const Rows ro = rows_of(QrShape::Dense, n, P, JB);
+    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, 0, V, ro.m(), 0, P, JB, n, ro.r1, ro.r2, 1, c.s));
+    ZOLO_CHECK(apply_block_reflector<T>(c, V, ro.m(), 0, Tall + ob * sG, nbo, 0, JB, M + P, ldm, 0, ro.len1, n - P, 0,
+                                        P, n, false, (T*)q.W, (T*)q.W2, 0, 1));
   }
   return kOk;
 }
@@ -238,8 +298,8 @@ int apply_trd_block(const Ctx& c, const double* A, int64_t lda, const double* ta
                                    nbo, 0, 1, c.s));
   }
   ZOLO_CHECK(combine_t<double>(c, Tm, 0, G, 0, tmp, 0, 0, jb, 1, kQrNb));
-  return apply_block_reflector<double>(c, V, mrows, 0, Tm, nbo, 0, jb, C + r0, ldc, 0, mrows, ncols, false, W, W2, 0,
-                                       1);
+  return apply_block_reflector<double>(c, V, mrows, 0, Tm, nbo, 0, jb, C + r0, ldc, 0, mrows, mrows, 0, 0, ncols, false,
+                                       W, W2, 0, 1);
 }
 
 template <typename T>
@@ -262,7 +322,7 @@ int stacked_geqrf(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int ba
   const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
   T* Tall = ar.take<T>((size_t)batch * nout * kQrOuter * kQrOuter);
   ZOLO_ARENA_OK(ar);
-  return geqrf_impl<T>(c, S, lds, sS, n, batch, tau, Tall, ar);
+  return geqrf_impl<T>(c, S, lds, sS, n, batch, tau, Tall, ar, QrShape::Stack);
 }
 
 template <typename T>
@@ -271,27 +331,47 @@ int stacked_orgqr(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* ta
   const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
   T* Tall = ar.take<T>((size_t)batch * nout * kQrOuter * kQrOuter);
   ZOLO_ARENA_OK(ar);
-  return orgqr_impl<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, Tall, false, ar);
+  return orgqr_impl<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, Tall, false, ar, QrShape::Stack);
 }
 
 template <typename T>
 int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
-               T* tau_out, Arena& ar) {
+               T* tau_out, Arena& ar, QrShape sh) {
   T* tau = tau_out ? tau_out : ar.take<T>((size_t)batch * n);
   const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
   T* Tall = ar.take<T>((size_t)batch * nout * kQrOuter * kQrOuter);
   ZOLO_ARENA_OK(ar);
   const size_t mark = ar.used();
-  ZOLO_CHECK(geqrf_impl<T>(c, S, lds, sS, n, batch, tau, Tall, ar));
+  ZOLO_CHECK(geqrf_impl<T>(c, S, lds, sS, n, batch, tau, Tall, ar, sh));
   if (!Q) return kOk;
   ar.reset();  // ORGQR reuses the GEQRF scratch; the outer-block T factors are kept
   ar.take<char>(mark);
-  return orgqr_impl<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, Tall, true, ar);
+  return orgqr_impl<T>(c, S, lds, sS, tau, Q, ldq, sQ, n, batch, Tall, true, ar, sh);
+}
+
+template <typename T>
+size_t dense_qr_bytes(int64_t n) {
+  const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
+  return stacked_qr_bytes<T>(n, 1) + sizeof(T) * ((size_t)nout * kQrOuter * kQrOuter + n) + 512;
+}
+
+template <typename T>
+int dense_geqrf(const Ctx& c, T* A, int64_t lda, int64_t n, T* tau, T* Tall, Arena& ar) {
+  return geqrf_impl<T>(c, A, lda, 0, n, 1, tau, Tall, ar, QrShape::Dense);
+}
+
+template <typename T>
+int dense_apply_q(const Ctx& c, const T* A, int64_t lda, int64_t n, const T* Tall, T* M, int64_t ldm, Arena& ar) {
+  return apply_q_dense_impl<T>(c, A, lda, Tall, M, ldm, n, ar);
 }
 
 #define ZOLO_QRB_INST(T)                                                                                            \
   template size_t stacked_qr_bytes<T>(int64_t, int);                                                                \
-  template int stacked_qr<T>(const Ctx&, T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, T*, Arena&);     \
+  template int stacked_qr<T>(const Ctx&, T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, T*, Arena&,      \
+                             QrShape);                                                                             \
+  template size_t dense_qr_bytes<T>(int64_t);                                                                       \
+  template int dense_geqrf<T>(const Ctx&, T*, int64_t, int64_t, T*, T*, Arena&);                                   \
+  template int dense_apply_q<T>(const Ctx&, const T*, int64_t, int64_t, const T*, T*, int64_t, Arena&);            \
   template int stacked_geqrf<T>(const Ctx&, T*, int64_t, int64_t, int64_t, int, T*, Arena&);                        \
   template int stacked_orgqr<T>(const Ctx&, const T*, int64_t, int64_t, const T*, T*, int64_t, int64_t, int64_t,   \
                                 int, Arena&);

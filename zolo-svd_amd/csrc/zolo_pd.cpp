@@ -173,6 +173,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
     const bool use_qr = (it == 0 || con > 1e3) && maxc > 3e2;
     ar.reset();
     ar.take<char>(mark);
+    int nz = rk;  // Z slots FormX2 sums (the factored QR step returns one)
     if (use_qr) {
       if (low_active) {  // cannot happen with the reference's rule after a Cholesky step; keep FP64 anyway
         ZOLO_CHECK(k_scale_copy<float, double>(X32, n, X, n, n, n, 1.0, s));
@@ -185,7 +186,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
       ZOLO_HIP(hipMemcpyAsync(sc.shift, hs, rk * sizeof(double), hipMemcpyHostToDevice, s));
       ZOLO_HIP(hipMemcpyAsync(sc.alpha, ha, rk * sizeof(double), hipMemcpyHostToDevice, s));
       tm.mark(kPhQr);
-      ZOLO_CHECK(qr_step<double>(c, X, n, n, rk, sc.shift, sc.alpha, Z, n, nn, ar));
+      ZOLO_CHECK(qr_step<double>(c, X, n, n, rk, sc.shift, sc.alpha, Z, n, nn, ar, &nz));
       if (it < 16) info->schedule[it] = 'Q';
     } else {
       for (int q = 0; q < rk; ++q) {
@@ -217,7 +218,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
     if (low_active)
       ZOLO_CHECK(k_formx2<float>(X32, n, Z32, n, nn, rk, n, fone, sc.partial, kPartials, sc.out, s));
     else
-      ZOLO_CHECK(k_formx2<double>(X, n, Z, n, nn, rk, n, fone, sc.partial, kPartials, sc.out, s));
+      ZOLO_CHECK(k_formx2<double>(X, n, Z, n, nn, nz, n, fone, sc.partial, kPartials, sc.out, s));
     double e2;
     ZOLO_CHECK(fetch_double(sc.out, &e2, s));
     err = std::sqrt(e2);
@@ -341,7 +342,8 @@ int qdwh_run(char sym, int64_t n, double sigma, const double* A, int64_t lda, do
       ZOLO_HIP(hipMemcpyAsync(sc.shift, &hs, sizeof(double), hipMemcpyHostToDevice, s));
       ZOLO_HIP(hipMemcpyAsync(sc.alpha, &ha, sizeof(double), hipMemcpyHostToDevice, s));
       tm.mark(kPhQr);
-      ZOLO_CHECK(qr_step<double>(c, Q, n, n, 1, sc.shift, sc.alpha, Z, n, nn, ar));
+      int nz1 = 1;
+      ZOLO_CHECK(qr_step<double>(c, Q, n, n, 1, sc.shift, sc.alpha, Z, n, nn, ar, &nz1));
       if (it < 16) info->schedule[it] = 'Q';
     } else {
       hs = 0.0;
