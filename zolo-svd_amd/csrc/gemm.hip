@@ -119,8 +119,15 @@ __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_b
 struct TileCoord { int64_t i0, j0; bool skip; };
 
 // Map the linear workgroup id to a (row, col) tile, XCD-aware and grouped.
+// XCD x (= pid % 8, the round-robin dispatch) owns a contiguous eighth of the
+// logical tiles, which walk groups of 8 tile-rows column by column so that
+// neighbouring workgroups share A and B panels in the XCD's L2.  When the work
+// per tile varies along the rows (triangular outputs, banded operands),
+// "balance" orders the full row groups 0, last, 1, last-1, ... (with groups
+// small enough that every XCD gets at least two) so that every XCD's eighth
+// pairs heavy and light groups.
 template <int BM, int BN>
-__device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC) {
+__device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC, bool balance) {
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   const int64_t nt = tm * tn;
   int64_t pid = blockIdx.x;
@@ -128,16 +135,24 @@ __device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC) {
   if (pid < full) {  // XCD x (= pid % 8) owns logical tiles [x*nt/8, (x+1)*nt/8)
     pid = (pid % 8) * (full / 8) + pid / 8;
   }
-  const int64_t G = 8;  // tile-rows per group
-  const int64_t group = pid / (G * tn);
+  // tile-rows per group; balancing wants at least two full groups per XCD
+  const int64_t G = balance ? max((int64_t)1, min((int64_t)8, tm / 16)) : 8;
+  int64_t group = pid / (G * tn);
+  const int64_t r = pid - group * G * tn;
+  const int64_t nfull = tm / G;  // groups of exactly G tile-rows (a partial one stays last)
+  if (balance && group < nfull) group = (group & 1) ? nfull - 1 - (group >> 1) : (group >> 1);
   const int64_t first = group * G;
   const int64_t gsz = (tm - first) < G ? (tm - first) : G;
-  const int64_t r = pid - group * G * tn;
   TileCoord t;
   t.i0 = (first + r % gsz) * BM;
   t.j0 = (r / gsz) * BN;
   t.skip = lowerC && (t.j0 > t.i0 + BM - 1);
   return t;
+}
+
+// Work per tile depends on its position: triangular output or trimmed K range.
+__device__ inline bool banded(const GemmDesc& d) {
+  return d.lowerC || d.aLo >= 0 || d.aHi >= 0 || d.bLo >= 0 || d.bHi >= 0;
 }
 
 template <int BM, int BN>
@@ -166,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
   __shared__ T As[2][BK][TBM + PAD];
   __shared__ T Bs[2][BK][TBN + PAD];
 
-  const TileCoord tc = tile_of<TBM, TBN>(d.M, d.N, d.lowerC);
+  const TileCoord tc = tile_of<TBM, TBN>(d.M, d.N, d.lowerC, banded(d));
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;
@@ -427,7 +442,7 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
   __shared__ __attribute__((aligned(16))) unsigned short LA[2][2][kB3M][kB3Pitch];  // [stage][hi lo][row][k]
   __shared__ __attribute__((aligned(16))) unsigned short LB[2][2][kB3N][kB3Pitch];
 
-  const TileCoord tc = tile_of<kB3M, kB3N>(d.M, d.N, d.lowerC);
+  const TileCoord tc = tile_of<kB3M, kB3N>(d.M, d.N, d.lowerC, banded(d));
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;
