@@ -52,6 +52,14 @@ constexpr int kQrPanelWgCols = sizeof(T) == 4 ? 64 : 32;
 template <typename T>
 int k_qr_panel_wg(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, T* tau, int64_t strideTau,
                   int batch, hipStream_t s);
+// The whole panel in one cooperative launch per group of shifts (one grid-wide
+// barrier per column); *done = false when the grid would not stay resident,
+// and the caller then takes k_qr_panel_step.  part: >= 2 * batch * (nwg * 64 + 64)
+// doubles with nwg = ceil(rows / 512).
+template <typename T>
+int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
+                    double* part, int64_t part_doubles, T* tau, int64_t strideTau, int batch, hipStream_t s,
+                    bool* done);
 template <typename T>
 int k_qr_extract_v(const T* S, int64_t lds, int64_t strideS, T* V, int64_t ldv, int64_t strideV, int64_t p, int jb,
                    int64_t n, int64_t r1_end, int64_t r2_end, int batch, hipStream_t s);

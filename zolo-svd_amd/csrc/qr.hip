@@ -17,6 +17,10 @@
 // per-workgroup slot and are summed in a fixed order by the next launch, so
 // the result does not depend on scheduling.  The trailing update and the
 // explicit Q use the compact WY form (I - V T V^T) through the MFMA GEMM.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace zolo {
@@ -161,6 +165,117 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
     red[wave][lane] = wave_colsum64(v, lane);
     __syncthreads();
     if (t < jb) G[(int64_t)w * jb + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+  }
+}
+
+// Column sums of x * v[c] over the 64 lanes for the first rem columns (zero
+// beyond): the first butterfly step forms the products on the fly, so only 32
+// temporaries are live next to v.  Lane l returns the sum of column l.
+template <typename V>
+__device__ inline V wave_colsum64_scaled(const V* v, V x, int rem, int lane) {
+  V a[32];
+  const bool up = (lane & 32) != 0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const V lo = (j < rem) ? x * v[j] : V(0);
+    const V hi = (j + 32 < rem) ? x * v[j + 32] : V(0);
+    const V keep = up ? hi : lo;
+    const V send = up ? lo : hi;
+    a[j] = keep + __shfl_xor(send, 32, 64);
+  }
+  butterfly_step<16>(a, lane);
+  butterfly_step<8>(a, lane);
+  butterfly_step<4>(a, lane);
+  butterfly_step<2>(a, lane);
+  butterfly_step<1>(a, lane);
+  return a[0];
+}
+
+// Whole panel (jb <= 64 columns) in one cooperative launch: one thread per
+// panel row (rows [p, r1_end) then [n, r2_end)) keeps the row's panel values
+// in registers for all jb columns, shifted so that the current column is v[0].
+// Per column: the Gram row g_c = sum_{i>j} S[i][j] S[i][c] and the pivot row
+// go to a per-workgroup slot, one grid-wide barrier, then every workgroup
+// sums the slots in the same fixed order, forms the reflector (dlarfg) and
+// applies it to its rows.  The slots alternate between two buffers, so the
+// next column's partials never overwrite ones still being read.  Shifts
+// q0 .. q0 + gridDim.y - 1.
+constexpr int kCoopT = 512;
+
+template <typename T>
+__global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
+                                                               int64_t p, int jb, int64_t r1_end, int64_t r2_end,
+                                                               double* part, int nwg, T* tau, int64_t strideTau,
+                                                               int q0) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  __shared__ double red[kCoopT / 64][NBP];
+  __shared__ double gs[NBP], rr[NBP], wv[NBP];
+  __shared__ double sc[3];
+  const int qi = blockIdx.y, q = q0 + qi;
+  T* Sq = S + (int64_t)q * strideS;
+  const int t = threadIdx.x, w = blockIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t l = (int64_t)w * kCoopT + t, len1 = r1_end - p;
+  const int64_t i = l < len1 ? p + l : n + (l - len1);
+  const bool valid = l < len1 + (r2_end - n);
+  const int64_t rec = (int64_t)nwg * NBP + NBP;
+  double* base = part + (int64_t)qi * 2 * rec;
+  double v[NBP];
+#pragma unroll
+  for (int c = 0; c < NBP; ++c) v[c] = (valid && c < jb) ? (double)Sq[i + (p + c) * lds] : 0.0;
+  for (int jj = 0; jj < jb; ++jj) {
+    const int64_t j = p + jj;
+    const int rem = jb - jj;  // panel columns left: v[0 .. rem)
+    double* G = base + (int64_t)(jj & 1) * rec;
+    // bottom rows beyond n + j take no part in the reflector of column j
+    const bool live = valid && i >= j && (i < r1_end || i <= n + j);
+    if (live && i == j) {
+#pragma unroll
+      for (int c = 0; c < NBP; ++c)
+        if (c < rem) G[(int64_t)nwg * NBP + c] = v[c];
+    }
+    const double x = (live && i > j) ? v[0] : 0.0;
+    red[wave][lane] = wave_colsum64_scaled(v, x, rem, lane);
+    __syncthreads();
+    if (t < rem) {
+      double sum = 0.0;
+#pragma unroll
+      for (int k = 0; k < kCoopT / 64; ++k) sum += red[k][t];
+      G[(int64_t)w * NBP + t] = sum;
+    }
+    grid.sync();
+    if (t < rem) {
+      double sum = 0.0;
+      for (int k = 0; k < nwg; ++k) sum += G[(int64_t)k * NBP + t];
+      gs[t] = sum;
+      rr[t] = G[(int64_t)nwg * NBP + t];
+    }
+    __syncthreads();
+    if (t == 0) {
+      const double alpha = rr[0], xn2 = gs[0];
+      double beta = alpha, tv = 0.0, scal = 0.0;
+      if (xn2 > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tv = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      sc[0] = beta; sc[1] = tv; sc[2] = scal;
+      if (w == 0) tau[(int64_t)q * strideTau + j] = (T)tv;
+    }
+    __syncthreads();
+    if (t < rem) wv[t] = (t > 0) ? sc[1] * (rr[t] + sc[2] * gs[t]) : 0.0;
+    __syncthreads();
+    if (live) {
+      const double vi = (i == j) ? 1.0 : v[0] * sc[2];
+      v[0] = (i == j) ? sc[0] : vi;
+#pragma unroll
+      for (int c = 1; c < NBP; ++c)
+        if (c < rem) v[c] -= vi * wv[c];
+    }
+    if (valid) Sq[i + j * lds] = (T)v[0];
+#pragma unroll
+    for (int c = 0; c + 1 < NBP; ++c) v[c] = v[c + 1];
+    v[NBP - 1] = 0.0;
+    __syncthreads();  // red, gs, rr, wv and sc are rewritten for the next column
   }
 }
 
@@ -333,6 +448,48 @@ int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
 }
 
 template <typename T>
+int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
+                    double* part, int64_t part_doubles, T* tau, int64_t strideTau, int batch, hipStream_t s,
+                    bool* done) {
+  *done = false;
+  if (jb > NBP || jb <= 0 || r1_end < p + jb || r1_end > n || r2_end < n || r2_end > n + p + jb) {
+    set_error("qr panel (cooperative): width or row ranges out of range");
+    return kErrArg;
+  }
+  // workgroups one launch may keep resident, per device
+  static int cap[64] = {0};
+  int dev = 0;
+  ZOLO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return kOk;
+  if (cap[dev] == 0) {
+    int per_cu = 0, cus = 0, coop = 0;
+    ZOLO_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    ZOLO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    ZOLO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)qr_panel_coop_kernel<T>, kCoopT, 0));
+    cap[dev] = coop ? std::max(per_cu * cus, -1) : -1;
+    if (cap[dev] == 0) cap[dev] = -1;
+  }
+  const int64_t rows = (r1_end - p) + (r2_end - n);
+  const int nwg = (int)((rows + kCoopT - 1) / kCoopT);
+  if (cap[dev] < nwg) return kOk;  // too large for one resident grid: the caller takes the per-column kernels
+  const int per = std::min(batch, cap[dev] / nwg);
+  const int64_t rec = (int64_t)nwg * NBP + NBP;
+  if ((int64_t)per * 2 * rec > part_doubles) {
+    set_error("qr panel (cooperative): partials buffer too small");
+    return kErrArg;
+  }
+  for (int q0 = 0; q0 < batch; q0 += per) {
+    const int nq = std::min(per, batch - q0);
+    int nw = nwg;
+    void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &nw, &tau, &strideTau, (void*)&q0};
+    ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T>, dim3(nwg, nq), dim3(kCoopT), args, 0,
+                                        s));
+  }
+  *done = true;
+  return kOk;
+}
+
+template <typename T>
 int k_qr_panel_wg(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, T* tau, int64_t strideTau,
                   int batch, hipStream_t s) {
   constexpr int NC = sizeof(T) == 4 ? 64 : 32;
@@ -386,7 +543,9 @@ int k_set_identity_top(T* Q, int64_t ldq, int64_t strideQ, int64_t m, int64_t n,
   template int k_qr_form_t<T>(const T*, int64_t, int64_t, const T*, int64_t, int64_t, int, T*, int64_t, int64_t,    \
                               int, hipStream_t);                                                                   \
   template int k_set_identity_top<T>(T*, int64_t, int64_t, int64_t, int64_t, int, hipStream_t);                \
-  template int k_qr_panel_wg<T>(T*, int64_t, int64_t, int64_t, int64_t, int, T*, int64_t, int, hipStream_t);
+  template int k_qr_panel_wg<T>(T*, int64_t, int64_t, int64_t, int64_t, int, T*, int64_t, int, hipStream_t);      \
+  template int k_qr_panel_coop<T>(T*, int64_t, int64_t, int64_t, int64_t, int, int64_t, int64_t, double*, int64_t, \
+                                  T*, int64_t, int, hipStream_t, bool*);
 ZOLO_QR_INST(double)
 ZOLO_QR_INST(float)
 

@@ -150,7 +150,8 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
   QrScratch q;
   ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
   const int nwg_max = qr_nwg(n + nb);
-  double* part = ar.take<double>((size_t)batch * nb * ((size_t)nwg_max * nb + nb));
+  const int64_t part_doubles = (int64_t)batch * nb * ((int64_t)nwg_max * nb + nb);
+  double* part = ar.take<double>((size_t)part_doubles);
   ZOLO_ARENA_OK(ar);
   T* V = (T*)q.V;
   T* G = (T*)q.G;
@@ -169,7 +170,10 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
       if (fused) {
         ZOLO_CHECK(k_qr_panel_wg<T>(S, lds, sS, n, p, jb, tau, n, batch, c.s));
       } else {
-        for (int64_t j = p; j <= p + jb; ++j)
+        bool done = false;
+        ZOLO_CHECK(k_qr_panel_coop<T>(S, lds, sS, n, p, jb, ri.r1, ri.r2, part, part_doubles, tau, n, batch, c.s,
+                                      &done));
+        for (int64_t j = p; !done && j <= p + jb; ++j)
           ZOLO_CHECK(k_qr_panel_step<T>(S, lds, sS, n, p, jb, ri.r1, ri.r2, j, j > p, j < p + jb, part, nwg, tau, n,
                                         batch, c.s));
       }
