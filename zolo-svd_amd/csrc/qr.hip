@@ -201,6 +201,7 @@ __device__ inline V wave_colsum64_scaled(const V* v, V x, int rem, int lane) {
 // next column's partials never overwrite ones still being read.  Shifts
 // q0 .. q0 + gridDim.y - 1.
 constexpr int kCoopT = 512;
+constexpr int kCoopMaxSlots = 8;  // partial slots per thread when summing: nwg <= 8 * (kCoopT / 64)
 
 template <typename T>
 __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
@@ -243,9 +244,24 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
       G[(int64_t)w * NBP + t] = sum;
     }
     grid.sync();
+    {  // sum the nwg partials of each column: 8 groups of 64 threads, all loads independent
+      const int c = t & 63, g = t >> 6;
+      double pv[kCoopMaxSlots];
+#pragma unroll
+      for (int m = 0; m < kCoopMaxSlots; ++m) {
+        const int k = g + m * (kCoopT / 64);
+        pv[m] = (c < rem && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
+      }
+      double sum = 0.0;
+#pragma unroll
+      for (int m = 0; m < kCoopMaxSlots; ++m) sum += pv[m];
+      red[g][c] = sum;
+    }
+    __syncthreads();
     if (t < rem) {
       double sum = 0.0;
-      for (int k = 0; k < nwg; ++k) sum += G[(int64_t)k * NBP + t];
+#pragma unroll
+      for (int g = 0; g < kCoopT / 64; ++g) sum += red[g][t];
       gs[t] = sum;
       rr[t] = G[(int64_t)nwg * NBP + t];
     }
@@ -471,7 +487,8 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   }
   const int64_t rows = (r1_end - p) + (r2_end - n);
   const int nwg = (int)((rows + kCoopT - 1) / kCoopT);
-  if (cap[dev] < nwg) return kOk;  // too large for one resident grid: the caller takes the per-column kernels
+  // too large for one resident grid or for the partial sums: the caller takes the per-column kernels
+  if (cap[dev] < nwg || nwg > kCoopMaxSlots * (kCoopT / 64)) return kOk;
   const int per = std::min(batch, cap[dev] / nwg);
   const int64_t rec = (int64_t)nwg * NBP + NBP;
   if ((int64_t)per * 2 * rec > part_doubles) {
