@@ -48,6 +48,9 @@ typedef struct zolo_info {
   double time_ms;         /* device time of the whole call (hipEvent)                  */
   double phase_ms[8];     /* 0 bounds, 1 QR steps, 2 Cholesky steps, 3 FormX2,
                              4 polish, 5 H, 6 other                                    */
+  int shifts[16];         /* shifts used by each iteration: r, except a low-precision
+                             Cholesky step, which takes the fewest shifts whose bound
+                             reaches FP32 convergence in that step                     */
 } zolo_info_t;
 
 /* ---- library lifetime ---------------------------------------------------- */

@@ -28,6 +28,7 @@ def test_pzolopd1_f64_matches_oracle(zl, n, kind, kappa, seed):
     # same schedule and iteration count as the reference restatement
     sched = "".join("Q" if x == "QR" else "C" for x in lo.schedule)
     assert info.schedule == sched and info.iters == lo.iters
+    assert info.shifts == [rk] * info.iters
     assert np.allclose(info.con, lo.con, rtol=1e-9)
     Ho = zo.polar_h(Uo, A)
     be, orth = _errs(A, U, H)
@@ -52,6 +53,17 @@ def test_pzolopd1_mixed(zl, prec, n, kind, kappa, seed):
     U, H, info = zl.pzolopd1(A, alpha, beta, rk, precision=prec)
     assert info.schedule[0] == "Q" and set(info.schedule[1:]) <= {"c"}
     assert info.polish_passes >= 1
+    # the QR step keeps r; a low-precision step takes the fewest shifts whose
+    # bound reaches FP32 convergence in one step (scalar recurrence of the oracle)
+    assert info.shifts[0] == rk
+    for k in range(1, info.iters):
+        rs = info.shifts[k]
+        assert 1 <= rs <= rk
+        _, c1 = zo.zolo_fone_con(info.con[k], zo.compute_coeff(info.con[k], zo.DLAMCH_P, rs), rs)
+        assert c1 - 1.0 <= 6e-8 or rs == rk
+        if rs > 1:
+            _, c0 = zo.zolo_fone_con(info.con[k], zo.compute_coeff(info.con[k], zo.DLAMCH_P, rs - 1), rs - 1)
+            assert c0 - 1.0 > 6e-8
     be, orth = _errs(A, U, H)
     assert be <= 1e-13 and orth <= 1e-13, (be, orth)
     Uo, _ = zo.pzolopd1(A, alpha, beta, rk)

@@ -173,7 +173,26 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
     const bool use_qr = (it == 0 || con > 1e3) && maxc > 3e2;
     ar.reset();
     ar.take<char>(mark);
-    int nz = rk;  // Z slots FormX2 sums (the factored QR step returns one)
+    // Shifts of this step.  The FP64 path keeps the reference's r.  A
+    // low-precision Cholesky step takes the fewest shifts whose Zolotarev
+    // bound reaches FP32 convergence in this one step (the polish then works
+    // to FP64 from that iterate), falling back to r.
+    int rs = rk;
+    if (mixed && !use_qr) {
+      std::vector<double> cr(3 * rk);
+      for (int r = 1; r < rk; ++r) {
+        double f1 = 0.0, c1 = 0.0;
+        compute_coeff(con, tol, r, cr.data());
+        zolo_fone_con(con, cr.data(), r, &f1, &c1);
+        if (c1 - 1.0 <= kLowConverged) {
+          rs = r;
+          std::copy(cr.begin(), cr.begin() + 3 * r, coeff.begin());
+          break;
+        }
+      }
+    }
+    if (it < 16) info->shifts[it] = rs;
+    int nz = rs;  // Z slots FormX2 sums (the factored QR step returns one)
     if (use_qr) {
       if (low_active) {  // cannot happen with the reference's rule after a Cholesky step; keep FP64 anyway
         ZOLO_CHECK(k_scale_copy<float, double>(X32, n, X, n, n, n, 1.0, s));
@@ -189,12 +208,12 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
       ZOLO_CHECK(qr_step<double>(c, X, n, n, rk, sc.shift, sc.alpha, Z, n, nn, ar, &nz));
       if (it < 16) info->schedule[it] = 'Q';
     } else {
-      for (int q = 0; q < rk; ++q) {
+      for (int q = 0; q < rs; ++q) {
         hs[q] = coeff[2 * q];
-        ha[q] = -coeff[2 * rk + q];
+        ha[q] = -coeff[2 * rs + q];
       }
-      ZOLO_HIP(hipMemcpyAsync(sc.shift, hs, rk * sizeof(double), hipMemcpyHostToDevice, s));
-      ZOLO_HIP(hipMemcpyAsync(sc.alpha, ha, rk * sizeof(double), hipMemcpyHostToDevice, s));
+      ZOLO_HIP(hipMemcpyAsync(sc.shift, hs, rs * sizeof(double), hipMemcpyHostToDevice, s));
+      ZOLO_HIP(hipMemcpyAsync(sc.alpha, ha, rs * sizeof(double), hipMemcpyHostToDevice, s));
       tm.mark(kPhChol);
       if (!mixed) {
         ZOLO_CHECK(chol_step<double>(c, X, n, n, rk, sc.shift, 1.0, 0.0, sc.alpha, Z, n, nn, ar, sc.info, symS));
@@ -208,15 +227,15 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
           ZOLO_CHECK(k_scale_copy<double, float>(X, n, X32, n, n, n, 1.0, s));
           low_active = true;
         }
-        ZOLO_CHECK(chol_step<float>(c, X32, n, n, rk, sc.shift, 1.0, 0.0, sc.alpha, Z32, n, nn, ar, sc.info, symS));
+        ZOLO_CHECK(chol_step<float>(c, X32, n, n, rs, sc.shift, 1.0, 0.0, sc.alpha, Z32, n, nn, ar, sc.info, symS));
         if (it < 16) info->schedule[it] = 'c';
       }
     }
     double fone;
-    zolo_fone_con(con, coeff.data(), rk, &fone, &con);
+    zolo_fone_con(con, coeff.data(), rs, &fone, &con);
     tm.mark(kPhForm);
     if (low_active)
-      ZOLO_CHECK(k_formx2<float>(X32, n, Z32, n, nn, rk, n, fone, sc.partial, kPartials, sc.out, s));
+      ZOLO_CHECK(k_formx2<float>(X32, n, Z32, n, nn, nz, n, fone, sc.partial, kPartials, sc.out, s));
     else
       ZOLO_CHECK(k_formx2<double>(X, n, Z, n, nn, nz, n, fone, sc.partial, kPartials, sc.out, s));
     double e2;
@@ -345,6 +364,7 @@ int qdwh_run(char sym, int64_t n, double sigma, const double* A, int64_t lda, do
       int nz1 = 1;
       ZOLO_CHECK(qr_step<double>(c, Q, n, n, 1, sc.shift, sc.alpha, Z, n, nn, ar, &nz1));
       if (it < 16) info->schedule[it] = 'Q';
+      if (it < 16) info->shifts[it] = 1;
     } else {
       hs = 0.0;
       ha = ak - bk / ck;
@@ -353,6 +373,7 @@ int qdwh_run(char sym, int64_t n, double sigma, const double* A, int64_t lda, do
       tm.mark(kPhChol);
       ZOLO_CHECK(chol_step<double>(c, Q, n, n, 1, sc.shift, ck, 1.0, sc.alpha, Z, n, nn, ar, sc.info, symS));
       if (it < 16) info->schedule[it] = 'C';
+      if (it < 16) info->shifts[it] = 1;
     }
     // X1 = (b/c) Q + Z ; err = ||Q - X1||_F ; Q = X1
     tm.mark(kPhForm);
