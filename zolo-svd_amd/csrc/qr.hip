@@ -17,8 +17,6 @@
 // per-workgroup slot and are summed in a fixed order by the next launch, so
 // the result does not depend on scheduling.  The trailing update and the
 // explicit Q use the compact WY form (I - V T V^T) through the MFMA GEMM.
-#include <hip/hip_cooperative_groups.h>
-
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -168,19 +166,19 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
   }
 }
 
-// Column sums of x * v[c] over the 64 lanes for the first rem columns (zero
-// beyond): the first butterfly step forms the products on the fly, so only 32
-// temporaries are live next to v.  Lane l returns the sum of column l.
+// Column sums of x * v[c] over the 64 lanes for the columns c in [lo, hi)
+// (zero elsewhere): the first butterfly step forms the products on the fly, so
+// only 32 temporaries are live next to v.  Lane l returns the sum of column l.
 template <typename V>
-__device__ inline V wave_colsum64_scaled(const V* v, V x, int rem, int lane) {
+__device__ inline V wave_colsum64_masked(const V* v, V x, int lo, int hi, int lane) {
   V a[32];
   const bool up = (lane & 32) != 0;
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
-    const V lo = (j < rem) ? x * v[j] : V(0);
-    const V hi = (j + 32 < rem) ? x * v[j + 32] : V(0);
-    const V keep = up ? hi : lo;
-    const V send = up ? lo : hi;
+    const V l0 = (j >= lo && j < hi) ? x * v[j] : V(0);
+    const V h0 = (j + 32 >= lo && j + 32 < hi) ? x * v[j + 32] : V(0);
+    const V keep = up ? h0 : l0;
+    const V send = up ? l0 : h0;
     a[j] = keep + __shfl_xor(send, 32, 64);
   }
   butterfly_step<16>(a, lane);
@@ -191,24 +189,38 @@ __device__ inline V wave_colsum64_scaled(const V* v, V x, int rem, int lane) {
   return a[0];
 }
 
+// Barrier of the nwg workgroups of one shift (all resident: the launch is
+// cooperative).  The counter only grows: arrival k of every workgroup waits
+// until it reads k * nwg.  Device-scope fences publish the partials written
+// before the barrier and make the other workgroups' partials visible after it.
+__device__ inline void shift_barrier(unsigned* ctr, unsigned target) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(ctr, 1u);
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 // Whole panel (jb <= 64 columns) in one cooperative launch: one thread per
 // panel row (rows [p, r1_end) then [n, r2_end)) keeps the row's panel values
-// in registers for all jb columns, shifted so that the current column is v[0].
-// Per column: the Gram row g_c = sum_{i>j} S[i][j] S[i][c] and the pivot row
-// go to a per-workgroup slot, one grid-wide barrier, then every workgroup
-// sums the slots in the same fixed order, forms the reflector (dlarfg) and
-// applies it to its rows.  The slots alternate between two buffers, so the
-// next column's partials never overwrite ones still being read.  Shifts
-// q0 .. q0 + gridDim.y - 1.
+// in registers for all jb columns and stores them once at the end.  Per
+// column: the Gram row g_c = sum_{i>j} S[i][j] S[i][c] and the pivot row go to
+// a per-workgroup slot, one barrier among the workgroups of the shift, then
+// every workgroup sums the slots in the same fixed order, forms the reflector
+// (dlarfg) and applies it to its rows.  The slots alternate between two
+// buffers, so the next column's partials never overwrite ones still being
+// read.  Shifts q0 .. q0 + gridDim.y - 1; bar holds one zeroed counter per shift.
 constexpr int kCoopT = 512;
 constexpr int kCoopMaxSlots = 8;  // partial slots per thread when summing: nwg <= 8 * (kCoopT / 64)
 
 template <typename T>
 __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
                                                                int64_t p, int jb, int64_t r1_end, int64_t r2_end,
-                                                               double* part, int nwg, T* tau, int64_t strideTau,
-                                                               int q0) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+                                                               double* part, unsigned* bar, int nwg, T* tau,
+                                                               int64_t strideTau, int q0) {
   __shared__ double red[kCoopT / 64][NBP];
   __shared__ double gs[NBP], rr[NBP], wv[NBP];
   __shared__ double sc[3];
@@ -225,32 +237,35 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
   for (int c = 0; c < NBP; ++c) v[c] = (valid && c < jb) ? (double)Sq[i + (p + c) * lds] : 0.0;
   for (int jj = 0; jj < jb; ++jj) {
     const int64_t j = p + jj;
-    const int rem = jb - jj;  // panel columns left: v[0 .. rem)
     double* G = base + (int64_t)(jj & 1) * rec;
     // bottom rows beyond n + j take no part in the reflector of column j
     const bool live = valid && i >= j && (i < r1_end || i <= n + j);
     if (live && i == j) {
 #pragma unroll
       for (int c = 0; c < NBP; ++c)
-        if (c < rem) G[(int64_t)nwg * NBP + c] = v[c];
+        if (c >= jj && c < jb) G[(int64_t)nwg * NBP + c] = v[c];
     }
-    const double x = (live && i > j) ? v[0] : 0.0;
-    red[wave][lane] = wave_colsum64_scaled(v, x, rem, lane);
+    double vj = 0.0;
+#pragma unroll
+    for (int c = 0; c < NBP; ++c)
+      if (c == jj) vj = v[c];
+    const double x = (live && i > j) ? vj : 0.0;
+    red[wave][lane] = wave_colsum64_masked(v, x, jj, jb, lane);
     __syncthreads();
-    if (t < rem) {
+    if (t >= jj && t < jb) {
       double sum = 0.0;
 #pragma unroll
       for (int k = 0; k < kCoopT / 64; ++k) sum += red[k][t];
       G[(int64_t)w * NBP + t] = sum;
     }
-    grid.sync();
+    shift_barrier(bar + qi, (unsigned)(jj + 1) * (unsigned)nwg);
     {  // sum the nwg partials of each column: 8 groups of 64 threads, all loads independent
       const int c = t & 63, g = t >> 6;
       double pv[kCoopMaxSlots];
 #pragma unroll
       for (int m = 0; m < kCoopMaxSlots; ++m) {
         const int k = g + m * (kCoopT / 64);
-        pv[m] = (c < rem && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
+        pv[m] = (c >= jj && c < jb && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
       }
       double sum = 0.0;
 #pragma unroll
@@ -258,7 +273,7 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
       red[g][c] = sum;
     }
     __syncthreads();
-    if (t < rem) {
+    if (t >= jj && t < jb) {
       double sum = 0.0;
 #pragma unroll
       for (int g = 0; g < kCoopT / 64; ++g) sum += red[g][t];
@@ -267,7 +282,7 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
     }
     __syncthreads();
     if (t == 0) {
-      const double alpha = rr[0], xn2 = gs[0];
+      const double alpha = rr[jj], xn2 = gs[jj];
       double beta = alpha, tv = 0.0, scal = 0.0;
       if (xn2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
@@ -278,20 +293,23 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
       if (w == 0) tau[(int64_t)q * strideTau + j] = (T)tv;
     }
     __syncthreads();
-    if (t < rem) wv[t] = (t > 0) ? sc[1] * (rr[t] + sc[2] * gs[t]) : 0.0;
+    if (t > jj && t < jb) wv[t] = sc[1] * (rr[t] + sc[2] * gs[t]);
     __syncthreads();
     if (live) {
-      const double vi = (i == j) ? 1.0 : v[0] * sc[2];
-      v[0] = (i == j) ? sc[0] : vi;
+      const double vi = (i == j) ? 1.0 : vj * sc[2];
+      const double newj = (i == j) ? sc[0] : vi;
 #pragma unroll
-      for (int c = 1; c < NBP; ++c)
-        if (c < rem) v[c] -= vi * wv[c];
+      for (int c = 0; c < NBP; ++c) {
+        if (c == jj) v[c] = newj;
+        if (c > jj && c < jb) v[c] -= vi * wv[c];
+      }
     }
-    if (valid) Sq[i + j * lds] = (T)v[0];
-#pragma unroll
-    for (int c = 0; c + 1 < NBP; ++c) v[c] = v[c + 1];
-    v[NBP - 1] = 0.0;
     __syncthreads();  // red, gs, rr, wv and sc are rewritten for the next column
+  }
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < NBP; ++c)
+      if (c < jb) Sq[i + (p + c) * lds] = (T)v[c];
   }
 }
 
@@ -491,14 +509,18 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   if (cap[dev] < nwg || nwg > kCoopMaxSlots * (kCoopT / 64)) return kOk;
   const int per = std::min(batch, cap[dev] / nwg);
   const int64_t rec = (int64_t)nwg * NBP + NBP;
-  if ((int64_t)per * 2 * rec > part_doubles) {
+  // partial slots of the group, then one barrier counter per shift
+  if ((int64_t)per * 2 * rec + (per + 1) / 2 + 1 > part_doubles) {
     set_error("qr panel (cooperative): partials buffer too small");
     return kErrArg;
   }
+  unsigned* bar = reinterpret_cast<unsigned*>(part + (int64_t)per * 2 * rec);
   for (int q0 = 0; q0 < batch; q0 += per) {
     const int nq = std::min(per, batch - q0);
     int nw = nwg;
-    void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &nw, &tau, &strideTau, (void*)&q0};
+    ZOLO_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * nq, s));
+    void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &bar, &nw, &tau, &strideTau,
+                    (void*)&q0};
     ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T>, dim3(nwg, nq), dim3(kCoopT), args, 0,
                                         s));
   }
