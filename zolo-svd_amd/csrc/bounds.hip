@@ -171,39 +171,70 @@ __device__ inline double opT(const double* M, int64_t ld, int trans, int64_t i, 
   return trans ? M[k + i * ld] : M[i + k * ld];
 }
 
-// Solve the diagonal block [b0, b0+bs) in place (lower: forward, upper: backward).
-__global__ __launch_bounds__(kTb) void trsv_diag_kernel(const double* M, int64_t ld, int trans, int lower, int unit,
-                                                        double* x, int64_t b0, int bs) {
+// Inverses of the kTb x kTb diagonal blocks of the LU factors, one workgroup
+// per block: Linv_b = L_bb^{-1} (unit lower), Uinv_b = U_bb^{-1} (upper).
+// Thread c forms column c by substitution in LDS.
+__global__ __launch_bounds__(kTb) void lu_block_inv_kernel(const double* LU, int64_t ld, int64_t n, double* Linv,
+                                                           double* Uinv) {
   __shared__ double a[kTb][kTb + 1];
-  __shared__ double xb[kTb];
+  __shared__ double xl[kTb][kTb + 1];
+  __shared__ double xu[kTb][kTb + 1];
+  const int64_t b0 = (int64_t)blockIdx.x * kTb;
+  const int bs = (int)min((int64_t)kTb, n - b0);
   const int t = threadIdx.x;
-  for (int e = t; e < bs * bs; e += kTb) {
-    const int i = e % bs, k = e / bs;
-    a[i][k] = opT(M, ld, trans, b0 + i, b0 + k);
+  for (int e = t; e < kTb * kTb; e += kTb) {
+    const int i = e % kTb, k = e / kTb;
+    a[i][k] = (i < bs && k < bs) ? LU[(b0 + i) + (b0 + k) * ld] : (i == k ? 1.0 : 0.0);
   }
-  if (t < bs) xb[t] = x[b0 + t];
   __syncthreads();
-  for (int s = 0; s < bs; ++s) {
-    const int k = lower ? s : bs - 1 - s;
-    if (t == 0 && !unit) xb[k] /= a[k][k];
-    __syncthreads();
-    const bool below = lower ? (t > k) : (t < k);
-    if (t < bs && below) xb[t] -= a[t][k] * xb[k];
-    __syncthreads();
+  const int c = t;
+  // L x = e_c (unit lower): x[i] = -sum_{k in [c, i)} a[i][k] x[k], x[c] = 1
+  for (int i = 0; i < kTb; ++i) xl[i][c] = (i == c) ? 1.0 : 0.0;
+  for (int i = c + 1; i < kTb; ++i) {
+    double sum = 0.0;
+    for (int k = c; k < i; ++k) sum += a[i][k] * xl[k][c];
+    xl[i][c] = -sum;
   }
-  if (t < bs) x[b0 + t] = xb[t];
+  // U x = e_c (upper): x[i] = (delta_ic - sum_{k in (i, c]} a[i][k] x[k]) / a[i][i]
+  for (int i = 0; i < kTb; ++i) xu[i][c] = 0.0;
+  for (int i = c; i >= 0; --i) {
+    double sum = (i == c) ? 1.0 : 0.0;
+    for (int k = i + 1; k <= c; ++k) sum -= a[i][k] * xu[k][c];
+    xu[i][c] = sum / a[i][i];
+  }
+  __syncthreads();
+  double* Lb = Linv + (int64_t)blockIdx.x * kTb * kTb;
+  double* Ub = Uinv + (int64_t)blockIdx.x * kTb * kTb;
+  for (int e = t; e < kTb * kTb; e += kTb) {
+    const int i = e % kTb, k = e / kTb;
+    Lb[i + k * kTb] = xl[i][k];
+    Ub[i + k * kTb] = xu[i][k];
+  }
 }
 
-// Rows r outside the block in the solve direction: x[r] -= sum_k op(T)[r][b0+k] x[b0+k].
-__global__ __launch_bounds__(kT) void trsv_update_kernel(const double* M, int64_t ld, int trans, int64_t r0,
-                                                         int64_t r1, double* x, int64_t b0, int bs) {
-  __shared__ double xb[kTb];
-  if ((int)threadIdx.x < bs) xb[threadIdx.x] = x[b0 + threadIdx.x];
+// One block of a blocked triangular solve: every workgroup forms
+// x_b' = op(D_b)^{-1} x_b from the block's explicit inverse (Dinv, op =
+// transposed when dtrans), workgroup 0 writes it to y, and rows r in [r0, r1)
+// get x[r] -= sum_k op(M)[r][b0+k] x_b'[k].  x_b itself is not written, so the
+// workgroups never race on it.
+__global__ __launch_bounds__(kT) void trsv_step_kernel(const double* M, int64_t ld, int trans, const double* Dinv,
+                                                       int dtrans, double* x, double* y, int64_t b0, int bs,
+                                                       int64_t r0, int64_t r1) {
+  __shared__ double xb[kTb], xs[kTb];
+  const int t = threadIdx.x;
+  if (t < bs) xb[t] = x[b0 + t];
   __syncthreads();
-  for (int64_t r = r0 + (int64_t)blockIdx.x * kT + threadIdx.x; r < r1; r += (int64_t)gridDim.x * kT) {
-    double s = 0.0;
-    for (int k = 0; k < bs; ++k) s += opT(M, ld, trans, r, b0 + k) * xb[k];
-    x[r] -= s;
+  if (t < bs) {
+    double sum = 0.0;
+    for (int k = 0; k < bs; ++k) sum += (dtrans ? Dinv[k + t * kTb] : Dinv[t + k * kTb]) * xb[k];
+    xs[t] = sum;
+    if (blockIdx.x == 0) y[b0 + t] = sum;
+  }
+  __syncthreads();
+  for (int64_t r = r0 + (int64_t)blockIdx.x * kT + t; r < r1; r += (int64_t)gridDim.x * kT) {
+    double sum = 0.0;
+    for (int k = 0; k < bs; ++k) sum += opT(M, ld, trans, r, b0 + k) * xs[k];
+    x[r] -= sum;
   }
 }
 
@@ -281,17 +312,21 @@ int getrf(const Ctx& c, double* A, int64_t lda, int64_t n, int64_t* ipiv, int* i
   return kOk;
 }
 
-// x <- op(T)^{-1} x, blocked.
-int trsv(const Ctx& c, const double* M, int64_t ld, int64_t n, int trans, int lower, int unit, double* x) {
+// y <- op(T)^{-1} x, blocked, with the explicit diagonal-block inverses of the
+// LU factors (x is used as workspace).  op(T): L (lower, unit), U (upper),
+// U^T (lower), L^T (upper, unit).
+int trsv(const Ctx& c, const double* M, int64_t ld, int64_t n, int trans, int lower, const double* Linv,
+         const double* Uinv, double* x, double* y) {
   const int64_t nb = (n + kTb - 1) / kTb;
+  const bool useL = (lower && !trans) || (!lower && trans);  // op(T) built from L
+  const double* Dinv = useL ? Linv : Uinv;
   for (int64_t s = 0; s < nb; ++s) {
     const int64_t b = lower ? s : nb - 1 - s;
     const int64_t b0 = b * kTb;
     const int bs = (int)std::min<int64_t>(kTb, n - b0);
-    hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(kTb), 0, c.s, M, ld, trans, lower, unit, x, b0, bs);
     const int64_t r0 = lower ? b0 + bs : 0, r1 = lower ? n : b0;
-    if (r1 > r0)
-      hipLaunchKernelGGL(trsv_update_kernel, dim3(grid_for(r1 - r0)), dim3(kT), 0, c.s, M, ld, trans, r0, r1, x, b0, bs);
+    hipLaunchKernelGGL(trsv_step_kernel, dim3(grid_for(std::max<int64_t>(r1 - r0, 1))), dim3(kT), 0, c.s, M, ld,
+                       trans, Dinv + b * kTb * kTb, trans, x, y, b0, bs, r0, r1);
   }
   ZOLO_HIP(hipGetLastError());
   return kOk;
@@ -300,8 +335,9 @@ int trsv(const Ctx& c, const double* M, int64_t ld, int64_t n, int trans, int lo
 }  // namespace
 
 size_t bounds_bytes(int64_t n) {
-  return sizeof(double) * (size_t)(n * n + 2 * n + kLuNb * kLuNb + kLuNb * n + 8) + sizeof(int64_t) * n +
-         sizeof(Piv) * (size_t)(n / kT + 2) + 16 * 256;
+  const int64_t nb = (n + kTb - 1) / kTb;
+  return sizeof(double) * (size_t)(n * n + 3 * n + kLuNb * kLuNb + kLuNb * n + 8 + 2 * nb * kTb * kTb) +
+         sizeof(int64_t) * n + sizeof(Piv) * (size_t)(n / kT + 2) + 16 * 256;
 }
 
 int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta, Arena& ar,
@@ -310,7 +346,11 @@ int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* 
   c.s = s;
   double* LU = ar.take<double>((size_t)n * n);
   double* x = ar.take<double>(n);
+  double* y = ar.take<double>(n);
   double* col = ar.take<double>(n);
+  const int64_t nblk = (n + kTb - 1) / kTb;
+  double* Linv = ar.take<double>((size_t)nblk * kTb * kTb);
+  double* Uinv = ar.take<double>((size_t)nblk * kTb * kTb);
   double* L11i = ar.take<double>(kLuNb * kLuNb);
   double* W = ar.take<double>((size_t)kLuNb * n);
   double* out = ar.take<double>(8);
@@ -343,16 +383,18 @@ int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* 
     set_error("bounds: matrix is exactly singular (LU pivot " + std::to_string(hinfo) + ")");
     return kErrArg;
   }
+  hipLaunchKernelGGL(lu_block_inv_kernel, dim3((unsigned)nblk), dim3(kTb), 0, s, LU, n, n, Linv, Uinv);
+  ZOLO_HIP(hipGetLastError());
   // dlacn2 on inv(U) inv(L) (xGECON applies no permutation).
   std::vector<double> h(n), sgn(n);
   auto solve = [&](int trans) -> int {
     ZOLO_HIP(hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     if (!trans) {
-      ZOLO_CHECK(trsv(c, LU, n, n, 0, 1, 1, x));   // L (unit lower)
-      ZOLO_CHECK(trsv(c, LU, n, n, 0, 0, 0, x));   // U
+      ZOLO_CHECK(trsv(c, LU, n, n, 0, 1, Linv, Uinv, x, y));   // L (unit lower)
+      ZOLO_CHECK(trsv(c, LU, n, n, 0, 0, Linv, Uinv, y, x));   // U
     } else {
-      ZOLO_CHECK(trsv(c, LU, n, n, 1, 1, 0, x));   // U^T (lower)
-      ZOLO_CHECK(trsv(c, LU, n, n, 1, 0, 1, x));   // L^T (unit upper)
+      ZOLO_CHECK(trsv(c, LU, n, n, 1, 1, Linv, Uinv, x, y));   // U^T (lower)
+      ZOLO_CHECK(trsv(c, LU, n, n, 1, 0, Linv, Uinv, y, x));   // L^T (unit upper)
     }
     ZOLO_HIP(hipMemcpyAsync(h.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     ZOLO_HIP(hipStreamSynchronize(s));
