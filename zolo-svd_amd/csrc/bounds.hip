@@ -280,31 +280,68 @@ inline unsigned grid_for(int64_t total) {
 // One thread per row in the panel kernels.
 int lu_nwg(int64_t n) { return (int)std::max<int64_t>(1, (n + kT - 1) / kT); }
 
-// In-place LU with partial pivoting (row swaps applied to full rows).
+// Triangular solve with the unit-lower kLuNb-wide diagonal block at (p, p):
+// B (jb x nc, rows p.., ld lda) <- L11^{-1} B, through the block's explicit
+// inverse and a GEMM into W, copied back.
+int lu_trsm_block(const Ctx& c, double* A, int64_t lda, int64_t p, int jb, double* B, int64_t nc, double* L11i,
+                  double* W) {
+  hipLaunchKernelGGL(unit_lower_inv_kernel, dim3(1), dim3(kT), 0, c.s, A + p + p * lda, lda, jb, L11i, (int64_t)kLuNb);
+  GemmDesc d;  // W = L11^{-1} B
+  d.transA = 0; d.transB = 0; d.M = jb; d.N = nc; d.K = jb;
+  d.A = L11i; d.lda = kLuNb; d.B = B; d.ldb = lda; d.C = W; d.ldc = jb; d.aHi = 0;
+  ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+  hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for(jb * nc)), dim3(kT), 0, c.s, W, (int64_t)jb, B, lda, (int64_t)jb, nc);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+// In-place LU with partial pivoting (row swaps applied to full rows), two
+// levels: panels of kLuNb columns factored column by column and applied only
+// inside their outer block of kLuOuter columns; each outer block then updates
+// the trailing matrix with K = kLuOuter (A22 -= L21 U12 on MFMA).
+constexpr int kLuOuter = 256;
+
 int getrf(const Ctx& c, double* A, int64_t lda, int64_t n, int64_t* ipiv, int* info, Piv* part, double* L11i,
           double* W) {
   const int nwg = lu_nwg(n);
-  for (int64_t p = 0; p < n; p += kLuNb) {
-    const int jb = (int)std::min<int64_t>(kLuNb, n - p);
-    const int64_t pe = p + jb;
-    hipLaunchKernelGGL(lu_argmax_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, p, nwg, part);
-    for (int64_t j = p; j < pe; ++j) {
-      hipLaunchKernelGGL(lu_swap_kernel, dim3(grid_for(n)), dim3(kT), 0, c.s, A, lda, n, j, nwg, part, ipiv, info);
-      hipLaunchKernelGGL(lu_update_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, j, pe, nwg, part);
+  for (int64_t P = 0; P < n; P += kLuOuter) {
+    const int JB = (int)std::min<int64_t>(kLuOuter, n - P);
+    const int64_t PE = P + JB;
+    for (int64_t p = P; p < PE; p += kLuNb) {
+      const int jb = (int)std::min<int64_t>(kLuNb, PE - p);
+      const int64_t pe = p + jb;
+      hipLaunchKernelGGL(lu_argmax_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, p, nwg, part);
+      for (int64_t j = p; j < pe; ++j) {
+        hipLaunchKernelGGL(lu_swap_kernel, dim3(grid_for(n)), dim3(kT), 0, c.s, A, lda, n, j, nwg, part, ipiv, info);
+        hipLaunchKernelGGL(lu_update_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, j, pe, nwg, part);
+      }
+      ZOLO_HIP(hipGetLastError());
+      const int64_t nc = PE - pe;  // rest of the outer block
+      if (nc <= 0) continue;
+      ZOLO_CHECK(lu_trsm_block(c, A, lda, p, jb, A + p + pe * lda, nc, L11i, W));
+      GemmDesc d;  // A[pe:, pe:PE] -= L21 U12
+      d.transA = 0; d.transB = 0; d.M = n - pe; d.N = nc; d.K = jb;
+      d.A = A + pe + p * lda; d.lda = lda; d.B = A + p + pe * lda; d.ldb = lda; d.C = A + pe + pe * lda; d.ldc = lda;
+      d.alpha = -1.0; d.beta = 1.0;
+      ZOLO_CHECK(gemm(Prec::F64, d, c.s));
     }
-    ZOLO_HIP(hipGetLastError());
-    const int64_t nc = n - pe;
+    const int64_t nc = n - PE;
     if (nc <= 0) continue;
-    hipLaunchKernelGGL(unit_lower_inv_kernel, dim3(1), dim3(kT), 0, c.s, A + p + p * lda, lda, jb, L11i, (int64_t)kLuNb);
-    GemmDesc d;  // W = L11^{-1} A12
-    d.transA = 0; d.transB = 0; d.M = jb; d.N = nc; d.K = jb;
-    d.A = L11i; d.lda = kLuNb; d.B = A + p + pe * lda; d.ldb = lda; d.C = W; d.ldc = jb; d.aHi = 0;
-    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
-    hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for(jb * nc)), dim3(kT), 0, c.s, W, (int64_t)jb, A + p + pe * lda, lda,
-                       (int64_t)jb, nc);
-    d = GemmDesc();  // A22 -= L21 U12
-    d.transA = 0; d.transB = 0; d.M = nc; d.N = nc; d.K = jb;
-    d.A = A + pe + p * lda; d.lda = lda; d.B = A + p + pe * lda; d.ldb = lda; d.C = A + pe + pe * lda; d.ldc = lda;
+    // U12 = L11^{-1} A12 by blocks of kLuNb rows (forward), then A22 -= L21 U12
+    for (int64_t b = P; b < PE; b += kLuNb) {
+      const int bs = (int)std::min<int64_t>(kLuNb, PE - b);
+      ZOLO_CHECK(lu_trsm_block(c, A, lda, b, bs, A + b + PE * lda, nc, L11i, W));
+      if (b + bs < PE) {
+        GemmDesc d;  // A12[b+bs:PE] -= L[b+bs:PE, b:b+bs] U12[b:b+bs]
+        d.transA = 0; d.transB = 0; d.M = PE - (b + bs); d.N = nc; d.K = bs;
+        d.A = A + (b + bs) + b * lda; d.lda = lda; d.B = A + b + PE * lda; d.ldb = lda;
+        d.C = A + (b + bs) + PE * lda; d.ldc = lda; d.alpha = -1.0; d.beta = 1.0;
+        ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+      }
+    }
+    GemmDesc d;  // A22 -= L21 U12   (K = JB)
+    d.transA = 0; d.transB = 0; d.M = nc; d.N = nc; d.K = JB;
+    d.A = A + PE + P * lda; d.lda = lda; d.B = A + P + PE * lda; d.ldb = lda; d.C = A + PE + PE * lda; d.ldc = lda;
     d.alpha = -1.0; d.beta = 1.0;
     ZOLO_CHECK(gemm(Prec::F64, d, c.s));
   }
