@@ -133,26 +133,30 @@ __global__ __launch_bounds__(kT) void lu_update_kernel(double* A, int64_t lda, i
   }
 }
 
-// Unit-lower inverse of the jb x jb diagonal block (one workgroup).
+// Unit-lower inverse of the jb x jb diagonal block (one workgroup): thread c
+// forms column c by forward substitution, the block and the result in LDS.
 __global__ __launch_bounds__(kT) void unit_lower_inv_kernel(const double* A, int64_t lda, int jb, double* Li,
                                                             int64_t ldl) {
   __shared__ double a[kLuNb][kLuNb + 1];
+  __shared__ double x[kLuNb][kLuNb + 1];
   for (int e = threadIdx.x; e < jb * jb; e += kT) {
     const int i = e % jb, k = e / jb;
     a[i][k] = A[i + k * lda];
+    x[i][k] = (i == k) ? 1.0 : 0.0;
   }
   __syncthreads();
   if ((int)threadIdx.x < jb) {
     const int c = threadIdx.x;
-    double x[kLuNb];
-    for (int i = 0; i < jb; ++i) x[i] = 0.0;
-    x[c] = 1.0;
     for (int i = c + 1; i < jb; ++i) {
       double s = 0.0;
-      for (int k = c; k < i; ++k) s += a[i][k] * x[k];
-      x[i] = -s;
+      for (int k = c; k < i; ++k) s += a[i][k] * x[k][c];
+      x[i][c] = -s;
     }
-    for (int i = 0; i < jb; ++i) Li[i + c * ldl] = x[i];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < jb * jb; e += kT) {
+    const int i = e % jb, k = e / jb;
+    Li[i + k * ldl] = x[i][k];
   }
 }
 
