@@ -120,25 +120,62 @@ __global__ __launch_bounds__(kT) void trd_house_kernel(double* A, int64_t ld, in
   }
 }
 
-// y-partials of A22 v: partial[chunk][r] = sum_{c in chunk} A[r][c] v[c], rows/cols >= i+1.
-constexpr int kGemvChunk = 256;
-__global__ __launch_bounds__(kT) void trd_gemv_kernel(const double* A, int64_t ld, int64_t n, int64_t i,
+// y-partials of A22 v for the symmetric trailing matrix A22 (rows/cols >= i+1),
+// reading only its lower triangle in 64 x 64 tiles: tile (R, C), R >= C,
+// adds A_RC v_C to the rows of R (slot C) and, below the diagonal, A_RC^T v_R
+// to the rows of C (slot R).  Every row r of block b thus gets exactly one
+// partial per slot k < T: partial[k][r].  One workgroup per tile, the tile in LDS.
+constexpr int kSymT = 64;
+__global__ __launch_bounds__(kT) void trd_symv_kernel(const double* A, int64_t ld, int64_t n, int64_t i,
                                                       const double* v, double* partial) {
-  __shared__ double vs[kGemvChunk];
-  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x + i + 1;
-  const int64_t c0 = i + 1 + (int64_t)blockIdx.y * kGemvChunk;
-  const int64_t c1 = std::min<int64_t>(n, c0 + kGemvChunk);
-  for (int64_t c = c0 + threadIdx.x; c < c1; c += kT) vs[c - c0] = v[c];
-  __syncthreads();
-  if (r >= n) return;
-  double acc0 = 0.0, acc1 = 0.0;
-  int64_t c = c0;
-  for (; c + 1 < c1; c += 2) {
-    acc0 += A[r + c * ld] * vs[c - c0];
-    acc1 += A[r + (c + 1) * ld] * vs[c + 1 - c0];
+  __shared__ double tile[kSymT][kSymT + 1];
+  __shared__ double vr[kSymT], vc[kSymT];
+  __shared__ double red[4][kSymT];
+  const int64_t r0 = i + 1;
+  // tile index -> (R, C), R >= C, row-major over the lower triangle
+  const int64_t tix = blockIdx.x;
+  int64_t R = (int64_t)((sqrt(8.0 * (double)tix + 1.0) - 1.0) * 0.5);
+  while ((R + 1) * (R + 2) / 2 <= tix) ++R;
+  while (R * (R + 1) / 2 > tix) --R;
+  const int64_t C = tix - R * (R + 1) / 2;
+  const int64_t rb = r0 + R * kSymT, cb = r0 + C * kSymT;
+  const int t = threadIdx.x;
+  // load the tile (column-major in A: 64 contiguous rows per column)
+  for (int e = t; e < kSymT * kSymT; e += kT) {
+    const int rr = e % kSymT, cc = e / kSymT;
+    const int64_t row = rb + rr, col = cb + cc;
+    tile[rr][cc] = (row < n && col < n) ? A[row + col * ld] : 0.0;
   }
-  if (c < c1) acc0 += A[r + c * ld] * vs[c - c0];
-  partial[(int64_t)blockIdx.y * n + r] = acc0 + acc1;
+  if (t < kSymT) {
+    vr[t] = (rb + t < n) ? v[rb + t] : 0.0;
+    vc[t] = (cb + t < n) ? v[cb + t] : 0.0;
+  }
+  __syncthreads();
+  const int lane = t & 63, q = t >> 6;  // 4 groups of 64: each sums 16 of the 64 terms
+  {  // rows of R: y_r += sum_c tile[r][c] vc[c]
+    double s0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSymT / 4; ++k) {
+      const int c = q * (kSymT / 4) + k;
+      s0 += tile[lane][c] * vc[c];
+    }
+    red[q][lane] = s0;
+  }
+  __syncthreads();
+  if (t < kSymT && rb + t < n) partial[C * n + rb + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  if (R == C) return;
+  __syncthreads();
+  {  // rows of C: y_c += sum_r tile[r][c] vr[r]
+    double s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSymT / 4; ++k) {
+      const int rr = q * (kSymT / 4) + k;
+      s1 += tile[rr][lane] * vr[rr];
+    }
+    red[q][lane] = s1;
+  }
+  __syncthreads();
+  if (t < kSymT && cb + t < n) partial[R * n + cb + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
 }
 
 // w = tau (A22 v - Vp t_w - Wp t_a), t_w = Wp^T v, t_a = Vp^T v (from dpart);
@@ -543,9 +580,9 @@ int ke_trd_column(double* A, int64_t ld, int64_t n, int64_t k0, int64_t i, doubl
   hipLaunchKernelGGL(trd_house_kernel, dim3(nwg), dim3(kT), 0, s, A, ld, n, k0, i, W, w.part, nwg, w.sc, v, tau, e,
                      w.dpart);
   const int64_t m1 = n - i - 1;
-  const unsigned gx = blocks_for(m1), gy = (unsigned)((m1 + kGemvChunk - 1) / kGemvChunk);
-  hipLaunchKernelGGL(trd_gemv_kernel, dim3(gx, gy), dim3(kT), 0, s, A, ld, n, i, v, w.ypart);
-  hipLaunchKernelGGL(trd_w_kernel, dim3(nwg), dim3(kT), 0, s, A, ld, n, k0, i, W, v, w.ypart, (int)gy, w.dpart, nwg, tau,
+  const int64_t T = (m1 + kSymT - 1) / kSymT;
+  hipLaunchKernelGGL(trd_symv_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(kT), 0, s, A, ld, n, i, v, w.ypart);
+  hipLaunchKernelGGL(trd_w_kernel, dim3(nwg), dim3(kT), 0, s, A, ld, n, k0, i, W, v, w.ypart, (int)T, w.dpart, nwg, tau,
                      w.wpart);
   hipLaunchKernelGGL(trd_wfix_kernel, dim3(nwg), dim3(kT), 0, s, W, n, (int)(i - k0), v, w.wpart, nwg, tau, i);
   ZOLO_HIP(hipGetLastError());

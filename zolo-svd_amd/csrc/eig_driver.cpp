@@ -59,7 +59,7 @@ std::vector<Level> plan_levels(int64_t n) {
 
 size_t syevd_bytes(int64_t n) {
   const size_t nn = (size_t)n * n;
-  const size_t chunks = (size_t)((n + 255) / 256);
+  const size_t chunks = (size_t)((n + 63) / 64);  // symmetric matrix-vector partial slots
   size_t b = 0;
   b += sizeof(double) * ((size_t)n * kTrdNb + 5 * (size_t)n + 64 + 8 + 64 * 2 * kTrdNb + 64 + chunks * n);  // trd
   b += sizeof(double) * (2 * nn);                                       // Qn, U
@@ -93,7 +93,7 @@ int syevd_run(int64_t n, double* H, int64_t ldh, double* lambda, double* Z, int6
   tw.sc = ar.take<double>(8);
   tw.dpart = ar.take<double>(64 * 2 * kTrdNb);
   tw.wpart = ar.take<double>(64);
-  tw.ypart = ar.take<double>((size_t)((n + 255) / 256) * n);
+  tw.ypart = ar.take<double>((size_t)((n + 63) / 64) * n);
   ZOLO_ARENA_OK(ar);
   ZOLO_CHECK(ke_fill(W, n, n, kTrdNb, 0.0, s));
   ZOLO_CHECK(ke_fill(e, n, n, 1, 0.0, s));
