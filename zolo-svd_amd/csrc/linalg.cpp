@@ -231,7 +231,7 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
   T* Tx = ar.take<T>((size_t)nout * 256 * 256);
   ZOLO_ARENA_OK(ar);
   ZOLO_CHECK(k_scale_copy<T, T>(X, ldx, Xw, n, n, n, 1.0, c.s));
-  ZOLO_CHECK(dense_geqrf<T>(c, Xw, n, n, taux, Tx, ar));
+  ZOLO_CHECK(dense_geqrf<T>(c, Xw, n, 0, n, 1, taux, Tx, ar));
   // [R; s_q I] = [Q~1_q; Q~2_q] R~_q for every shift; rows sorted large to small
   // (R on top) keep the Householder QR row-wise accurate for tiny s_q
   ZOLO_CHECK(k_build_stack<T>(Xw, n, S, ls, ss, n, d_sdiag, r, c.s, true));
@@ -251,7 +251,7 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
     ZOLO_CHECK(gemm_t<T>(c, d));
   }
   // Z = Q_X Z~
-  ZOLO_CHECK(dense_apply_q<T>(c, Xw, n, n, Tx, Z, ldz, ar));
+  ZOLO_CHECK(dense_apply_q<T>(c, Xw, n, 0, n, 1, Tx, Z, ldz, 0, ar));
   *nz = 1;
   return kOk;
 }

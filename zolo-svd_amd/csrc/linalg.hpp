@@ -101,13 +101,14 @@ template <typename T>
 int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, int64_t sQ, int64_t n, int batch,
                T* tau_out, Arena& ar, QrShape sh = QrShape::Stack);
 
-// Dense n x n Householder QR in place (tau: n, Tall: ceil(n/256) * 256^2 outer
-// T factors) and M <- Q M with the reflectors it left.
+// Dense n x n Householder QR of a batch in place (tau: batch x n, Tall: batch x
+// ceil(n/256) x 256^2 outer T factors) and M <- Q M with the reflectors it left.
 template <typename T>
-size_t dense_qr_bytes(int64_t n);
+size_t dense_qr_bytes(int64_t n, int batch = 1);
 template <typename T>
-int dense_geqrf(const Ctx& c, T* A, int64_t lda, int64_t n, T* tau, T* Tall, Arena& ar);
+int dense_geqrf(const Ctx& c, T* A, int64_t lda, int64_t sA, int64_t n, int batch, T* tau, T* Tall, Arena& ar);
 template <typename T>
-int dense_apply_q(const Ctx& c, const T* A, int64_t lda, int64_t n, const T* Tall, T* M, int64_t ldm, Arena& ar);
+int dense_apply_q(const Ctx& c, const T* A, int64_t lda, int64_t sA, int64_t n, int batch, const T* Tall, T* M,
+                  int64_t ldm, int64_t sM, Arena& ar);
 
 }  // namespace zolo

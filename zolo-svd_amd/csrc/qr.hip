@@ -189,30 +189,69 @@ __device__ inline V wave_colsum64_masked(const V* v, V x, int lo, int hi, int la
   return a[0];
 }
 
+// One butterfly stage over lane halves (W = 32) or 16-lane rows (W = 16) with
+// the lane-swap instructions: permlane32_swap exchanges lanes 32..63 of X with
+// lanes 0..31 of Y (permlane16_swap the odd rows of X with the even rows of
+// Y), so X + Y afterwards holds column "X" in the lower and column "Y" in the
+// upper lanes, summed over each pair of lanes W apart.
+template <int W>
+__device__ inline double swap_add(double X, double Y) {
+  const unsigned xl = __double2loint(X), xh = __double2hiint(X);
+  const unsigned yl = __double2loint(Y), yh = __double2hiint(Y);
+  if constexpr (W == 32) {
+    const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+  } else {
+    const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+  }
+}
+
+// Column sums of x * v[c] over the 64 lanes for all 64 columns: lane l
+// returns the sum of column l.  The first two stages use the lane swaps, the
+// last four shuffles; the products are formed in the first stage.
+__device__ inline double wave_colsum64_swap(const double* v, double x, int lane) {
+  double a[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) a[j] = swap_add<32>(x * v[j], x * v[j + 32]);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) a[j] = swap_add<16>(a[j], a[j + 16]);
+  butterfly_step<8>(a, lane);
+  butterfly_step<4>(a, lane);
+  butterfly_step<2>(a, lane);
+  butterfly_step<1>(a, lane);
+  return a[0];
+}
+
 // Barrier of the nwg workgroups of one shift (all resident: the launch is
 // cooperative).  The counter only grows: arrival k of every workgroup waits
-// until it reads k * nwg.  Device-scope fences publish the partials written
-// before the barrier and make the other workgroups' partials visible after it.
+// until it reads k * nwg.  The arrival is a device-scope release (it publishes
+// the partials written before the barrier); the spin reads the counter
+// relaxed and one device-scope acquire after it makes the other workgroups'
+// partials visible.
 __device__ inline void shift_barrier(unsigned* ctr, unsigned target) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(ctr, 1u);
-    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
-    __threadfence();
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
 
-// Whole panel (jb <= 64 columns) in one cooperative launch: one thread per
-// panel row (rows [p, r1_end) then [n, r2_end)) keeps the row's panel values
-// in registers for all jb columns and stores them once at the end.  Per
-// column: the Gram row g_c = sum_{i>j} S[i][j] S[i][c] and the pivot row go to
-// a per-workgroup slot, one barrier among the workgroups of the shift, then
+// Whole panel (jb <= 64 columns) in one launch: one thread per panel row (rows
+// [p, r1_end) then [n, r2_end)) keeps the row's panel values in registers,
+// shifted so that the current column is always v[0] (no register is selected
+// by the loop counter), and stores each column when it is finished.  Per
+// column: the Gram row g_c = sum_{i>j} S[i][j] S[i][j+c] and the pivot row go
+// to a per-workgroup slot, one barrier among the workgroups of the shift, then
 // every workgroup sums the slots in the same fixed order, forms the reflector
 // (dlarfg) and applies it to its rows.  The slots alternate between two
 // buffers, so the next column's partials never overwrite ones still being
-// read.  Shifts q0 .. q0 + gridDim.y - 1; bar holds one zeroed counter per shift.
+// read.  Shifts q0 .. q0 + gridDim.y - 1; bar holds one zeroed counter per
+// shift, or is null when nwg == 1 (no barrier between workgroups).
 constexpr int kCoopT = 512;
 constexpr int kCoopMaxSlots = 8;  // partial slots per thread when summing: nwg <= 8 * (kCoopT / 64)
 
@@ -232,40 +271,41 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
   const bool valid = l < len1 + (r2_end - n);
   const int64_t rec = (int64_t)nwg * NBP + NBP;
   double* base = part + (int64_t)qi * 2 * rec;
+  // v[c]: the row's value in panel column jj + c; zero for jj + c >= jb
   double v[NBP];
 #pragma unroll
   for (int c = 0; c < NBP; ++c) v[c] = (valid && c < jb) ? (double)Sq[i + (p + c) * lds] : 0.0;
+  T* out = Sq + i + p * lds;
   for (int jj = 0; jj < jb; ++jj) {
     const int64_t j = p + jj;
+    const int rem = jb - jj;  // columns jj .. jb - 1 are v[0 .. rem)
     double* G = base + (int64_t)(jj & 1) * rec;
     // bottom rows beyond n + j take no part in the reflector of column j
     const bool live = valid && i >= j && (i < r1_end || i <= n + j);
     if (live && i == j) {
 #pragma unroll
       for (int c = 0; c < NBP; ++c)
-        if (c >= jj && c < jb) G[(int64_t)nwg * NBP + c] = v[c];
+        if (c < rem) G[(int64_t)nwg * NBP + c] = v[c];
     }
-    double vj = 0.0;
-#pragma unroll
-    for (int c = 0; c < NBP; ++c)
-      if (c == jj) vj = v[c];
-    const double x = (live && i > j) ? vj : 0.0;
-    red[wave][lane] = wave_colsum64_masked(v, x, jj, jb, lane);
+    const double x = (live && i > j) ? v[0] : 0.0;
+    // lane 0: the squared norm below the diagonal; lanes c >= 1: the Gram row
+    red[wave][lane] = wave_colsum64_swap(v, x, lane);
     __syncthreads();
-    if (t >= jj && t < jb) {
+    if (t < rem) {
       double sum = 0.0;
 #pragma unroll
       for (int k = 0; k < kCoopT / 64; ++k) sum += red[k][t];
       G[(int64_t)w * NBP + t] = sum;
     }
-    shift_barrier(bar + qi, (unsigned)(jj + 1) * (unsigned)nwg);
+    if (bar) shift_barrier(bar + qi, (unsigned)(jj + 1) * (unsigned)nwg);
+    else __syncthreads();  // one workgroup per shift
     {  // sum the nwg partials of each column: 8 groups of 64 threads, all loads independent
       const int c = t & 63, g = t >> 6;
       double pv[kCoopMaxSlots];
 #pragma unroll
       for (int m = 0; m < kCoopMaxSlots; ++m) {
         const int k = g + m * (kCoopT / 64);
-        pv[m] = (c >= jj && c < jb && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
+        pv[m] = (c < rem && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
       }
       double sum = 0.0;
 #pragma unroll
@@ -273,7 +313,7 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
       red[g][c] = sum;
     }
     __syncthreads();
-    if (t >= jj && t < jb) {
+    if (t < rem) {
       double sum = 0.0;
 #pragma unroll
       for (int g = 0; g < kCoopT / 64; ++g) sum += red[g][t];
@@ -282,7 +322,7 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
     }
     __syncthreads();
     if (t == 0) {
-      const double alpha = rr[jj], xn2 = gs[jj];
+      const double alpha = rr[0], xn2 = gs[0];
       double beta = alpha, tv = 0.0, scal = 0.0;
       if (xn2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
@@ -293,23 +333,22 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
       if (w == 0) tau[(int64_t)q * strideTau + j] = (T)tv;
     }
     __syncthreads();
-    if (t > jj && t < jb) wv[t] = sc[1] * (rr[t] + sc[2] * gs[t]);
+    // w_c = tau (r_c + scal g_c): the update of column c is v_i w_c; zero past the panel
+    if (t < NBP) wv[t] = (t >= 1 && t < rem) ? sc[1] * (rr[t] + sc[2] * gs[t]) : 0.0;
     __syncthreads();
     if (live) {
-      const double vi = (i == j) ? 1.0 : vj * sc[2];
-      const double newj = (i == j) ? sc[0] : vi;
+      const double vi = (i == j) ? 1.0 : v[0] * sc[2];
+      v[0] = (i == j) ? sc[0] : vi;
 #pragma unroll
-      for (int c = 0; c < NBP; ++c) {
-        if (c == jj) v[c] = newj;
-        if (c > jj && c < jb) v[c] -= vi * wv[c];
-      }
+      for (int c = 1; c < NBP; ++c) v[c] -= vi * wv[c];
     }
-    __syncthreads();  // red, gs, rr, wv and sc are rewritten for the next column
-  }
-  if (valid) {
+    // column jj is final in every row (rows above j were finished by earlier reflectors)
+    if (valid) *out = (T)v[0];
+    out += lds;
 #pragma unroll
-    for (int c = 0; c < NBP; ++c)
-      if (c < jb) Sq[i + (p + c) * lds] = (T)v[c];
+    for (int c = 0; c + 1 < NBP; ++c) v[c] = v[c + 1];
+    v[NBP - 1] = 0.0;
+    __syncthreads();  // red, gs, rr, wv and sc are rewritten for the next column
   }
 }
 
@@ -489,6 +528,28 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   if (jb > NBP || jb <= 0 || r1_end < p + jb || r1_end > n || r2_end < n || r2_end > n + p + jb) {
     set_error("qr panel (cooperative): width or row ranges out of range");
     return kErrArg;
+  }
+  const int64_t rows0 = (r1_end - p) + (r2_end - n);
+  if (rows0 <= kCoopT) {
+    // one workgroup per shift: no barrier between workgroups, an ordinary
+    // launch (also inside a graph capture)
+    if ((int64_t)batch * 2 * (2 * NBP) > part_doubles) {
+      set_error("qr panel (one workgroup): partials buffer too small");
+      return kErrArg;
+    }
+    int one = 1;
+    unsigned* nobar = nullptr;
+    int q0 = 0;
+    hipLaunchKernelGGL((qr_panel_coop_kernel<T>), dim3(1, batch), dim3(kCoopT), 0, s, S, lds, strideS, n, p, jb,
+                       r1_end, r2_end, part, nobar, one, tau, strideTau, q0);
+    ZOLO_HIP(hipGetLastError());
+    *done = true;
+    return kOk;
+  }
+  {  // cooperative launches cannot be captured into a graph: take the per-column kernels there
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ZOLO_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return kOk;
   }
   // workgroups one launch may keep resident, per device
   static int cap[64] = {0};

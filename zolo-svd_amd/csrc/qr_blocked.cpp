@@ -255,20 +255,23 @@ int orgqr_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* tau, 
 // M <- Q M for the Q of a dense n x n GEQRF (reflectors and outer-block T
 // factors from geqrf_impl), M n x n: the outer blocks from last to first.
 template <typename T>
-int apply_q_dense_impl(const Ctx& c, const T* S, int64_t lds, const T* Tall, T* M, int64_t ldm, int64_t n, Arena& ar) {
+int apply_q_dense_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const T* Tall, T* M, int64_t ldm,
+                       int64_t sM, int64_t n, int batch, Arena& ar) {
   const int nbo = kQrOuter;
   QrScratch q;
-  ZOLO_CHECK(take_scratch<T>(ar, n, 1, q));
+  ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
   T* V = (T*)q.V;
-  const int64_t sG = (int64_t)nbo * nbo;
+  const int64_t mr = n + nbo;
+  const int64_t sV = mr * nbo, sG = (int64_t)nbo * nbo, sW = (int64_t)nbo * n;
   const int64_t nout = (n + nbo - 1) / nbo;
+  const int64_t sTall = nout * sG;
   for (int64_t ob = nout - 1; ob >= 0; --ob) {
     const int64_t P = ob * nbo;
     const int JB = (int)std::min<int64_t>(nbo, n - P);
     const Rows ro = rows_of(QrShape::Dense, n, P, JB);
-    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, 0, V, ro.m(), 0, P, JB, n, ro.r1, ro.r2, 1, c.s));
-    ZOLO_CHECK(apply_block_reflector<T>(c, V, ro.m(), 0, Tall + ob * sG, nbo, 0, JB, M + P, ldm, 0, ro.len1, n - P, 0,
-                                        P, n, false, (T*)q.W, (T*)q.W2, 0, 1));
+    ZOLO_CHECK(k_qr_extract_v<T>(S, lds, sS, V, ro.m(), sV, P, JB, n, ro.r1, ro.r2, batch, c.s));
+    ZOLO_CHECK(apply_block_reflector<T>(c, V, ro.m(), sV, Tall + ob * sG, nbo, sTall, JB, M + P, ldm, sM, ro.len1,
+                                        n - P, 0, P, n, false, (T*)q.W, (T*)q.W2, sW, batch));
   }
   return kOk;
 }
@@ -354,28 +357,30 @@ int stacked_qr(const Ctx& c, T* S, int64_t lds, int64_t sS, T* Q, int64_t ldq, i
 }
 
 template <typename T>
-size_t dense_qr_bytes(int64_t n) {
+size_t dense_qr_bytes(int64_t n, int batch) {
   const int64_t nout = (n + kQrOuter - 1) / kQrOuter;
-  return stacked_qr_bytes<T>(n, 1) + sizeof(T) * ((size_t)nout * kQrOuter * kQrOuter + n) + 512;
+  return stacked_qr_bytes<T>(n, batch) + sizeof(T) * (size_t)batch * ((size_t)nout * kQrOuter * kQrOuter + n) + 512;
 }
 
 template <typename T>
-int dense_geqrf(const Ctx& c, T* A, int64_t lda, int64_t n, T* tau, T* Tall, Arena& ar) {
-  return geqrf_impl<T>(c, A, lda, 0, n, 1, tau, Tall, ar, QrShape::Dense);
+int dense_geqrf(const Ctx& c, T* A, int64_t lda, int64_t sA, int64_t n, int batch, T* tau, T* Tall, Arena& ar) {
+  return geqrf_impl<T>(c, A, lda, sA, n, batch, tau, Tall, ar, QrShape::Dense);
 }
 
 template <typename T>
-int dense_apply_q(const Ctx& c, const T* A, int64_t lda, int64_t n, const T* Tall, T* M, int64_t ldm, Arena& ar) {
-  return apply_q_dense_impl<T>(c, A, lda, Tall, M, ldm, n, ar);
+int dense_apply_q(const Ctx& c, const T* A, int64_t lda, int64_t sA, int64_t n, int batch, const T* Tall, T* M,
+                  int64_t ldm, int64_t sM, Arena& ar) {
+  return apply_q_dense_impl<T>(c, A, lda, sA, Tall, M, ldm, sM, n, batch, ar);
 }
 
 #define ZOLO_QRB_INST(T)                                                                                            \
   template size_t stacked_qr_bytes<T>(int64_t, int);                                                                \
   template int stacked_qr<T>(const Ctx&, T*, int64_t, int64_t, T*, int64_t, int64_t, int64_t, int, T*, Arena&,      \
                              QrShape);                                                                             \
-  template size_t dense_qr_bytes<T>(int64_t);                                                                       \
-  template int dense_geqrf<T>(const Ctx&, T*, int64_t, int64_t, T*, T*, Arena&);                                   \
-  template int dense_apply_q<T>(const Ctx&, const T*, int64_t, int64_t, const T*, T*, int64_t, Arena&);            \
+  template size_t dense_qr_bytes<T>(int64_t, int);                                                                  \
+  template int dense_geqrf<T>(const Ctx&, T*, int64_t, int64_t, int64_t, int, T*, T*, Arena&);                     \
+  template int dense_apply_q<T>(const Ctx&, const T*, int64_t, int64_t, int64_t, int, const T*, T*, int64_t,       \
+                                int64_t, Arena&);                                                                  \
   template int stacked_geqrf<T>(const Ctx&, T*, int64_t, int64_t, int64_t, int, T*, Arena&);                        \
   template int stacked_orgqr<T>(const Ctx&, const T*, int64_t, int64_t, const T*, T*, int64_t, int64_t, int64_t,   \
                                 int, Arena&);
