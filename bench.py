@@ -79,9 +79,14 @@ def make_workload(torch, n, seed, factors="haar"):
 
 
 def run_pd(zl, dA, prec, dU, dH):
+    import torch
+    t0 = time.perf_counter()
     al, be = zl.bounds_device(dA, "zolo")
+    torch.cuda.synchronize()
+    t_bounds = (time.perf_counter() - t0) * 1e3
     rk = zl.choosem(1.0 / be)
     _, _, info = zl.pzolopd1_device(dA, al, be, rk, precision=prec, dU=dU, dH=dH)
+    info.phase_ms["bounds"] = t_bounds
     return info
 
 
@@ -130,6 +135,22 @@ def cpu_baseline(n, schedule, r):
             "sample": f"one shifted Cholesky solve (Gram + POTRF + POTRS, n={ns}, numpy/scipy FP64) timed "
                       f"{dt:.2f} s = {rate / 1e9:.0f} GFLOP/s, extrapolated to the executed schedule "
                       f"{schedule} with r={r} using dense LAPACK flop counts"}
+
+
+def gemm_classes(zl):
+    """Every GEMM class launched in the timed region: launches, time, achieved TFLOP/s."""
+    rows = []
+    for prec in (0, 1, 2):
+        for ta in (0, 1):
+            for tb in (0, 1):
+                for tm, tn in ((128, 128), (64, 128), (128, 64), (64, 64), (256, 128)):
+                    cnt, ms, fl = zl.profile_gemm(prec, ta, tb, tm, tn)
+                    if cnt:
+                        exec_fl = fl * (3 if prec == 2 else 1)
+                        rows.append({"kernel": f"{PREC_NAME[prec]} op({'NT'[ta]},{'NT'[tb]}) {tm}x{tn}",
+                                     "launches": cnt, "ms": round(ms, 1),
+                                     "tflops": round(exec_fl / (ms * 1e-3) / 1e12, 1)})
+    return sorted(rows, key=lambda r: -r["ms"])
 
 
 def roofline(zl):
@@ -291,6 +312,7 @@ def main():
     t1 = time.perf_counter()
     sec = (t1 - t0) / args.steps
     rl = roofline(zl)
+    classes = gemm_classes(zl)
     zl.profile_enable(False)
     info = infos[-1]
     err = errors(torch, A, dU, dH, Ug, Vg)
@@ -347,6 +369,7 @@ def main():
         "phase_ms": {k: round(v, 1) for k, v in info.phase_ms.items() if v > 0},
         "compare": compare,
         "roofline": rl,
+        "gemm_classes": classes,
         "cpu_baseline": cpu,
     }
     if small:

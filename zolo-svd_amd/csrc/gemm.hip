@@ -170,7 +170,7 @@ __device__ inline void k_range(const GemmDesc& d, int64_t i0, int64_t j0, int64_
 // each wave owning FM x FN blocks of 16x16 MFMA outputs.
 // ---------------------------------------------------------------------------
 template <typename T, int TA, int TB, int TBM, int TBN>
-__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
+__global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmDesc d) {
   constexpr int BK = 16;
   constexpr int PAD = (sizeof(T) == 8) ? 4 : 8;
   constexpr int WM = (TBM == TBN) ? 2 : (TBM == 64 ? 1 : 4);
@@ -338,12 +338,54 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
   // rows of a column: stores and the beta loads are 128-byte (FP64) or
   // 64-byte (FP32) contiguous segments.  Lane layout of the 16x16x4 outputs:
   // column 4*v + lane/16 (f64) or 4*(lane/16) + v (f32), row lane%16.
+  // With beta != 0 every row group reads all its C values before it writes
+  // any: the loads go out together and are waited for once (a load after a
+  // store to C could alias it, so the compiler would otherwise wait for each
+  // element in turn).
   const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
   const T al = T(alpha), be = T(d.beta);
+  const bool rd = d.beta != 0.0;
+  if (interior) {  // no bounds checks: straight-line loads and stores
+#pragma unroll
+    for (int m = 0; m < FM; ++m) {
+      T* Cr = C + i0 + wm * (FM * 16) + m * 16 + (lane & 15);
+      T cv[FN][4];
+      if (rd) {
+#pragma unroll
+        for (int n = 0; n < FN; ++n)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int64_t col = j0 + wn * (FN * 16) + n * 16 +
+                                ((sizeof(T) == 8) ? (4 * v + (lane >> 4)) : ((lane >> 4) * 4 + v));
+            cv[n][v] = Cr[col * d.ldc];
+          }
+      }
+#pragma unroll
+      for (int n = 0; n < FN; ++n)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t col = j0 + wn * (FN * 16) + n * 16 +
+                              ((sizeof(T) == 8) ? (4 * v + (lane >> 4)) : ((lane >> 4) * 4 + v));
+          const T val = al * acc[m][n][v];
+          Cr[col * d.ldc] = rd ? val + be * cv[n][v] : val;
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < FM; ++m) {
     const int64_t row = i0 + wm * (FM * 16) + m * 16 + (lane & 15);
     if (row >= d.M) continue;
+    T cv[FN][4];
+#pragma unroll
+    for (int n = 0; n < FN; ++n) {
+      const int64_t colb = j0 + wn * (FN * 16) + n * 16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t col = colb + ((sizeof(T) == 8) ? (4 * v + (lane >> 4)) : ((lane >> 4) * 4 + v));
+        cv[n][v] = (rd && col < d.N) ? C[row + col * d.ldc] : T(0);
+      }
+    }
 #pragma unroll
     for (int n = 0; n < FN; ++n) {
       const int64_t colb = j0 + wn * (FN * 16) + n * 16;
@@ -351,9 +393,8 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmDesc d) {
       for (int v = 0; v < 4; ++v) {
         const int64_t col = colb + ((sizeof(T) == 8) ? (4 * v + (lane >> 4)) : ((lane >> 4) * 4 + v));
         if (col < d.N) {
-          T* p = C + row + col * d.ldc;
           const T val = al * acc[m][n][v];
-          *p = (d.beta == 0.0) ? val : val + be * (*p);
+          C[row + col * d.ldc] = rd ? val + be * cv[n][v] : val;
         }
       }
     }
@@ -539,12 +580,42 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
   }
 
   // lane layout of the 16x16x32 outputs (C^T blocks): row lane%16, column 4*(lane/16) + v
+  // (beta != 0: each row group's C values are all read before any is written)
   const double alpha = d.alphaVec ? d.alpha * d.alphaVec[z] : d.alpha;
   const float alf = float(alpha), be = float(d.beta);
+  const bool rd = d.beta != 0.0;
+  if (i0 + kB3M <= d.M && j0 + kB3N <= d.N) {  // interior tile: straight-line loads and stores
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      float* Cr = C + i0 + wm * 64 + m * 16 + (lane & 15);
+      float cv[4][4];
+      if (rd) {
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) cv[n][v] = Cr[(j0 + wn * 64 + n * 16 + (lane >> 4) * 4 + v) * d.ldc];
+      }
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float val = alf * acc[m][n][v];
+          Cr[(j0 + wn * 64 + n * 16 + (lane >> 4) * 4 + v) * d.ldc] = rd ? val + be * cv[n][v] : val;
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const int64_t row = i0 + wm * 64 + m * 16 + (lane & 15);
     if (row >= d.M) continue;
+    float cv[4][4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const int64_t colb = j0 + wn * 64 + n * 16 + (lane >> 4) * 4;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cv[n][v] = (rd && colb + v < d.N) ? C[row + (colb + v) * d.ldc] : 0.f;
+    }
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int64_t colb = j0 + wn * 64 + n * 16 + (lane >> 4) * 4;
@@ -552,9 +623,8 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
       for (int v = 0; v < 4; ++v) {
         const int64_t col = colb + v;
         if (col < d.N) {
-          float* p = C + row + col * d.ldc;
           const float val = alf * acc[m][n][v];
-          *p = (d.beta == 0.0) ? val : val + be * (*p);
+          C[row + col * d.ldc] = rd ? val + be * cv[n][v] : val;
         }
       }
     }
