@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "driver.hpp"
+#include "grid_sync.hpp"
 
 namespace zolo {
 namespace {
@@ -133,6 +134,158 @@ __global__ __launch_bounds__(kT) void lu_update_kernel(double* A, int64_t lda, i
   }
 }
 
+// Whole LU panel (columns [p, p + jb), jb <= 64, rows [p, n)) in one launch:
+// one thread per row keeps the row's panel values in registers, shifted so
+// that the current column is always v[0], and stores each column when it is
+// final.  Per column: the workgroup's argmax of |v[0]| over rows >= j (ties to
+// the smaller row, as IDAMAX) and that row go to the workgroup's slot, row j
+// to a shared slot; one barrier among the workgroups; every workgroup picks
+// the winner of the slots in the same order, swaps the pivot row and row j in
+// registers and applies the rank-1 update (dgetf2).  Columns that are already
+// final, and the columns outside the panel, get the row interchanges
+// afterwards (lu_laswp_panel_kernel).  Slots alternate between two buffers;
+// bar is a zeroed counter, or null when nwg == 1.
+constexpr int kLuCoopT = 512;
+constexpr int kLuSlot = kLuNb + 2;  // candidate row, |value|, row index
+constexpr int kLuMaxWg = 64;        // workgroups of one panel: the winner is reduced by one wave
+// doubles of the panel slots (two buffers) followed by the barrier counter
+constexpr int64_t kLuSlotDoubles = 2 * ((int64_t)kLuMaxWg * kLuSlot + kLuNb) + 8;
+
+__global__ __launch_bounds__(kLuCoopT) void lu_panel_coop_kernel(double* A, int64_t lda, int64_t n, int64_t p, int jb,
+                                                                 double* slots, unsigned* bar, int nwg,
+                                                                 int64_t* ipiv, int* info) {
+  __shared__ double wbv[kLuCoopT / 64], prow[kLuNb], rj[kLuNb], cand[kLuNb], rowj[kLuNb];
+  __shared__ int64_t wbi[kLuCoopT / 64];
+  __shared__ int64_t s_win, s_piv;
+  __shared__ int s_slot;
+  __shared__ double s_bv, s_wbv;
+  const int t = threadIdx.x, w = blockIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t i = p + (int64_t)w * kLuCoopT + t;
+  const bool valid = i < n;
+  const int64_t rec = (int64_t)nwg * kLuSlot + kLuNb;
+  double v[kLuNb];
+#pragma unroll
+  for (int c = 0; c < kLuNb; ++c) v[c] = (valid && c < jb) ? A[i + (p + c) * lda] : 0.0;
+  double* out = A + i + p * lda;
+  for (int jj = 0; jj < jb; ++jj) {
+    const int64_t j = p + jj;
+    const int rem = jb - jj;
+    double* G = slots + (int64_t)(jj & 1) * rec;
+    double bv = -1.0;
+    int64_t bi = INT64_MAX;
+    if (valid && i >= j) { bv = fabs(v[0]); bi = i; }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o, 64);
+      const int64_t oi = __shfl_xor(bi, o, 64);
+      if (piv_better(ov, oi, bv, bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { wbv[wave] = bv; wbi[wave] = bi; }
+    if (valid && i == j) {
+#pragma unroll
+      for (int c = 0; c < kLuNb; ++c) rowj[c] = v[c];
+    }
+    __syncthreads();
+    if (t == 0) {
+      double b = wbv[0];
+      int64_t b_i = wbi[0];
+      for (int k = 1; k < kLuCoopT / 64; ++k)
+        if (piv_better(wbv[k], wbi[k], b, b_i)) { b = wbv[k]; b_i = wbi[k]; }
+      s_wbv = b;
+      s_win = b_i;
+    }
+    __syncthreads();
+    if (valid && i == s_win) {
+#pragma unroll
+      for (int c = 0; c < kLuNb; ++c) cand[c] = v[c];
+    }
+    __syncthreads();
+    // wave 0 publishes the slots (the barrier's release is issued by its thread 0)
+    if (t < kLuNb) {
+      const bool has = s_win != INT64_MAX;  // this workgroup has rows >= j
+      if (t < rem) G[(int64_t)w * kLuSlot + t] = has ? cand[t] : 0.0;
+      if (t < rem && (j - p) / kLuCoopT == w) G[(int64_t)nwg * kLuSlot + t] = rowj[t];
+      if (t == 0) {
+        G[(int64_t)w * kLuSlot + kLuNb] = s_wbv;
+        G[(int64_t)w * kLuSlot + kLuNb + 1] = (double)s_win;  // row indices are exact in a double
+      }
+    }
+    if (bar) grid_barrier(bar, (unsigned)(jj + 1) * (unsigned)nwg);
+    else __syncthreads();
+    if (t < 64) {  // winner among the workgroups' candidates (nwg <= 64), fixed order
+      double b = -1.0;
+      int64_t b_i = INT64_MAX;
+      int slot = 0;
+      if (t < nwg) {
+        b = G[(int64_t)t * kLuSlot + kLuNb];
+        b_i = (int64_t)G[(int64_t)t * kLuSlot + kLuNb + 1];
+        slot = t;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(b, o, 64);
+        const int64_t oi = __shfl_xor(b_i, o, 64);
+        const int os = __shfl_xor(slot, o, 64);
+        if (piv_better(ob, oi, b, b_i)) { b = ob; b_i = oi; slot = os; }
+      }
+      if (t == 0) { s_piv = b_i; s_slot = slot; s_bv = b; }
+    }
+    __syncthreads();
+    const int64_t piv = s_piv;
+    if (t < kLuNb) {
+      prow[t] = (t < rem) ? G[(int64_t)s_slot * kLuSlot + t] : 0.0;
+      rj[t] = (t < rem) ? G[(int64_t)nwg * kLuSlot + t] : 0.0;
+    }
+    if (w == 0 && t == 0) {
+      ipiv[j] = piv;
+      if (s_bv == 0.0 && *info == 0) *info = (int)(j + 1);
+    }
+    __syncthreads();
+    if (valid && piv != j) {
+      if (i == j) {
+#pragma unroll
+        for (int c = 0; c < kLuNb; ++c) v[c] = prow[c];
+      } else if (i == piv) {
+#pragma unroll
+        for (int c = 0; c < kLuNb; ++c) v[c] = rj[c];
+      }
+    }
+    const double d = prow[0];
+    if (valid && i > j && d != 0.0) {
+      const double l = v[0] / d;
+      v[0] = l;
+#pragma unroll
+      for (int c = 1; c < kLuNb; ++c) v[c] -= l * prow[c];
+    }
+    if (valid) *out = v[0];
+    out += lda;
+#pragma unroll
+    for (int c = 0; c + 1 < kLuNb; ++c) v[c] = v[c + 1];
+    v[kLuNb - 1] = 0.0;
+    __syncthreads();  // LDS and the winner are rewritten for the next column
+  }
+}
+
+// Row interchanges of one factored panel (pivots ipiv[p .. p + jb), in order)
+// for every column c of the matrix except what the panel kernel already
+// swapped: panel column p + cc only takes the interchanges of steps after cc.
+// One thread per column.
+__global__ __launch_bounds__(kT) void lu_laswp_panel_kernel(double* A, int64_t lda, int64_t n, int64_t p, int jb,
+                                                            const int64_t* ipiv) {
+  const int64_t c = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (c >= n) return;
+  const int64_t first = (c >= p && c < p + jb) ? c - p + 1 : 0;
+  double* col = A + c * lda;
+  for (int64_t jj = first; jj < jb; ++jj) {
+    const int64_t j = p + jj, pv = ipiv[j];
+    if (pv != j) {
+      const double a = col[j];
+      col[j] = col[pv];
+      col[pv] = a;
+    }
+  }
+}
+
 // Unit-lower inverse of the jb x jb diagonal block (one workgroup): thread c
 // forms column c by forward substitution, the block and the result in LDS.
 __global__ __launch_bounds__(kT) void unit_lower_inv_kernel(const double* A, int64_t lda, int jb, double* Li,
@@ -171,9 +324,6 @@ __global__ __launch_bounds__(kT) void copy2d_kernel(const double* S, int64_t lds
 
 // ---- triangular solves with one right-hand side --------------------------
 // op(T)[i][k] = trans ? M[k + i*ld] : M[i + k*ld]
-__device__ inline double opT(const double* M, int64_t ld, int trans, int64_t i, int64_t k) {
-  return trans ? M[k + i * ld] : M[i + k * ld];
-}
 
 // Inverses of the kTb x kTb diagonal blocks of the LU factors, one workgroup
 // per block: Linv_b = L_bb^{-1} (unit lower), Uinv_b = U_bb^{-1} (upper).
@@ -220,25 +370,59 @@ __global__ __launch_bounds__(kTb) void lu_block_inv_kernel(const double* LU, int
 // x_b' = op(D_b)^{-1} x_b from the block's explicit inverse (Dinv, op =
 // transposed when dtrans), workgroup 0 writes it to y, and rows r in [r0, r1)
 // get x[r] -= sum_k op(M)[r][b0+k] x_b'[k].  x_b itself is not written, so the
-// workgroups never race on it.
+// workgroups never race on it.  op(M) = M (column-oriented update): 64 rows per
+// pass, the 64 block columns split in four quarters summed through LDS.
+// op(M) = M^T: 32 rows per pass, eight lanes per row reading the row's 64
+// contiguous values of M (k = lane + 8 e) and summed by shuffles.
+constexpr int kTrsvRowsN = 64, kTrsvRowsT = 32;
 __global__ __launch_bounds__(kT) void trsv_step_kernel(const double* M, int64_t ld, int trans, const double* Dinv,
                                                        int dtrans, double* x, double* y, int64_t b0, int bs,
                                                        int64_t r0, int64_t r1) {
   __shared__ double xb[kTb], xs[kTb];
+  __shared__ double part[kT / kTrsvRowsN][kTrsvRowsN];
   const int t = threadIdx.x;
   if (t < bs) xb[t] = x[b0 + t];
   __syncthreads();
-  if (t < bs) {
+  if (t < kTb) {
     double sum = 0.0;
-    for (int k = 0; k < bs; ++k) sum += (dtrans ? Dinv[k + t * kTb] : Dinv[t + k * kTb]) * xb[k];
-    xs[t] = sum;
-    if (blockIdx.x == 0) y[b0 + t] = sum;
+    if (t < bs)
+      for (int k = 0; k < bs; ++k) sum += (dtrans ? Dinv[k + t * kTb] : Dinv[t + k * kTb]) * xb[k];
+    xs[t] = sum;  // zero past the block
+    if (blockIdx.x == 0 && t < bs) y[b0 + t] = sum;
   }
   __syncthreads();
-  for (int64_t r = r0 + (int64_t)blockIdx.x * kT + t; r < r1; r += (int64_t)gridDim.x * kT) {
-    double sum = 0.0;
-    for (int k = 0; k < bs; ++k) sum += opT(M, ld, trans, r, b0 + k) * xs[k];
-    x[r] -= sum;
+  if (!trans) {
+    const int rl = t % kTrsvRowsN, q = t / kTrsvRowsN;
+    for (int64_t base = r0 + (int64_t)blockIdx.x * kTrsvRowsN; base < r1; base += (int64_t)gridDim.x * kTrsvRowsN) {
+      const int64_t r = base + rl;
+      double sum = 0.0;
+      if (r < r1) {
+        const double* Mr = M + r + (b0 + q * 16) * ld;
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+          if (q * 16 + kk < bs) sum += Mr[kk * ld] * xs[q * 16 + kk];
+      }
+      part[q][rl] = sum;
+      __syncthreads();
+      if (q == 0 && r < r1) x[r] -= (part[0][rl] + part[1][rl]) + (part[2][rl] + part[3][rl]);
+      __syncthreads();
+    }
+  } else {
+    const int rl = t >> 3, q = t & 7;
+    for (int64_t base = r0 + (int64_t)blockIdx.x * kTrsvRowsT; base < r1; base += (int64_t)gridDim.x * kTrsvRowsT) {
+      const int64_t r = base + rl;
+      double sum = 0.0;
+      if (r < r1) {
+        const double* Mc = M + b0 + r * ld;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (q + 8 * e < bs) sum += Mc[q + 8 * e] * xs[q + 8 * e];
+      }
+      sum += __shfl_xor(sum, 1, 8);
+      sum += __shfl_xor(sum, 2, 8);
+      sum += __shfl_xor(sum, 4, 8);
+      if (q == 0 && r < r1) x[r] -= sum;
+    }
   }
 }
 
@@ -305,8 +489,48 @@ int lu_trsm_block(const Ctx& c, double* A, int64_t lda, int64_t p, int jb, doubl
 // the trailing matrix with K = kLuOuter (A22 -= L21 U12 on MFMA).
 constexpr int kLuOuter = 256;
 
+// Panel [p, p + jb) in one launch of lu_panel_coop_kernel followed by the
+// row interchanges of the other columns; *done = false when the rows do not
+// fit one resident grid (the caller then takes the per-column kernels).
+int lu_panel_coop(const Ctx& c, double* A, int64_t lda, int64_t n, int64_t p, int jb, int64_t* ipiv, int* info,
+                  double* slots, bool* done) {
+  *done = false;
+  static int cap[64] = {0};
+  int dev = 0;
+  ZOLO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return kOk;
+  if (cap[dev] == 0) {
+    int per_cu = 0, cus = 0, coop = 0;
+    ZOLO_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    ZOLO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    ZOLO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_coop_kernel, kLuCoopT, 0));
+    cap[dev] = coop ? per_cu * cus : -1;
+    if (cap[dev] <= 0) cap[dev] = -1;
+  }
+  const int64_t rows = n - p;
+  int nwg = (int)((rows + kLuCoopT - 1) / kLuCoopT);
+  if (nwg > kLuMaxWg) return kOk;
+  unsigned* bar = reinterpret_cast<unsigned*>(slots + 2 * ((int64_t)kLuMaxWg * kLuSlot + kLuNb));
+  if (nwg == 1) {
+    unsigned* nobar = nullptr;
+    hipLaunchKernelGGL(lu_panel_coop_kernel, dim3(1), dim3(kLuCoopT), 0, c.s, A, lda, n, p, jb, slots, nobar, nwg,
+                       ipiv, info);
+    ZOLO_HIP(hipGetLastError());
+  } else {
+    if (cap[dev] < nwg) return kOk;
+    ZOLO_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned), c.s));
+    void* args[] = {&A, &lda, &n, &p, &jb, &slots, &bar, &nwg, &ipiv, &info};
+    ZOLO_HIP(hipLaunchCooperativeKernel((const void*)lu_panel_coop_kernel, dim3(nwg), dim3(kLuCoopT), args, 0, c.s));
+  }
+  hipLaunchKernelGGL(lu_laswp_panel_kernel, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, c.s, A, lda, n, p, jb,
+                     (const int64_t*)ipiv);
+  ZOLO_HIP(hipGetLastError());
+  *done = true;
+  return kOk;
+}
+
 int getrf(const Ctx& c, double* A, int64_t lda, int64_t n, int64_t* ipiv, int* info, Piv* part, double* L11i,
-          double* W) {
+          double* W, double* slots) {
   const int nwg = lu_nwg(n);
   for (int64_t P = 0; P < n; P += kLuOuter) {
     const int JB = (int)std::min<int64_t>(kLuOuter, n - P);
@@ -314,10 +538,15 @@ int getrf(const Ctx& c, double* A, int64_t lda, int64_t n, int64_t* ipiv, int* i
     for (int64_t p = P; p < PE; p += kLuNb) {
       const int jb = (int)std::min<int64_t>(kLuNb, PE - p);
       const int64_t pe = p + jb;
-      hipLaunchKernelGGL(lu_argmax_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, p, nwg, part);
-      for (int64_t j = p; j < pe; ++j) {
-        hipLaunchKernelGGL(lu_swap_kernel, dim3(grid_for(n)), dim3(kT), 0, c.s, A, lda, n, j, nwg, part, ipiv, info);
-        hipLaunchKernelGGL(lu_update_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, j, pe, nwg, part);
+      bool done = false;
+      ZOLO_CHECK(lu_panel_coop(c, A, lda, n, p, jb, ipiv, info, slots, &done));
+      if (!done) {
+        hipLaunchKernelGGL(lu_argmax_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, p, nwg, part);
+        for (int64_t j = p; j < pe; ++j) {
+          hipLaunchKernelGGL(lu_swap_kernel, dim3(grid_for(n)), dim3(kT), 0, c.s, A, lda, n, j, nwg, part, ipiv,
+                             info);
+          hipLaunchKernelGGL(lu_update_kernel, dim3(nwg), dim3(kT), 0, c.s, A, lda, n, j, pe, nwg, part);
+        }
       }
       ZOLO_HIP(hipGetLastError());
       const int64_t nc = PE - pe;  // rest of the outer block
@@ -366,8 +595,10 @@ int trsv(const Ctx& c, const double* M, int64_t ld, int64_t n, int trans, int lo
     const int64_t b0 = b * kTb;
     const int bs = (int)std::min<int64_t>(kTb, n - b0);
     const int64_t r0 = lower ? b0 + bs : 0, r1 = lower ? n : b0;
-    hipLaunchKernelGGL(trsv_step_kernel, dim3(grid_for(std::max<int64_t>(r1 - r0, 1))), dim3(kT), 0, c.s, M, ld,
-                       trans, Dinv + b * kTb * kTb, trans, x, y, b0, bs, r0, r1);
+    const int64_t per = trans ? kTrsvRowsT : kTrsvRowsN;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((r1 - r0 + per - 1) / per, 8192));
+    hipLaunchKernelGGL(trsv_step_kernel, dim3(g), dim3(kT), 0, c.s, M, ld, trans, Dinv + b * kTb * kTb, trans, x, y,
+                       b0, bs, r0, r1);
   }
   ZOLO_HIP(hipGetLastError());
   return kOk;
@@ -377,8 +608,9 @@ int trsv(const Ctx& c, const double* M, int64_t ld, int64_t n, int trans, int lo
 
 size_t bounds_bytes(int64_t n) {
   const int64_t nb = (n + kTb - 1) / kTb;
-  return sizeof(double) * (size_t)(n * n + 3 * n + kLuNb * kLuNb + kLuNb * n + 8 + 2 * nb * kTb * kTb) +
-         sizeof(int64_t) * n + sizeof(Piv) * (size_t)(n / kT + 2) + 16 * 256;
+  return sizeof(double) * (size_t)(n * n + 3 * n + kLuNb * kLuNb + kLuNb * n + 8 + 2 * nb * kTb * kTb +
+                                  kLuSlotDoubles) +
+         sizeof(int64_t) * n + sizeof(Piv) * (size_t)(n / kT + 2) + 17 * 256;
 }
 
 int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta, Arena& ar,
@@ -398,6 +630,7 @@ int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* 
   int64_t* ipiv = ar.take<int64_t>(n);
   Piv* part = ar.take<Piv>((size_t)(n / kT + 2));
   int* dinfo = ar.take<int>(4);
+  double* slots = ar.take<double>((size_t)kLuSlotDoubles);
   ZOLO_ARENA_OK(ar);
   // alpha = ||A||_F ; X = A/alpha
   ZOLO_CHECK(k_sumsq<double>(A, lda, n, n, col, (int)std::min<int64_t>(n, 1024), out, s));
@@ -416,7 +649,7 @@ int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* 
     ZOLO_HIP(hipMemcpyAsync(&xnorm1, out, sizeof(double), hipMemcpyDeviceToHost, s));
   }
   ZOLO_HIP(hipMemsetAsync(dinfo, 0, sizeof(int), s));
-  ZOLO_CHECK(getrf(c, LU, n, n, ipiv, dinfo, part, L11i, W));
+  ZOLO_CHECK(getrf(c, LU, n, n, ipiv, dinfo, part, L11i, W, slots));
   int hinfo = 0;
   ZOLO_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
   ZOLO_HIP(hipStreamSynchronize(s));

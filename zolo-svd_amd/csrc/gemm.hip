@@ -50,10 +50,34 @@ double band_pairs(int64_t R, int64_t K, int lo, int hi) {
   return n;
 }
 
+// Number of (i, j, k) with i < M, j < N and max(0, i - alo, j - blo) <= k < K:
+// the products of op(A) zero for k < i - alo times op(B) zero for k < j - blo.
+// For each i the count over j is two arithmetic series.
+double two_band_triples(int64_t M, int64_t N, int64_t K, int64_t alo, int64_t blo) {
+  double t = 0.0;
+  for (int64_t i = 0; i < M; ++i) {
+    const int64_t c = std::max<int64_t>(0, i - alo);
+    if (c >= K) continue;
+    // j <= c + blo: k from c; j > c + blo: k from j - blo (until K)
+    const int64_t j1 = std::min<int64_t>(N, c + blo + 1);
+    t += (double)j1 * (double)(K - c);
+    const int64_t j2 = std::min<int64_t>(N, K + blo);  // j - blo < K
+    if (j2 > j1) {
+      const double cnt = (double)(j2 - j1);
+      const double first = (double)(K - (j1 - blo)), last = (double)(K - (j2 - 1 - blo));
+      t += cnt * (first + last) / 2.0;
+    }
+  }
+  return t;
+}
+
 double algorithmic_flops(const GemmDesc& d) {
   // 2 M N K scaled by the nonzero fraction of each banded operand (exact for
   // the triangular and trapezoidal operands of the path) and by the computed
-  // fraction of C for lower-triangle-only outputs.
+  // fraction of C for lower-triangle-only outputs; both operands with lower
+  // bands: the exact count of contributing products.
+  if (d.aLo >= 0 && d.bLo >= 0 && d.aHi < 0 && d.bHi < 0 && !d.lowerC && d.K > 0)
+    return 2.0 * two_band_triples(d.M, d.N, d.K, d.aLo, d.bLo) * d.batch;
   double f = 2.0 * (double)d.M * (double)d.N * (double)d.K * d.batch;
   if (d.K > 0 && (d.aLo >= 0 || d.aHi >= 0)) f *= band_pairs(d.M, d.K, d.aLo, d.aHi) / ((double)d.M * d.K);
   if (d.K > 0 && (d.bLo >= 0 || d.bHi >= 0)) f *= band_pairs(d.N, d.K, d.bLo, d.bHi) / ((double)d.N * d.K);

@@ -19,6 +19,7 @@
 // explicit Q use the compact WY form (I - V T V^T) through the MFMA GEMM.
 #include <algorithm>
 
+#include "grid_sync.hpp"
 #include "kernels.hpp"
 
 namespace zolo {
@@ -225,22 +226,6 @@ __device__ inline double wave_colsum64_swap(const double* v, double x, int lane)
   return a[0];
 }
 
-// Barrier of the nwg workgroups of one shift (all resident: the launch is
-// cooperative).  The counter only grows: arrival k of every workgroup waits
-// until it reads k * nwg.  The arrival is a device-scope release (it publishes
-// the partials written before the barrier); the spin reads the counter
-// relaxed and one device-scope acquire after it makes the other workgroups'
-// partials visible.
-__device__ inline void shift_barrier(unsigned* ctr, unsigned target) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
 // Whole panel (jb <= 64 columns) in one launch: one thread per panel row (rows
 // [p, r1_end) then [n, r2_end)) keeps the row's panel values in registers,
 // shifted so that the current column is always v[0] (no register is selected
@@ -261,7 +246,7 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
                                                                double* part, unsigned* bar, int nwg, T* tau,
                                                                int64_t strideTau, int q0) {
   __shared__ double red[kCoopT / 64][NBP];
-  __shared__ double gs[NBP], rr[NBP], wv[NBP];
+  __shared__ double gs[NBP], rr[NBP], wv[NBP], prj[NBP];
   __shared__ double sc[3];
   const int qi = blockIdx.y, q = q0 + qi;
   T* Sq = S + (int64_t)q * strideS;
@@ -284,20 +269,22 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
     const bool live = valid && i >= j && (i < r1_end || i <= n + j);
     if (live && i == j) {
 #pragma unroll
-      for (int c = 0; c < NBP; ++c)
-        if (c < rem) G[(int64_t)nwg * NBP + c] = v[c];
+      for (int c = 0; c < NBP; ++c) prj[c] = v[c];
     }
     const double x = (live && i > j) ? v[0] : 0.0;
     // lane 0: the squared norm below the diagonal; lanes c >= 1: the Gram row
     red[wave][lane] = wave_colsum64_swap(v, x, lane);
     __syncthreads();
+    // wave 0 publishes the slots (the barrier's release is issued by its thread 0);
+    // row j (the pivot row) lies in the top range, in workgroup (j - p) / kCoopT
     if (t < rem) {
       double sum = 0.0;
 #pragma unroll
       for (int k = 0; k < kCoopT / 64; ++k) sum += red[k][t];
       G[(int64_t)w * NBP + t] = sum;
+      if ((j - p) / kCoopT == w) G[(int64_t)nwg * NBP + t] = prj[t];
     }
-    if (bar) shift_barrier(bar + qi, (unsigned)(jj + 1) * (unsigned)nwg);
+    if (bar) grid_barrier(bar + qi, (unsigned)(jj + 1) * (unsigned)nwg);
     else __syncthreads();  // one workgroup per shift
     {  // sum the nwg partials of each column: 8 groups of 64 threads, all loads independent
       const int c = t & 63, g = t >> 6;
