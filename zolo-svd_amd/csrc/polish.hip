@@ -28,43 +28,57 @@ inline unsigned grid_for(int64_t total) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
 }
 
-// F = lower(G) mirrored, minus I, written as float (G holds U^T U in its lower triangle).
-__global__ __launch_bounds__(kT) void gram_to_f_kernel(const double* G, int64_t n, float* F) {
-  const int64_t total = n * n;
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
-    const int64_t i = e % n, j = e / n;
-    double v = (i >= j) ? G[i + j * n] : G[j + i * n];
-    if (i == j) v -= 1.0;
-    F[e] = (float)v;
+// Elementwise functions of (X[i][j], X[j][i]) over 64 x 64 tiles: the tile of
+// X^T is read in coalesced columns into LDS and read back transposed.
+//   MODE 0: out1 = F = lower(X) mirrored minus I (X holds U^T U in its lower triangle)
+//   MODE 1: out1 = Hs = (X + X^T)/2, out2 = R = X - X^T (= 2K)
+template <int MODE>
+__global__ __launch_bounds__(kT) void pair_tile_kernel(const double* X, int64_t n, float* out1, float* out2) {
+  __shared__ double tt[64][65];
+  const int64_t bi = (int64_t)blockIdx.x * 64, bj = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  if (MODE == 1 || bi < bj + 64) {  // MODE 0 reads the transpose only above the diagonal
+    for (int b = ty; b < 64; b += kT / 64)  // tt[b][a] = X[bj + a][bi + b]
+      tt[b][tx] = (bj + tx < n && bi + b < n) ? X[(bj + tx) + (bi + b) * n] : 0.0;
+  }
+  __syncthreads();
+  const int64_t i = bi + tx;
+  if (i >= n) return;
+  for (int r = ty; r < 64; r += kT / 64) {
+    const int64_t j = bj + r;
+    if (j >= n) break;
+    const int64_t e = i + j * n;
+    if (MODE == 0) {
+      double v = (i >= j) ? X[e] : tt[tx][r];
+      if (i == j) v -= 1.0;
+      out1[e] = (float)v;
+    } else {
+      const double a = X[e], b = tt[tx][r];
+      out1[e] = (float)(0.5 * (a + b));
+      out2[e] = (float)(a - b);
+    }
   }
 }
 
-// Hs = (C + C^T)/2 (float), R = (C - C^T) (= 2K, float).
-__global__ __launch_bounds__(kT) void split_kernel(const double* C, int64_t n, float* Hs, float* R) {
-  const int64_t total = n * n;
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
-    const int64_t i = e % n, j = e / n;
-    const double a = C[i + j * n], b = C[j + i * n];
-    Hs[e] = (float)(0.5 * (a + b));
-    R[e] = (float)(a - b);
-  }
-}
-
-// D += coef * (M - M^T)
-__global__ __launch_bounds__(kT) void skew_acc_kernel(float* D, const float* M, int64_t n, float coef) {
-  const int64_t total = n * n;
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
-    const int64_t i = e % n, j = e / n;
-    D[e] += coef * (M[i + j * n] - M[j + i * n]);
-  }
-}
-
-// D = M - M^T
-__global__ __launch_bounds__(kT) void skew_of_kernel(float* D, const float* M, int64_t n) {
-  const int64_t total = n * n;
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
-    const int64_t i = e % n, j = e / n;
-    D[e] = M[i + j * n] - M[j + i * n];
+// 64 x 64 tiles of D = M - M^T (ACC = false) or D += coef (M - M^T) (ACC = true):
+// the tile of M^T is read in coalesced columns into LDS and read back
+// transposed (row pitch 65: no bank conflicts).
+template <bool ACC>
+__global__ __launch_bounds__(kT) void skew_tile_kernel(float* D, const float* M, int64_t n, float coef) {
+  __shared__ float tt[64][65];
+  const int64_t bi = (int64_t)blockIdx.x * 64, bj = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  // tt[b][a] = M[bj + a][bi + b]
+  for (int b = ty; b < 64; b += kT / 64)
+    tt[b][tx] = (bj + tx < n && bi + b < n) ? M[(bj + tx) + (bi + b) * n] : 0.f;
+  __syncthreads();
+  if (bi + tx >= n) return;
+  for (int r = ty; r < 64; r += kT / 64) {
+    if (bj + r >= n) break;
+    const int64_t e = (bi + tx) + (bj + r) * n;
+    const float v = M[e] - tt[tx][r];
+    if (ACC) D[e] += coef * v;
+    else D[e] = v;
   }
 }
 
@@ -156,6 +170,7 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
   double* out = ar.take<double>(16);
   ZOLO_ARENA_OK(ar);
   const unsigned g = grid_for(nn);
+  const dim3 tiles((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
   auto reduce = [&](bool is_max, double* h) -> int {
     hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(kT), 0, s, part, kNp, is_max ? 1 : 0, out);
     ZOLO_HIP(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -189,11 +204,12 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
     d.transA = 1; d.M = n; d.N = n; d.K = n;
     d.A = U; d.lda = ldu; d.B = Xref; d.ldb = ldx; d.C = C; d.ldc = n;
     ZOLO_CHECK(gemm(Prec::F64, d, s));
-    hipLaunchKernelGGL(gram_to_f_kernel, dim3(g), dim3(kT), 0, s, G, n, F);
-    hipLaunchKernelGGL(split_kernel, dim3(g), dim3(kT), 0, s, C, n, Hs, R);
+    float* none = nullptr;
+    hipLaunchKernelGGL(pair_tile_kernel<0>, tiles, dim3(kT), 0, s, (const double*)G, n, F, none);
+    hipLaunchKernelGGL(pair_tile_kernel<1>, tiles, dim3(kT), 0, s, (const double*)C, n, Hs, R);
     // R = 2K - (F Hs - Hs F)/2 = R - 0.5 (M - M^T), M = F Hs
     ZOLO_CHECK(lgemm(F, Hs, M));
-    hipLaunchKernelGGL(skew_acc_kernel, dim3(g), dim3(kT), 0, s, R, M, n, -0.5f);
+    hipLaunchKernelGGL(skew_tile_kernel<true>, tiles, dim3(kT), 0, s, R, M, n, -0.5f);
     ZOLO_HIP(hipGetLastError());
     // CG on L(S) = S Hs + Hs S = (S Hs) - (S Hs)^T for skew S
     ZOLO_HIP(hipMemsetAsync(S, 0, sizeof(float) * nn, s));
@@ -210,7 +226,7 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
     const double tol2 = rho * rho * rr0;
     for (int it = 0; it < 40 && rr > tol2 && rr > 0.0; ++it) {
       ZOLO_CHECK(lgemm(P, Hs, M));
-      hipLaunchKernelGGL(skew_of_kernel, dim3(g), dim3(kT), 0, s, AP, M, n);
+      hipLaunchKernelGGL(skew_tile_kernel<false>, tiles, dim3(kT), 0, s, AP, M, n, 1.0f);
       double pap;
       ZOLO_CHECK(dot(P, AP, &pap));
       if (!(pap > 0.0)) break;
