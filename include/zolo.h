@@ -80,7 +80,9 @@ int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, doubl
 
 /* Replaces PZOLOPD2 (pzolopd2.f90:2): as zolo_pd_host with the iteration cap of
  * pzolopd2.f90:394 (12 instead of 6).  The restart branch of pzolopd2 calls
- * FormXTwo, which the reference does not define; it is not reproduced. */
+ * FormXTwo, which the reference does not define; in its place, when CON has
+ * reached 1 while ERR is still above TOL2, the bounds of the current iterate are
+ * re-estimated and the iteration continues from it (info.restarts counts them). */
 int zolo_pd2_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                   int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info);
 
