@@ -155,6 +155,7 @@ __global__ __launch_bounds__(kLuCoopT) void lu_panel_coop_kernel(double* A, int6
                                                                  double* slots, unsigned* bar, int nwg,
                                                                  int64_t* ipiv, int* info) {
   __shared__ double wbv[kLuCoopT / 64], prow[kLuNb], rj[kLuNb], cand[kLuNb], rowj[kLuNb];
+  __shared__ double crow[kLuMaxWg][kLuNb];  // every workgroup's candidate row
   __shared__ int64_t wbi[kLuCoopT / 64];
   __shared__ int64_t s_win, s_piv;
   __shared__ int s_slot;
@@ -212,6 +213,13 @@ __global__ __launch_bounds__(kLuCoopT) void lu_panel_coop_kernel(double* A, int6
     }
     if (bar) grid_barrier(bar, (unsigned)(jj + 1) * (unsigned)nwg);
     else __syncthreads();
+    // one round of loads: every candidate row and row j into LDS, while wave 0
+    // reduces the candidates' (value, row) pairs
+    for (int e = t; e < nwg * kLuNb; e += kLuCoopT) {
+      const int k = e / kLuNb, c = e % kLuNb;
+      crow[k][c] = (c < rem) ? G[(int64_t)k * kLuSlot + c] : 0.0;
+    }
+    if (t < kLuNb) rj[t] = (t < rem) ? G[(int64_t)nwg * kLuSlot + t] : 0.0;
     if (t < 64) {  // winner among the workgroups' candidates (nwg <= 64), fixed order
       double b = -1.0;
       int64_t b_i = INT64_MAX;
@@ -232,10 +240,7 @@ __global__ __launch_bounds__(kLuCoopT) void lu_panel_coop_kernel(double* A, int6
     }
     __syncthreads();
     const int64_t piv = s_piv;
-    if (t < kLuNb) {
-      prow[t] = (t < rem) ? G[(int64_t)s_slot * kLuSlot + t] : 0.0;
-      rj[t] = (t < rem) ? G[(int64_t)nwg * kLuSlot + t] : 0.0;
-    }
+    if (t < kLuNb) prow[t] = crow[s_slot][t];
     if (w == 0 && t == 0) {
       ipiv[j] = piv;
       if (s_bv == 0.0 && *info == 0) *info = (int)(j + 1);

@@ -294,10 +294,13 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
         const int k = g + m * (kCoopT / 64);
         pv[m] = (c < rem && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
       }
+      // the pivot row rides along with the first group's partial loads
+      const double pr = (g == 0 && c < rem) ? G[(int64_t)nwg * NBP + c] : 0.0;
       double sum = 0.0;
 #pragma unroll
       for (int m = 0; m < kCoopMaxSlots; ++m) sum += pv[m];
       red[g][c] = sum;
+      if (g == 0) rr[c] = pr;
     }
     __syncthreads();
     if (t < rem) {
@@ -305,7 +308,6 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
 #pragma unroll
       for (int g = 0; g < kCoopT / 64; ++g) sum += red[g][t];
       gs[t] = sum;
-      rr[t] = G[(int64_t)nwg * NBP + t];
     }
     __syncthreads();
     if (t == 0) {
@@ -445,6 +447,7 @@ __global__ __launch_bounds__(kT) void qr_form_t_kernel(const T* G, int64_t ldg, 
                                                        int64_t strideT) {
   __shared__ double g[NBP][NBP + 1];
   __shared__ double tt[NBP][NBP + 1];
+  __shared__ double ts[NBP];
   const int q = blockIdx.x;
   const T* Gq = G + q * strideG;
   const T* tq = tau + q * strideTau;
@@ -455,9 +458,10 @@ __global__ __launch_bounds__(kT) void qr_form_t_kernel(const T* G, int64_t ldg, 
     g[r][c] = (double)Gq[r + c * ldg];
     tt[r][c] = 0.0;
   }
+  if (t < jb) ts[t] = (double)tq[p + t];  // all of tau up front: no global load inside the column loop
   __syncthreads();
   for (int i = 0; i < jb; ++i) {
-    const double ti = (double)tq[p + i];
+    const double ti = ts[i];
     double s = 0.0;
     if (t < i) {
       for (int k = t; k < i; ++k) s += tt[t][k] * g[k][i];
