@@ -196,25 +196,53 @@ __device__ inline V wave_colsum64_masked(const V* v, V x, int lo, int hi, int la
 // Y), so X + Y afterwards holds column "X" in the lower and column "Y" in the
 // upper lanes, summed over each pair of lanes W apart.
 template <int W>
-__device__ inline double swap_add(double X, double Y) {
-  const unsigned xl = __double2loint(X), xh = __double2hiint(X);
-  const unsigned yl = __double2loint(Y), yh = __double2hiint(Y);
+__device__ inline void lane_swap(unsigned& x, unsigned& y) {
   if constexpr (W == 32) {
-    const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
-    const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
-    return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+    const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    x = r[0];
+    y = r[1];
   } else {
-    const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
-    const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
-    return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+    const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+    x = r[0];
+    y = r[1];
   }
+}
+template <int W>
+__device__ inline double swap_add(double X, double Y) {
+  unsigned xl = __double2loint(X), xh = __double2hiint(X);
+  unsigned yl = __double2loint(Y), yh = __double2hiint(Y);
+  lane_swap<W>(xl, yl);
+  lane_swap<W>(xh, yh);
+  return __hiloint2double(xh, xl) + __hiloint2double(yh, yl);
+}
+template <int W>
+__device__ inline float swap_add(float X, float Y) {
+  unsigned x = __float_as_uint(X), y = __float_as_uint(Y);
+  lane_swap<W>(x, y);
+  return __uint_as_float(x) + __uint_as_float(y);
+}
+
+// Column sums over the 64 lanes of the wave for 32 columns, a[c] (c < 32)
+// holding each lane's term of column c (a is overwritten): the first stage
+// with the 16-lane swap, then four shuffle stages; lane l returns the sum of
+// column l & 31 (the last stage adds the two lane halves with the 32-lane swap).
+template <typename V>
+__device__ inline V wave_colsum32_swap(V* a, int lane) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j) a[j] = swap_add<16>(a[j], a[j + 16]);
+  butterfly_step<8>(a, lane);
+  butterfly_step<4>(a, lane);
+  butterfly_step<2>(a, lane);
+  butterfly_step<1>(a, lane);
+  return swap_add<32>(a[0], a[0]);
 }
 
 // Column sums of x * v[c] over the 64 lanes for all 64 columns: lane l
 // returns the sum of column l.  The first two stages use the lane swaps, the
 // last four shuffles; the products are formed in the first stage.
-__device__ inline double wave_colsum64_swap(const double* v, double x, int lane) {
-  double a[32];
+template <typename V>
+__device__ inline V wave_colsum64_swap(const V* v, V x, int lane) {
+  V a[32];
 #pragma unroll
   for (int j = 0; j < 32; ++j) a[j] = swap_add<32>(x * v[j], x * v[j + 32]);
 #pragma unroll
@@ -378,10 +406,15 @@ __global__ __launch_bounds__(kPanelWgRows) void qr_panel_wg_kernel(T* S, int64_t
     const T x = (live && i > j) ? v[0] : (T)0;
     // lanes c >= 1 of the sum: the Gram row (columns jj + c); lane 0: the
     // squared norm below the diagonal
-    T g[NC];
+    T gs;
+    if constexpr (NC == 64) {
+      gs = wave_colsum64_swap<T>(v, x, lane);
+    } else {
+      T g[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) g[c] = x * v[c];
-    const T gs = (NC == 64) ? wave_colsum64(g, lane) : wave_colsum32(g, lane);
+      for (int c = 0; c < NC; ++c) g[c] = x * v[c];
+      gs = wave_colsum32_swap<T>(g, lane);
+    }
     if (lane < NC) red[wave][lane] = (double)gs;
     __syncthreads();
     if (wave == 0) {
