@@ -103,6 +103,10 @@ int own_stream(hipStream_t* s) {
   return kOk;
 }
 
+// Fallback and continuation calls with fewer matrices than this launch their
+// kernels directly instead of through a graph.
+constexpr int kDirectBelow = 64;
+
 // Per-process cache of the GEMM tile chosen for each (bucket size, precision).
 std::map<std::pair<int64_t, int>, int> g_tile_cache;
 
@@ -712,6 +716,9 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
       BatchOptions o2 = opt;
       o2.l0_source = 0;
       o2.beta_in = nullptr;
+      // a few matrices: launching directly costs less than capturing and
+      // instantiating a graph for them
+      if ((int)redo.size() < kDirectBelow) o2.use_graph = false;
       BatchResult r2;
       ZOLO_CHECK(pd_batched_run((int)redo.size(), rn.data(), rA.data(), rla.data(), rU.data(), rlu.data(), o2, r2,
                                 s));
@@ -751,6 +758,7 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
     oc.l0_source = 0;
     oc.beta_in = nullptr;
     oc.continue_rounds = 0;
+    if ((int)redo.size() < kDirectBelow) oc.use_graph = false;
     BatchResult r3;
     ZOLO_CHECK(pd_batched_run((int)redo.size(), rn.data(), rA.data(), rld.data(), rU.data(), rld.data(), oc, r3, s));
     for (size_t k = 0; k < redo.size(); ++k) {
