@@ -364,7 +364,7 @@ def main():
                 "s geometric 1..1e-8 with 5% at 1e-8)",
         "config": {"workload": f"Zolo-PD {args.precision} n={n}, kappa=1e8 with 5% cluster at 1e-8",
                    "n": n, "r": info.r, "precision": args.precision, "schedule": info.schedule,
-                   "polish_passes": info.polish_passes, "parallelism": "single GPU"},
+                   "polish_passes": info.polish_passes, "cg_iters": info.cg_iters, "parallelism": "single GPU"},
         "errors": err,
         "phase_ms": {k: round(v, 1) for k, v in info.phase_ms.items() if v > 0},
         "compare": compare,

@@ -51,6 +51,7 @@ typedef struct zolo_info {
   int shifts[16];         /* shifts used by each iteration: r, except a low-precision
                              Cholesky step, which takes the fewest shifts whose bound
                              reaches FP32 convergence in that step                     */
+  int cg_iters[4];        /* conjugate-gradient iterations of each polish pass          */
 } zolo_info_t;
 
 /* ---- library lifetime ---------------------------------------------------- */

@@ -25,6 +25,8 @@ size_t normest_bytes(int64_t m, int64_t n);
 // FP64 polish of a low-precision polar factor against an FP64 iterate Xref
 // that has the same polar factor (first-order Lyapunov correction passes).
 size_t polish_bytes(int64_t n);
-int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx, int64_t n, char* ws, int* passes);
+// cg (4 entries, may be null): CG iterations of each pass.
+int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx, int64_t n, char* ws, int* passes,
+           int* cg = nullptr);
 
 }  // namespace zolo

@@ -152,7 +152,8 @@ size_t polish_bytes(int64_t n) {
   return 2 * nn * sizeof(double) + 8 * nn * sizeof(float) + (kNp + 16) * sizeof(double) + 32 * 256;
 }
 
-int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx, int64_t n, char* ws, int* passes) {
+int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx, int64_t n, char* ws, int* passes,
+           int* cg) {
   const int64_t nn = n * n;
   const hipStream_t s = c.s;
   Arena ar(ws, polish_bytes(n));
@@ -224,7 +225,8 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
     const double delta = std::sqrt(rr0 / std::max(hh, 1e-300));
     const double rho = std::max(rho_floor, std::min(1e-2, 1e-15 / std::max(delta, 1e-300)));
     const double tol2 = rho * rho * rr0;
-    for (int it = 0; it < 40 && rr > tol2 && rr > 0.0; ++it) {
+    int it = 0;
+    for (; it < 40 && rr > tol2 && rr > 0.0; ++it) {
       ZOLO_CHECK(lgemm(P, Hs, M));
       hipLaunchKernelGGL(skew_tile_kernel<false>, tiles, dim3(kT), 0, s, AP, M, n, 1.0f);
       double pap;
@@ -238,6 +240,7 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
       hipLaunchKernelGGL(xpby_f_kernel, dim3(g), dim3(kT), 0, s, P, R, nn, (float)(rn / rr));
       rr = rn;
     }
+    if (cg && pass < 4) cg[pass] = it;
     // E = S - F/2 (into AP) ; U += U*E
     hipLaunchKernelGGL(form_e_kernel, dim3(g), dim3(kT), 0, s, AP, S, F, nn);
     ZOLO_CHECK(k_scale_copy<double, float>(U, ldu, U32, n, n, n, 1.0, s));

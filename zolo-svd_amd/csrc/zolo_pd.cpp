@@ -278,7 +278,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
     ZOLO_CHECK(k_scale_copy<float, double>(X32, n, X, n, n, n, 1.0, s));
     if (!have_ref) ZOLO_CHECK(k_scale_copy<double, double>(A, lda, Xref, n, n, n, 1.0, s));
     int passes = 0;
-    ZOLO_CHECK(polish(c, X, n, Xref, n, n, polish_ws, &passes));
+    ZOLO_CHECK(polish(c, X, n, Xref, n, n, polish_ws, &passes, info->cg_iters));
     info->polish_passes = passes;
   }
   tm.mark(kPhOther);

@@ -21,6 +21,7 @@ module zolo_capi
     real(c_double) :: alpha, beta, time_ms
     real(c_double) :: phase_ms(8)
     integer(c_int) :: shifts(16)
+    integer(c_int) :: cg_iters(4)
   end type zolo_info_t
 
   !> arithmetic of the Cholesky iterations: 0 FP64 (default), 1 FP32, 2 BF16x3

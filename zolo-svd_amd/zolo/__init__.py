@@ -59,7 +59,7 @@ class _Info(C.Structure):
         ("schedule", C.c_char * 16),
         ("err", C.c_double * 16), ("con", C.c_double * 16), ("maxc", C.c_double * 16),
         ("alpha", C.c_double), ("beta", C.c_double), ("time_ms", C.c_double),
-        ("phase_ms", C.c_double * 8), ("shifts", C.c_int * 16),
+        ("phase_ms", C.c_double * 8), ("shifts", C.c_int * 16), ("cg_iters", C.c_int * 4),
     ]
 
 
@@ -83,6 +83,7 @@ class RunInfo:
     info: int = 0
     restarts: int = 0
     shifts: list = field(default_factory=list)
+    cg_iters: list = field(default_factory=list)
 
     @staticmethod
     def from_c(ci: _Info) -> "RunInfo":
@@ -91,7 +92,7 @@ class RunInfo:
                        con=list(ci.con[:k]), maxc=list(ci.maxc[:k]), alpha=ci.alpha, beta=ci.beta,
                        time_ms=ci.time_ms, phase_ms={p: ci.phase_ms[i] for i, p in enumerate(PHASES)},
                        polish_passes=ci.polish_passes, info=ci.info, restarts=ci.restarts,
-                       shifts=list(ci.shifts[:k]))
+                       shifts=list(ci.shifts[:k]), cg_iters=list(ci.cg_iters[:ci.polish_passes]))
 
 
 _P = C.c_void_p
