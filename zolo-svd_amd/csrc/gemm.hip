@@ -535,29 +535,31 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
   // A slab: 2 layout tiles (1024 chunks of 16 B per half), B slab: 1 tile (512
   // chunks); thread t takes A chunks t and t + 512 and B chunk t.  Chunk e of a
   // tile is row e / 4, k (e % 4) * 8.
-  uint4 ra[2][2], rb[2];
+  // The register stage as named vectors (an array of them was kept on the
+  // scratch stack by the compiler).
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  u32x4 ah0, ah1, al0, al1, bh0, bl0;
   const int64_t tiles_k = Kp / kB3K;
+  const int64_t oa0 = (i0 / kB3Rows) * tiles_k * (kB3Rows * kB3K) + (int64_t)t * 8;
+  const int64_t oa1 = oa0 + tiles_k * (kB3Rows * kB3K);
+  const int64_t ob0 = (j0 / kB3Rows) * tiles_k * (kB3Rows * kB3K) + (int64_t)t * 8;
   auto load_tiles = [&](int64_t k0) {
-    const int64_t kt = k0 / kB3K;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int64_t o = ((i0 / kB3Rows + c) * tiles_k + kt) * (kB3Rows * kB3K) + (int64_t)t * 8;
-      ra[0][c] = *reinterpret_cast<const uint4*>(Ah + o);
-      ra[1][c] = *reinterpret_cast<const uint4*>(Al + o);
-    }
-    const int64_t o = ((j0 / kB3Rows) * tiles_k + kt) * (kB3Rows * kB3K) + (int64_t)t * 8;
-    rb[0] = *reinterpret_cast<const uint4*>(Bh + o);
-    rb[1] = *reinterpret_cast<const uint4*>(Bl + o);
+    const int64_t ko = (k0 / kB3K) * (kB3Rows * kB3K);
+    ah0 = *reinterpret_cast<const u32x4*>(Ah + oa0 + ko);
+    al0 = *reinterpret_cast<const u32x4*>(Al + oa0 + ko);
+    ah1 = *reinterpret_cast<const u32x4*>(Ah + oa1 + ko);
+    al1 = *reinterpret_cast<const u32x4*>(Al + oa1 + ko);
+    bh0 = *reinterpret_cast<const u32x4*>(Bh + ob0 + ko);
+    bl0 = *reinterpret_cast<const u32x4*>(Bl + ob0 + ko);
   };
   const int lrow = t >> 2, lk = (t & 3) * 8;
   auto store_tiles = [&](int b) {
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      *reinterpret_cast<uint4*>(&LA[b][0][c * kB3Rows + lrow][lk]) = ra[0][c];
-      *reinterpret_cast<uint4*>(&LA[b][1][c * kB3Rows + lrow][lk]) = ra[1][c];
-    }
-    *reinterpret_cast<uint4*>(&LB[b][0][lrow][lk]) = rb[0];
-    *reinterpret_cast<uint4*>(&LB[b][1][lrow][lk]) = rb[1];
+    *reinterpret_cast<u32x4*>(&LA[b][0][lrow][lk]) = ah0;
+    *reinterpret_cast<u32x4*>(&LA[b][1][lrow][lk]) = al0;
+    *reinterpret_cast<u32x4*>(&LA[b][0][kB3Rows + lrow][lk]) = ah1;
+    *reinterpret_cast<u32x4*>(&LA[b][1][kB3Rows + lrow][lk]) = al1;
+    *reinterpret_cast<u32x4*>(&LB[b][0][lrow][lk]) = bh0;
+    *reinterpret_cast<u32x4*>(&LB[b][1][lrow][lk]) = bl0;
   };
   auto compute = [&](int b) {
     const int kc = (lane >> 4) * 8;
