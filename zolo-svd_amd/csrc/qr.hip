@@ -42,16 +42,6 @@ __device__ inline void butterfly_step(V* a, int lane) {
     a[j] = keep + __shfl_xor(send, H, 64);
   }
 }
-// Same for 32 columns: lane l returns the sum of column l & 31.
-template <typename V>
-__device__ inline V wave_colsum32(V* a, int lane) {
-  butterfly_step<16>(a, lane);
-  butterfly_step<8>(a, lane);
-  butterfly_step<4>(a, lane);
-  butterfly_step<2>(a, lane);
-  butterfly_step<1>(a, lane);
-  return a[0] + __shfl_xor(a[0], 32, 64);
-}
 template <typename V>
 __device__ inline V wave_colsum64(V* a, int lane) {
   butterfly_step<32>(a, lane);
@@ -167,29 +157,6 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
   }
 }
 
-// Column sums of x * v[c] over the 64 lanes for the columns c in [lo, hi)
-// (zero elsewhere): the first butterfly step forms the products on the fly, so
-// only 32 temporaries are live next to v.  Lane l returns the sum of column l.
-template <typename V>
-__device__ inline V wave_colsum64_masked(const V* v, V x, int lo, int hi, int lane) {
-  V a[32];
-  const bool up = (lane & 32) != 0;
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const V l0 = (j >= lo && j < hi) ? x * v[j] : V(0);
-    const V h0 = (j + 32 >= lo && j + 32 < hi) ? x * v[j + 32] : V(0);
-    const V keep = up ? h0 : l0;
-    const V send = up ? l0 : h0;
-    a[j] = keep + __shfl_xor(send, 32, 64);
-  }
-  butterfly_step<16>(a, lane);
-  butterfly_step<8>(a, lane);
-  butterfly_step<4>(a, lane);
-  butterfly_step<2>(a, lane);
-  butterfly_step<1>(a, lane);
-  return a[0];
-}
-
 // One butterfly stage over lane halves (W = 32) or 16-lane rows (W = 16) with
 // the lane-swap instructions: permlane32_swap exchanges lanes 32..63 of X with
 // lanes 0..31 of Y (permlane16_swap the odd rows of X with the even rows of
@@ -242,11 +209,14 @@ __device__ inline V wave_colsum32_swap(V* a, int lane) {
 // last four shuffles; the products are formed in the first stage.
 template <typename V>
 __device__ inline V wave_colsum64_swap(const V* v, V x, int lane) {
-  V a[32];
+  // the first two stages fused per j, so only 16 partial columns are live
+  V a[16];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) a[j] = swap_add<32>(x * v[j], x * v[j + 32]);
-#pragma unroll
-  for (int j = 0; j < 16; ++j) a[j] = swap_add<16>(a[j], a[j + 16]);
+  for (int j = 0; j < 16; ++j) {
+    const V lo = swap_add<32>(x * v[j], x * v[j + 32]);
+    const V hi = swap_add<32>(x * v[j + 16], x * v[j + 48]);
+    a[j] = swap_add<16>(lo, hi);
+  }
   butterfly_step<8>(a, lane);
   butterfly_step<4>(a, lane);
   butterfly_step<2>(a, lane);
@@ -493,14 +463,19 @@ __global__ __launch_bounds__(kT) void qr_form_t_kernel(const T* G, int64_t ldg, 
   }
   if (t < jb) ts[t] = (double)tq[p + t];  // all of tau up front: no global load inside the column loop
   __syncthreads();
+  // column i: T[0:i, i] = -tau_i T[0:i, 0:i] G[0:i, i]; each row's dot is
+  // split over four lanes (k = row + part, row + part + 4, ...) and combined by
+  // two shuffles inside the group of four
+  const int row = t >> 2, part = t & 3;
   for (int i = 0; i < jb; ++i) {
     const double ti = ts[i];
     double s = 0.0;
-    if (t < i) {
-      for (int k = t; k < i; ++k) s += tt[t][k] * g[k][i];
-    }
+    if (row < i)
+      for (int k = row + part; k < i; k += 4) s += tt[row][k] * g[k][i];
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
     __syncthreads();
-    if (t < i) tt[t][i] = -ti * s;
+    if (part == 0 && row < i) tt[row][i] = -ti * s;
     if (t == 0) tt[i][i] = ti;
     __syncthreads();
   }
