@@ -6,9 +6,13 @@
 //    triangular solves on the GPU, the sign/argmax logic runs on the host
 //  * the 2-norm power iteration of PDNORMEST (pdnormest.f90:43-80).
 //
-// LU panel: one launch swaps the pivot row (full width) and one launch
-// applies the column's rank-1 update to the panel while reducing the argmax of
-// the next column, with per-workgroup partials summed in a fixed order.
+// LU panel: each 64-column panel is factored in one launch with the rows in
+// registers (lu_panel_coop_kernel; one barrier per column among its
+// workgroups), then one launch applies the panel's row interchanges to the
+// other columns.  Matrices too tall for one resident grid take the per-column
+// kernels (a launch that swaps the pivot row and one that applies the rank-1
+// update while reducing the next column's argmax).  All reductions use
+// per-workgroup partials combined in a fixed order.
 #include <algorithm>
 #include <cmath>
 #include <vector>

@@ -10,13 +10,18 @@
 // bottom rows.  [R; s I] with R upper triangular: r1_end = p + jb (the top
 // rows below the panel are zero and stay zero), r2_end = n + p + jb.
 //
-// Panel factorisation: one launch per column.  Launch j applies the reflector
-// of column j-1 to the rest of the panel and, in the same pass over the rows
-// each thread owns, reduces the Gram row g_c = sum_i S[i][j] S[i][c] that the
-// reflector of column j needs (its norm is g_j).  Partial sums go to a
-// per-workgroup slot and are summed in a fixed order by the next launch, so
-// the result does not depend on scheduling.  The trailing update and the
-// explicit Q use the compact WY form (I - V T V^T) through the MFMA GEMM.
+// Panel factorisation, three forms: the whole panel in one workgroup for
+// stacks of at most kQrPanelWgRows rows (qr_panel_wg_kernel); the whole panel
+// in one launch with one barrier per column among the workgroups of a shift
+// (qr_panel_coop_kernel); and, where neither applies (graph capture of a
+// multi-workgroup panel, grids too large to be resident), one launch per
+// column (qr_panel_kernel): launch j applies the reflector of column j-1 to
+// the rest of the panel and, in the same pass over the rows each thread owns,
+// reduces the Gram row g_c = sum_i S[i][j] S[i][c] that the reflector of
+// column j needs (its norm is g_j).  Partial sums go to per-workgroup slots
+// and are summed in a fixed order, so the result does not depend on
+// scheduling.  The trailing update and the explicit Q use the compact WY form
+// (I - V T V^T) through the MFMA GEMM.
 #include <algorithm>
 
 #include "grid_sync.hpp"
