@@ -16,6 +16,7 @@
 #include <cfloat>
 
 #include "eig.hpp"
+#include "wave_sums.hpp"
 
 namespace zolo {
 namespace {
@@ -94,24 +95,28 @@ __global__ __launch_bounds__(kT) void trd_house_kernel(double* A, int64_t ld, in
   }
   __syncthreads();
   const double scal = s_scal;
+  // dots of v with the panel columns of W (slot c < j) and A (slot kTrdNb + c):
+  // each thread forms its rows' 64 products (all loads issued together), one
+  // lane-swap butterfly gives every slot's wave sum
+  static_assert(2 * kTrdNb == 64, "one 64-slot column sum");
+  const int lane = t & 63, wave = t >> 6;
+  double p[2 * kTrdNb];
+#pragma unroll
+  for (int c = 0; c < 2 * kTrdNb; ++c) p[c] = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * kT + t + i + 1; r < n; r += (int64_t)gridDim.x * kT) {
     double x = 1.0;
     if (r > i + 1) x = A[r + i * ld] * scal;
     A[r + i * ld] = x;
     v[r] = x;
+#pragma unroll
+    for (int c = 0; c < kTrdNb; ++c)
+      if (c < j) {
+        p[c] += W[r + (int64_t)c * n] * x;
+        p[kTrdNb + c] += A[r + (k0 + c) * ld] * x;
+      }
   }
   for (int64_t r = (int64_t)blockIdx.x * kT + t; r <= i; r += (int64_t)gridDim.x * kT) v[r] = 0.0;
-  // dots of v with the panel columns of W (c < j) and A (kTrdNb + c), one column at a
-  // time over the rows this thread wrote (no register arrays)
-  const int lane = t & 63, wave = t >> 6;
-  for (int c = 0; c < 2 * j; ++c) {
-    const int cc = c < j ? c : c - j;
-    const double* col = c < j ? W + (int64_t)cc * n : A + (k0 + cc) * ld;
-    double acc = 0.0;
-    for (int64_t r = (int64_t)blockIdx.x * kT + t + i + 1; r < n; r += (int64_t)gridDim.x * kT) acc += col[r] * v[r];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    if (lane == 0) dots[c < j ? cc : kTrdNb + cc][wave] = acc;
-  }
+  dots[lane][wave] = wave_colsum64_swap(p, 1.0, lane);
   __syncthreads();
   if (t < 2 * kTrdNb) {
     const int c = t < kTrdNb ? t : t - kTrdNb;
@@ -197,9 +202,24 @@ __global__ __launch_bounds__(kT) void trd_w_kernel(const double* A, int64_t ld, 
   const double tv = tau[i];
   double acc = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * kT + t + i + 1; r < n; r += (int64_t)gridDim.x * kT) {
-    double y = 0.0;
-    for (int k = 0; k < nchunk; ++k) y += partial[(int64_t)k * n + r];
-    for (int c = 0; c < j; ++c) y -= A[r + (k0 + c) * ld] * tw[c] + W[r + (int64_t)c * n] * ta[c];
+    // four independent accumulators, so the loads are issued ahead of the adds
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+    int k = 0;
+    for (; k + 4 <= nchunk; k += 4) {
+      y0 += partial[(int64_t)k * n + r];
+      y1 += partial[(int64_t)(k + 1) * n + r];
+      y2 += partial[(int64_t)(k + 2) * n + r];
+      y3 += partial[(int64_t)(k + 3) * n + r];
+    }
+    for (; k < nchunk; ++k) y0 += partial[(int64_t)k * n + r];
+    double z0 = 0.0, z1 = 0.0;
+    int c = 0;
+    for (; c + 2 <= j; c += 2) {
+      z0 += A[r + (k0 + c) * ld] * tw[c] + W[r + (int64_t)c * n] * ta[c];
+      z1 += A[r + (k0 + c + 1) * ld] * tw[c + 1] + W[r + (int64_t)(c + 1) * n] * ta[c + 1];
+    }
+    for (; c < j; ++c) z0 += A[r + (k0 + c) * ld] * tw[c] + W[r + (int64_t)c * n] * ta[c];
+    const double y = ((y0 + y1) + (y2 + y3)) - (z0 + z1);
     const double w = tv * y;
     W[r + (int64_t)j * n] = w;
     acc += w * v[r];

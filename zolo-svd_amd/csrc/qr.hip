@@ -26,37 +26,13 @@
 
 #include "grid_sync.hpp"
 #include "kernels.hpp"
+#include "wave_sums.hpp"
 
 namespace zolo {
 namespace {
 
 constexpr int kT = 256;
 constexpr int NBP = kQrNb;
-
-// Sum of a[c] over the 64 lanes of the wave for all 64 columns c at once:
-// a butterfly in which each step exchanges half of the remaining columns with
-// the partner lane (63 shuffles instead of 64 x 6).  Lane l returns the sum of
-// column l.  All indices are compile-time so a[] stays in registers.
-template <int H, typename V>
-__device__ inline void butterfly_step(V* a, int lane) {
-  const bool up = (lane & H) != 0;
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    const V keep = up ? a[j + H] : a[j];
-    const V send = up ? a[j] : a[j + H];
-    a[j] = keep + __shfl_xor(send, H, 64);
-  }
-}
-template <typename V>
-__device__ inline V wave_colsum64(V* a, int lane) {
-  butterfly_step<32>(a, lane);
-  butterfly_step<16>(a, lane);
-  butterfly_step<8>(a, lane);
-  butterfly_step<4>(a, lane);
-  butterfly_step<2>(a, lane);
-  butterfly_step<1>(a, lane);
-  return a[0];
-}
 
 // part layout per shift: for local column jj, a record of (nwg*jb + jb) doubles:
 // [0, nwg*jb): per-workgroup Gram partials, [nwg*jb, nwg*jb+jb): pivot row.
@@ -160,73 +136,6 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
     __syncthreads();
     if (t < jb) G[(int64_t)w * jb + t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
   }
-}
-
-// One butterfly stage over lane halves (W = 32) or 16-lane rows (W = 16) with
-// the lane-swap instructions: permlane32_swap exchanges lanes 32..63 of X with
-// lanes 0..31 of Y (permlane16_swap the odd rows of X with the even rows of
-// Y), so X + Y afterwards holds column "X" in the lower and column "Y" in the
-// upper lanes, summed over each pair of lanes W apart.
-template <int W>
-__device__ inline void lane_swap(unsigned& x, unsigned& y) {
-  if constexpr (W == 32) {
-    const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
-    x = r[0];
-    y = r[1];
-  } else {
-    const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
-    x = r[0];
-    y = r[1];
-  }
-}
-template <int W>
-__device__ inline double swap_add(double X, double Y) {
-  unsigned xl = __double2loint(X), xh = __double2hiint(X);
-  unsigned yl = __double2loint(Y), yh = __double2hiint(Y);
-  lane_swap<W>(xl, yl);
-  lane_swap<W>(xh, yh);
-  return __hiloint2double(xh, xl) + __hiloint2double(yh, yl);
-}
-template <int W>
-__device__ inline float swap_add(float X, float Y) {
-  unsigned x = __float_as_uint(X), y = __float_as_uint(Y);
-  lane_swap<W>(x, y);
-  return __uint_as_float(x) + __uint_as_float(y);
-}
-
-// Column sums over the 64 lanes of the wave for 32 columns, a[c] (c < 32)
-// holding each lane's term of column c (a is overwritten): the first stage
-// with the 16-lane swap, then four shuffle stages; lane l returns the sum of
-// column l & 31 (the last stage adds the two lane halves with the 32-lane swap).
-template <typename V>
-__device__ inline V wave_colsum32_swap(V* a, int lane) {
-#pragma unroll
-  for (int j = 0; j < 16; ++j) a[j] = swap_add<16>(a[j], a[j + 16]);
-  butterfly_step<8>(a, lane);
-  butterfly_step<4>(a, lane);
-  butterfly_step<2>(a, lane);
-  butterfly_step<1>(a, lane);
-  return swap_add<32>(a[0], a[0]);
-}
-
-// Column sums of x * v[c] over the 64 lanes for all 64 columns: lane l
-// returns the sum of column l.  The first two stages use the lane swaps, the
-// last four shuffles; the products are formed in the first stage.
-template <typename V>
-__device__ inline V wave_colsum64_swap(const V* v, V x, int lane) {
-  // the first two stages fused per j, so only 16 partial columns are live
-  V a[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const V lo = swap_add<32>(x * v[j], x * v[j + 32]);
-    const V hi = swap_add<32>(x * v[j + 16], x * v[j + 48]);
-    a[j] = swap_add<16>(lo, hi);
-  }
-  butterfly_step<8>(a, lane);
-  butterfly_step<4>(a, lane);
-  butterfly_step<2>(a, lane);
-  butterfly_step<1>(a, lane);
-  return a[0];
 }
 
 // Whole panel (jb <= 64 columns) in one launch: one thread per panel row (rows
