@@ -241,6 +241,22 @@ def svd_tail(torch, zl, n, seed):
            "orth_V": (torch.linalg.norm(V.T @ V - eye) / math.sqrt(n)).item(),
            "phase_ms": {k: round(v, 1) for k, v in ph.items()}}
     log(f"zolo-svd n={n}: {dt:.3f} s, {out}")
+    # the same SVD with the mixed-precision polar decomposition (BF16x3 + FP64 polish)
+    del dU, dV
+    zl.svd_device(dA, precision="bf16x3")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S, dU, dV, info, ph = zl.svd_device(dA, precision="bf16x3")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    U, V = dU.T, dV.T
+    out["mixed"] = {"s": dt, "schedule": info.schedule,
+                    "backward": (torch.linalg.norm(A - (U * S) @ V.T) / torch.linalg.norm(A)).item(),
+                    "sigma_max_abs_err": (torch.max(torch.abs(S - torch.sort(s).values))).item(),
+                    "orth_U": (torch.linalg.norm(U.T @ U - eye) / math.sqrt(n)).item(),
+                    "orth_V": (torch.linalg.norm(V.T @ V - eye) / math.sqrt(n)).item(),
+                    "phase_ms": {k: round(v, 1) for k, v in ph.items()}}
+    log(f"zolo-svd mixed n={n}: {dt:.3f} s, {out['mixed']}")
     return out
 
 
