@@ -396,22 +396,31 @@ __global__ __launch_bounds__(kT) void dc_rotate_kernel(const Merge* merges, doub
 }
 
 // f(tau) = 1 + rho sum z_i^2 / (delta_i - tau), delta_i = d_i - d_origin
-__device__ inline double secular(const double* dK, const double* zK, int k, double dorg, double tau, double rho) {
-  double f = 1.0;
-  for (int i = 0; i < k; ++i) f += rho * zK[i] * zK[i] / ((dK[i] - dorg) - tau);
-  return f;
+// The secular function 1 + rho sum_i z_i^2 / ((d_i - d_org) - tau), summed
+// by the 64 lanes of a wave (lane l takes terms l, l + 64, ...) in a fixed
+// order; lane 0's total is broadcast, so every lane takes the same branch.
+__device__ inline double secular_wave(const double* dK, const double* zK, int k, double dorg, double tau, double rho,
+                                      int lane) {
+  double s = 0.0;
+  for (int i = lane; i < k; i += 64) s += rho * zK[i] * zK[i] / ((dK[i] - dorg) - tau);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return 1.0 + __shfl(s, 0, 64);
 }
 
 // Roots of the secular equation by bisection on the offset from the nearer
-// pole (one thread per root).  root j in (d_j, d_{j+1}), the last in (d_k, d_k + rho).
+// pole, one wave per root (four roots per workgroup).  Root j in
+// (d_j, d_{j+1}), the last in (d_k, d_k + rho |z|^2).
+constexpr int kRootsPerBlock = kT / 64;
 __global__ __launch_bounds__(kT) void dc_roots_kernel(const Merge* merges, const int* cnt, const double* rho_in,
                                                       const double* dK_all, const double* zK_all, int* org,
                                                       double* tau_out) {
   const Merge mg = merges[blockIdx.y];
   const int off = mg.off;
   const int k = cnt[3 * blockIdx.y];
-  const int jr = blockIdx.x * kT + threadIdx.x;
-  if (jr >= k) return;
+  const int lane = threadIdx.x & 63;
+  const int jr = blockIdx.x * kRootsPerBlock + (threadIdx.x >> 6);
+  if (jr >= k) return;  // the whole wave (no workgroup barrier in this kernel)
   const double* dK = dK_all + off;
   const double* zK = zK_all + off;
   const double rho = rho_in[blockIdx.y];
@@ -419,7 +428,7 @@ __global__ __launch_bounds__(kT) void dc_roots_kernel(const Merge* merges, const
   double lo, hi;
   if (jr < k - 1) {
     const double gap = dK[jr + 1] - dK[jr];
-    const double fm = secular(dK, zK, k, dK[jr], 0.5 * gap, rho);
+    const double fm = secular_wave(dK, zK, k, dK[jr], 0.5 * gap, rho, lane);
     if (fm >= 0.0) {  // root in the left half: offset from d_j
       o = jr; lo = 0.0; hi = 0.5 * gap;
     } else {          // right half: offset from d_{j+1} (negative)
@@ -428,19 +437,24 @@ __global__ __launch_bounds__(kT) void dc_roots_kernel(const Merge* merges, const
   } else {
     o = jr; lo = 0.0;
     double zz = 0.0;
-    for (int i = 0; i < k; ++i) zz += zK[i] * zK[i];
-    hi = rho * zz;
-    while (secular(dK, zK, k, dK[o], hi, rho) < 0.0) hi *= 2.0;  // guard against rounding in rho |z|^2
+    for (int i = lane; i < k; i += 64) zz += zK[i] * zK[i];
+#pragma unroll
+    for (int sh = 32; sh > 0; sh >>= 1) zz += __shfl_xor(zz, sh, 64);
+    hi = rho * __shfl(zz, 0, 64);
+    // guard against rounding in rho |z|^2
+    while (secular_wave(dK, zK, k, dK[o], hi, rho, lane) < 0.0) hi *= 2.0;
   }
   const double dorg = dK[o];
   for (int it = 0; it < 200; ++it) {
     const double mid = 0.5 * (lo + hi);
     if (mid <= lo || mid >= hi) break;
-    if (secular(dK, zK, k, dorg, mid, rho) < 0.0) lo = mid;
+    if (secular_wave(dK, zK, k, dorg, mid, rho, lane) < 0.0) lo = mid;
     else hi = mid;
   }
-  org[off + jr] = o;
-  tau_out[off + jr] = 0.5 * (lo + hi);
+  if (lane == 0) {
+    org[off + jr] = o;
+    tau_out[off + jr] = 0.5 * (lo + hi);
+  }
 }
 
 // Gu-Eisenstat weights: zhat_i^2 = prod_j (lambda_j - d_i) / (rho prod_{j != i} (d_j - d_i)).
@@ -637,7 +651,8 @@ int ke_dc_level(const DcLevel& L, hipStream_t s) {
   hipLaunchKernelGGL(dc_prepare_kernel, dim3(M), dim3(tp), 0, s, merges, L.e, L.lam, L.Q, L.ldq, L.rho, L.dK, L.zK,
                      L.pK, L.dD, L.pD, L.rotcs, L.rotab, L.cnt, L.zbuf, L.dbuf, L.perm);
   hipLaunchKernelGGL(dc_rotate_kernel, dim3(gs, M), dim3(kT), 0, s, merges, L.Q, L.ldq, L.cnt, L.rotcs, L.rotab);
-  hipLaunchKernelGGL(dc_roots_kernel, dim3(gs, M), dim3(kT), 0, s, merges, L.cnt, L.rho, L.dK, L.zK, L.org, L.tau);
+  const unsigned gr = (unsigned)((L.smax + kRootsPerBlock - 1) / kRootsPerBlock);
+  hipLaunchKernelGGL(dc_roots_kernel, dim3(gr, M), dim3(kT), 0, s, merges, L.cnt, L.rho, L.dK, L.zK, L.org, L.tau);
   hipLaunchKernelGGL(dc_zhat_kernel, dim3(gs, M), dim3(kT), 0, s, merges, L.cnt, L.rho, L.dK, L.zK, L.org, L.tau,
                      L.zh);
   hipLaunchKernelGGL(dc_vnorm_kernel, dim3(gs, M), dim3(kT), 0, s, merges, L.cnt, L.dK, L.org, L.tau, L.zh,
