@@ -158,12 +158,11 @@ constexpr int64_t kLuSlotDoubles = 2 * ((int64_t)kLuMaxWg * kLuSlot + kLuNb) + 8
 __global__ __launch_bounds__(kLuCoopT) void lu_panel_coop_kernel(double* A, int64_t lda, int64_t n, int64_t p, int jb,
                                                                  double* slots, unsigned* bar, int nwg,
                                                                  int64_t* ipiv, int* info) {
-  __shared__ double wbv[kLuCoopT / 64], prow[kLuNb], rj[kLuNb], cand[kLuNb], rowj[kLuNb];
+  __shared__ double wbv[kLuCoopT / 64], rj[kLuNb], cand[kLuNb], rowj[kLuNb];
   __shared__ double crow[kLuMaxWg][kLuNb];  // every workgroup's candidate row
   __shared__ int64_t wbi[kLuCoopT / 64];
-  __shared__ int64_t s_win, s_piv;
+  __shared__ int64_t s_piv;
   __shared__ int s_slot;
-  __shared__ double s_bv, s_wbv;
   const int t = threadIdx.x, w = blockIdx.x, lane = t & 63, wave = t >> 6;
   const int64_t i = p + (int64_t)w * kLuCoopT + t;
   const bool valid = i < n;
@@ -191,28 +190,24 @@ __global__ __launch_bounds__(kLuCoopT) void lu_panel_coop_kernel(double* A, int6
       for (int c = 0; c < kLuNb; ++c) rowj[c] = v[c];
     }
     __syncthreads();
-    if (t == 0) {
-      double b = wbv[0];
-      int64_t b_i = wbi[0];
-      for (int k = 1; k < kLuCoopT / 64; ++k)
-        if (piv_better(wbv[k], wbi[k], b, b_i)) { b = wbv[k]; b_i = wbi[k]; }
-      s_wbv = b;
-      s_win = b_i;
-    }
-    __syncthreads();
-    if (valid && i == s_win) {
+    // every thread reduces the waves' candidates in the same order (no extra barrier)
+    double wb = wbv[0];
+    int64_t wi = wbi[0];
+    for (int k = 1; k < kLuCoopT / 64; ++k)
+      if (piv_better(wbv[k], wbi[k], wb, wi)) { wb = wbv[k]; wi = wbi[k]; }
+    if (valid && i == wi) {
 #pragma unroll
       for (int c = 0; c < kLuNb; ++c) cand[c] = v[c];
     }
     __syncthreads();
     // wave 0 publishes the slots (the barrier's release is issued by its thread 0)
     if (t < kLuNb) {
-      const bool has = s_win != INT64_MAX;  // this workgroup has rows >= j
+      const bool has = wi != INT64_MAX;  // this workgroup has rows >= j
       if (t < rem) G[(int64_t)w * kLuSlot + t] = has ? cand[t] : 0.0;
       if (t < rem && (j - p) / kLuCoopT == w) G[(int64_t)nwg * kLuSlot + t] = rowj[t];
       if (t == 0) {
-        G[(int64_t)w * kLuSlot + kLuNb] = s_wbv;
-        G[(int64_t)w * kLuSlot + kLuNb + 1] = (double)s_win;  // row indices are exact in a double
+        G[(int64_t)w * kLuSlot + kLuNb] = wb;
+        G[(int64_t)w * kLuSlot + kLuNb + 1] = (double)wi;  // row indices are exact in a double
       }
     }
     if (bar) grid_barrier(bar, (unsigned)(jj + 1) * (unsigned)nwg);
@@ -240,38 +235,42 @@ __global__ __launch_bounds__(kLuCoopT) void lu_panel_coop_kernel(double* A, int6
         const int os = __shfl_xor(slot, o, 64);
         if (piv_better(ob, oi, b, b_i)) { b = ob; b_i = oi; slot = os; }
       }
-      if (t == 0) { s_piv = b_i; s_slot = slot; s_bv = b; }
+      if (t == 0) {
+        s_piv = b_i;
+        s_slot = slot;
+        if (w == 0) {
+          ipiv[j] = b_i;
+          if (b == 0.0 && *info == 0) *info = (int)(j + 1);
+        }
+      }
     }
     __syncthreads();
     const int64_t piv = s_piv;
-    if (t < kLuNb) prow[t] = crow[s_slot][t];
-    if (w == 0 && t == 0) {
-      ipiv[j] = piv;
-      if (s_bv == 0.0 && *info == 0) *info = (int)(j + 1);
-    }
-    __syncthreads();
+    const int sl = s_slot;  // crow[sl] is the pivot row
     if (valid && piv != j) {
       if (i == j) {
 #pragma unroll
-        for (int c = 0; c < kLuNb; ++c) v[c] = prow[c];
+        for (int c = 0; c < kLuNb; ++c) v[c] = crow[sl][c];
       } else if (i == piv) {
 #pragma unroll
         for (int c = 0; c < kLuNb; ++c) v[c] = rj[c];
       }
     }
-    const double d = prow[0];
+    const double d = crow[sl][0];
     if (valid && i > j && d != 0.0) {
       const double l = v[0] / d;
       v[0] = l;
 #pragma unroll
-      for (int c = 1; c < kLuNb; ++c) v[c] -= l * prow[c];
+      for (int c = 1; c < kLuNb; ++c) v[c] -= l * crow[sl][c];
     }
     if (valid) *out = v[0];
     out += lda;
 #pragma unroll
     for (int c = 0; c + 1 < kLuNb; ++c) v[c] = v[c + 1];
     v[kLuNb - 1] = 0.0;
-    __syncthreads();  // LDS and the winner are rewritten for the next column
+    // No barrier here: the next column writes wbv/wbi, rowj and cand only
+    // after this column's last barrier (every reader has passed it), and crow,
+    // rj, s_piv, s_slot only after the next grid barrier.
   }
 }
 

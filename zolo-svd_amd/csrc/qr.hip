@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
                                                                double* part, unsigned* bar, int nwg, T* tau,
                                                                int64_t strideTau, int q0) {
   __shared__ double red[kCoopT / 64][NBP];
-  __shared__ double gs[NBP], rr[NBP], wv[NBP], prj[NBP];
+  __shared__ double wv[NBP], prj[NBP];
   __shared__ double sc[3];
   const int qi = blockIdx.y, q = q0 + qi;
   T* Sq = S + (int64_t)q * strideS;
@@ -198,6 +198,9 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
     }
     if (bar) grid_barrier(bar + qi, (unsigned)(jj + 1) * (unsigned)nwg);
     else __syncthreads();  // one workgroup per shift
+    // the pivot row rides along with the first group's partial loads (wave 0:
+    // lane c holds r_c)
+    double pr = 0.0;
     {  // sum the nwg partials of each column: 8 groups of 64 threads, all loads independent
       const int c = t & 63, g = t >> 6;
       double pv[kCoopMaxSlots];
@@ -206,36 +209,33 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
         const int k = g + m * (kCoopT / 64);
         pv[m] = (c < rem && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
       }
-      // the pivot row rides along with the first group's partial loads
-      const double pr = (g == 0 && c < rem) ? G[(int64_t)nwg * NBP + c] : 0.0;
+      if (g == 0 && c < rem) pr = G[(int64_t)nwg * NBP + c];
       double sum = 0.0;
 #pragma unroll
       for (int m = 0; m < kCoopMaxSlots; ++m) sum += pv[m];
       red[g][c] = sum;
-      if (g == 0) rr[c] = pr;
     }
     __syncthreads();
-    if (t < rem) {
-      double sum = 0.0;
+    if (wave == 0) {  // lane c: g_c, then the reflector (dlarfg) from lane 0's r_0 and g_0
+      double gsum = 0.0;
+      if (t < rem) {
 #pragma unroll
-      for (int g = 0; g < kCoopT / 64; ++g) sum += red[g][t];
-      gs[t] = sum;
-    }
-    __syncthreads();
-    if (t == 0) {
-      const double alpha = rr[0], xn2 = gs[0];
+        for (int g = 0; g < kCoopT / 64; ++g) gsum += red[g][t];
+      }
+      const double alpha = __shfl(pr, 0, 64), xn2 = __shfl(gsum, 0, 64);
       double beta = alpha, tv = 0.0, scal = 0.0;
       if (xn2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
         tv = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
       }
-      sc[0] = beta; sc[1] = tv; sc[2] = scal;
-      if (w == 0) tau[(int64_t)q * strideTau + j] = (T)tv;
+      // w_c = tau (r_c + scal g_c): the update of column c is v_i w_c; zero past the panel
+      wv[t] = (t >= 1 && t < rem) ? tv * (pr + scal * gsum) : 0.0;
+      if (t == 0) {
+        sc[0] = beta; sc[1] = tv; sc[2] = scal;
+        if (w == 0) tau[(int64_t)q * strideTau + j] = (T)tv;
+      }
     }
-    __syncthreads();
-    // w_c = tau (r_c + scal g_c): the update of column c is v_i w_c; zero past the panel
-    if (t < NBP) wv[t] = (t >= 1 && t < rem) ? sc[1] * (rr[t] + sc[2] * gs[t]) : 0.0;
     __syncthreads();
     if (live) {
       const double vi = (i == j) ? 1.0 : v[0] * sc[2];
@@ -249,7 +249,9 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
 #pragma unroll
     for (int c = 0; c + 1 < NBP; ++c) v[c] = v[c + 1];
     v[NBP - 1] = 0.0;
-    __syncthreads();  // red, gs, rr, wv and sc are rewritten for the next column
+    // No barrier here: the next column writes prj and red only after this
+    // column's last barrier, which every reader of them has passed, and wv/sc
+    // only after the next grid barrier.
   }
 }
 
