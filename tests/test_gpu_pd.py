@@ -100,6 +100,17 @@ def test_bounds(torch_gpu, zl, n, conv):
     assert abs(b - bo) <= max(1e-8, 1e-15 * 1e7) * bo, (b, bo)
 
 
+@pytest.mark.parametrize("n,zc", [(300, 7), (700, 70), (700, 650)])
+def test_bounds_exactly_singular(torch_gpu, zl, n, zc):
+    """A zero column stays exactly zero through the elimination, so its pivot is
+    0 (dgetrf INFO = column + 1) and the bound estimate refuses the matrix.
+    n = 700: the LU panels of the first columns span two workgroups."""
+    A, *_ = zo.make_matrix(n, "geometric", 1e3, n + zc)
+    A[:, zc] = 0.0
+    with pytest.raises(zl.ZoloError, match=rf"exactly singular \(LU pivot {zc + 1}\)"):
+        zl.bounds_device(zl.to_device(A), "zolo")
+
+
 @pytest.mark.parametrize("m,n", [(100, 100), (300, 120), (77, 200)])
 def test_normest(torch_gpu, zl, m, n):
     rng = np.random.default_rng(m + n)
