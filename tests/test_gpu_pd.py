@@ -43,6 +43,41 @@ def test_pzolopd1_f64_matches_oracle(zl, n, kind, kappa, seed):
     assert np.allclose(H, H.T)
 
 
+# orders below and just past the 64-wide blocks, one to eight shifts
+@pytest.mark.parametrize("n,rk", [(1, 1), (2, 2), (3, 8), (17, 3), (65, 5), (129, 1)])
+def test_pzolopd1_small_and_ragged(zl, n, rk):
+    A, Ug, s, Vg = zo.make_matrix(n, "geometric", 1e4, 100 + n)
+    alpha, beta = zo.bounds_estimate(A)
+    Uo, lo = zo.pzolopd1(A, alpha, beta, rk)
+    U, H, info = zl.pzolopd1(A, alpha, beta, rk)
+    sched = "".join("Q" if x == "QR" else "C" for x in lo.schedule)
+    assert info.schedule == sched and info.iters == lo.iters
+    be, orth = _errs(A, U, H)
+    beo, ortho = _errs(A, Uo, zo.polar_h(Uo, A))
+    assert be <= 10 * max(beo, 1e-15) and orth <= 10 * max(ortho, 1e-15), (be, beo, orth, ortho)
+    d_cf_o = np.linalg.norm(Uo - Ug @ Vg.T) / math.sqrt(n)
+    assert np.linalg.norm(U - Ug @ Vg.T) / math.sqrt(n) <= 10 * max(d_cf_o, 1e-14)
+
+
+# SYM = 'S' (zolochol.f90:99) on a symmetric indefinite A = Q diag(l) Q^T,
+# whose polar factor is Q sign(l) Q^T
+@pytest.mark.parametrize("n,rk", [(17, 3), (65, 5), (200, 4)])
+def test_pzolopd1_symmetric(zl, n, rk):
+    rng = np.random.default_rng(n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.geomspace(1, 1e-4, n) * np.where(np.arange(n) % 3 == 0, -1.0, 1.0)
+    A = (Q * lam) @ Q.T
+    A = (A + A.T) / 2
+    alpha, beta = zo.bounds_estimate(A)
+    Uo, lo = zo.pzolopd1(A, alpha, beta, rk, sym="S")
+    U, H, info = zl.pzolopd1(A, alpha, beta, rk, sym="S")
+    sched = "".join("Q" if x == "QR" else "C" for x in lo.schedule)
+    assert info.schedule == sched and info.iters == lo.iters
+    Ucf = (Q * np.sign(lam)) @ Q.T
+    d_o = np.linalg.norm(Uo - Ucf) / math.sqrt(n)
+    assert np.linalg.norm(U - Ucf) / math.sqrt(n) <= 10 * max(d_o, 1e-14), d_o
+
+
 @pytest.mark.parametrize("prec", ["f32", "bf16x3"])
 @pytest.mark.parametrize("n,kind,kappa,seed", [(256, "cluster", 1e8, 5), (200, "loguniform", 1e12, 6),
                                                (650, "cluster", 1e8, 13)])
