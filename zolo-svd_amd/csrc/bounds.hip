@@ -306,15 +306,17 @@ __global__ __launch_bounds__(kT) void unit_lower_inv_kernel(const double* A, int
     x[i][k] = (i == k) ? 1.0 : 0.0;
   }
   __syncthreads();
-  if ((int)threadIdx.x < jb) {
-    const int c = threadIdx.x;
-    for (int i = c + 1; i < jb; ++i) {
-      double s = 0.0;
-      for (int k = c; k < i; ++k) s += a[i][k] * x[k][c];
-      x[i][c] = -s;
+  // forward substitution for all columns at once (axpy form): at step k, row k
+  // of X is final and updates the rows below; each element subtracts the same
+  // terms in the same order as a per-column substitution
+  for (int k = 0; k + 1 < jb; ++k) {
+    const int m = jb - k - 1;
+    for (int e = threadIdx.x; e < m * (k + 1); e += kT) {
+      const int i = k + 1 + e % m, c = e / m;
+      x[i][c] -= a[i][k] * x[k][c];
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int e = threadIdx.x; e < jb * jb; e += kT) {
     const int i = e % jb, k = e / jb;
     Li[i + k * ldl] = x[i][k];

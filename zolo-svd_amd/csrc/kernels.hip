@@ -235,17 +235,22 @@ __global__ __launch_bounds__(kT) void chol_leaf_kernel(T* A, int64_t lda, int64_
     }
     __syncthreads();
   }
-  // Inverse: one thread per column c, forward substitution L x = e_c.
-  if (t < nb) {
-    const int c = t;
-    li[c][c] = (T)1 / a[c][c];
-    for (int i = c + 1; i < nb; ++i) {
-      T s = 0;
-      for (int k = c; k < i; ++k) s += a[i][k] * li[k][c];
-      li[i][c] = -s / a[i][i];
-    }
-  }
+  // Inverse by forward substitution L x = e_c for all columns at once (axpy
+  // form): at step k, row k of the inverse is divided by l_kk (it is then
+  // final) and updates the rows below; each element takes the same terms in
+  // the same order as a per-column substitution.
+  if (t < nb) li[t][t] = (T)1;
   __syncthreads();
+  for (int k = 0; k < nb; ++k) {
+    if (t <= k) li[k][t] /= a[k][k];
+    __syncthreads();
+    const int m = nb - k - 1;
+    for (int e = t; e < m * (k + 1); e += kT) {
+      const int i = k + 1 + e % m, c = e / m;
+      li[i][c] -= a[i][k] * li[k][c];
+    }
+    __syncthreads();
+  }
   for (int e = t; e < nb * nb; e += kT) {
     const int i = e % nb, j = e / nb;
     Aq[i + j * lda] = a[i][j];
