@@ -190,6 +190,28 @@ int zolo_syevd_device(int64_t n, double* H, int64_t ldh, double* w, double* Z, i
                       void* stream);
 int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U, int64_t ldu,
                     double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream);
+/* ---- spectral division (QDWH-eig / Zolo-eig; device pointers, FP64) ----------
+ * zolo_subit_device replaces PQDWHSUBIT (pqdwhsubit.f90:2): on entry Q is the
+ * polar factor of A - sigma I (A symmetric n x n, normA = ||A||_F > 0); C =
+ * (Q - I)/2, Rk = round(||C||_F^2) = the number of eigenvalues below sigma;
+ * subspace iteration (oversampling 3, at most two iterations) with full QR
+ * gives V = [V1 V2].  On exit Q = V, A = V^T A V, *rk = Rk, *err (may be NULL)
+ * = ||(V^T A V)(1:Rk, Rk+1:n)||_F / normA, *iters (may be NULL) = iterations;
+ * the reference accepts err <= 4 * 10 eps / 2.
+ * zolo_eig_device replaces ZoloEig (ZoloEig.f90:2): Q the polar factor of
+ * As - sigma I, As the matrix (both overwritten); on exit w(1:Rk) the
+ * eigenvalues below sigma, ascending, and V (n x Rk) their eigenvectors.
+ * zolo_eig_below_device is the driver pqdwheig.f90:113-144 with the Zolo
+ * polar factor: the polar factor of A - sigma I (precision and rk as
+ * zolo_pd_device, sym = 'S'), then zolo_eig_device; A is not modified,
+ * *count = Rk, info (may be NULL) describes the polar decomposition. */
+int zolo_subit_device(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err,
+                      int* iters, void* stream);
+int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
+                    int* rk, double* err, int* iters, void* stream);
+int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
+                          double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
+                          void* stream);
 /* FormX2 (FormX2.f90:2) on host arrays: A <- (A + Z)/fone, Z <- the same, err = ||A_new - A_old||_F. */
 int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err);
 /* Host-array form of zolo_syevd_device (the Fortran ELPA_PDSYEVD2 drop-in calls it). */

@@ -33,6 +33,11 @@ How each function is pinned (checked by ``tests/test_oracle.py``):
 * ``lacn2_norm1_inv`` -- against LAPACK dgecon (scipy) and the exact
   ||X^{-1}||_1 on small matrices.
 * ``normest`` -- against numpy's exact 2-norm.
+* ``pqdwhsubit`` / ``zoloeig`` / ``eig_below`` -- against LAPACK's dsyevd
+  (numpy.linalg.eigvalsh): the count of eigenvalues below sigma, those
+  eigenvalues, the residual ||A V - V diag(w)||, V^T V = I, and the angle
+  between span(V) and the exact invariant subspace; a plant of the wrong
+  projector sign ((Q + I)/2) fails the count.
 """
 from __future__ import annotations
 
@@ -584,6 +589,94 @@ def zolo_svd(A: np.ndarray, rk: int = 0):
     H = polar_h(Up, A)
     w, V = np.linalg.eigh(H)
     return w, Up @ V, V
+
+
+# ---------------------------------------------------------------------------
+# Spectral division: pqdwhsubit.f90, ZoloEig.f90, pqdwheig.f90
+# ---------------------------------------------------------------------------
+
+@dataclass
+class SubitLog:
+    rk: int = 0
+    iters: int = 0
+    err: list = field(default_factory=list)   # ||E(1:Rk, Rk+1:N)||_F / normA per iteration
+
+
+def pqdwhsubit(A: np.ndarray, normA: float, Q: np.ndarray, rng: np.random.Generator,
+               projector_sign: float = -1.0):
+    """PQDWHSUBIT (pqdwhsubit.f90:223-316) on one process.
+
+    Q is the polar factor of A - sigma I.  C = (Q - I)/2 (pqdwhsubit.f90:228-239),
+    Rk = NINT(||C||_F^2) (241-243), oversampling pp = 3 (225, 244); random start
+    block (259), QR + ORGQR(N, kk, kk) (266-267), Y = C X1 (269), QR +
+    ORGQR(N, N, kk) (271-272), E = V^T A V (276-279), err on E(1:Rk, Rk+1:N)
+    (282-285), accept at err <= 4 TOL1 (292) or after the second iteration
+    (297), else restart from C V1 (305-311).  The start block is uniform
+    [0, 1) as Fortran's random_number, from ``rng``.  ``projector_sign`` = +1
+    plants the wrong sign ((Q + I)/2) for the tests.
+    Returns (V, E, log).
+    """
+    n = A.shape[0]
+    tol1 = 10.0 * DLAMCH_P / 2.0
+    C = 0.5 * (Q + projector_sign * np.eye(n))
+    rk = int(round(float(np.linalg.norm(C)) ** 2))
+    rk = min(max(rk, 0), n)
+    kk = min(rk + 3, n)
+    log = SubitLog(rk=rk)
+    X = rng.uniform(0.0, 1.0, (n, kk))
+    it = 1
+    while True:
+        X1, _ = np.linalg.qr(X, mode="reduced")
+        Y = C @ X1
+        V, _ = np.linalg.qr(Y, mode="complete")
+        E = V.T @ (A @ V)
+        err = float(np.linalg.norm(E[:rk, rk:])) / normA
+        log.err.append(err)
+        if err <= 4.0 * tol1 or it == 2:
+            break
+        X = C @ V[:, :rk]
+        kk = rk
+        it += 1
+    log.iters = it
+    return V, E, log
+
+
+def zoloeig(Q: np.ndarray, As: np.ndarray, rng: np.random.Generator):
+    """ZoloEig (ZoloEig.f90:80-103): PQDWHSUBIT, then dsyevd on the leading
+    Rk x Rk block of E (upper triangle, as PDSYEVD('V', 'U')), V1 VV1.
+    Returns (w ascending, V (n x Rk), log)."""
+    normA = float(np.linalg.norm(As))
+    V, E, log = pqdwhsubit(As, normA, Q, rng)
+    rk = log.rk
+    if rk == 0:
+        return np.zeros(0), np.zeros((As.shape[0], 0)), log
+    w, VV = np.linalg.eigh(E[:rk, :rk], UPLO="U")
+    return w, V[:, :rk] @ VV, log
+
+
+def eig_below(A: np.ndarray, sigma: float, rk: int = 0, method: str = "zolo", seed: int = 0):
+    """pqdwheig.f90:113-144: polar factor of A - sigma I, then ZoloEig.
+
+    method 'zolo': Zolo-PD with the bounds of test_zolodrv1.f90 (sym 'S');
+    'qdwh': pqdwhfacs with SIGMA as the reference driver calls it.
+    Returns (w, V, log) -- the eigenpairs of A below sigma."""
+    n = A.shape[0]
+    if method == "qdwh":
+        Q, _, _ = pqdwhfacs(A, sigma, polar=False, sym="S")
+    else:
+        B = np.array(A, dtype=np.float64, copy=True)
+        B[np.diag_indices(n)] -= sigma
+        alpha, beta = bounds_estimate(B)
+        r = rk if rk > 0 else choosem(1.0 / beta)
+        Q, _ = pzolopd1(B, alpha, beta, r, sym="S")
+    return zoloeig(Q, A, np.random.default_rng(seed))
+
+
+def symmetric_matrix(n: int, lam: np.ndarray, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """A = W diag(lam) W^T with Haar W (symmetrised); returns (A, W)."""
+    W = haar(n, np.random.default_rng(seed))
+    A = (W * lam) @ W.T
+    return 0.5 * (A + A.T), W
 
 
 # ---------------------------------------------------------------------------

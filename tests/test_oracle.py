@@ -371,3 +371,67 @@ def test_pzolopd2_restart_closed_form():
     W, s, Vt = np.linalg.svd(A)
     assert np.linalg.norm(U2 - W @ Vt) / np.sqrt(A.shape[0]) < 1e-12
     assert zo.orthogonality(U2)[0] < 1e-14
+
+
+# ---------------------------------------------------------------- spectral division
+def _sym_case(n, seed, sigma_frac=0.4):
+    """Symmetric A = W diag(lam) W^T with a gap around sigma: lam in [-1, 1]
+    with no eigenvalue within 0.05 of sigma."""
+    rng = np.random.default_rng(seed)
+    lam = np.sort(rng.uniform(-1.0, 1.0, n))
+    sigma = float(np.quantile(lam, sigma_frac))
+    lam = np.where(np.abs(lam - sigma) < 0.05, lam + np.sign(lam - sigma + 1e-300) * 0.05, lam)
+    A, W = zo.symmetric_matrix(n, lam, seed + 1)
+    return A, np.sort(lam), sigma
+
+
+@pytest.mark.parametrize("n,seed,frac,method", [(40, 1, 0.3, "zolo"), (90, 2, 0.6, "zolo"), (64, 3, 0.5, "qdwh")])
+def test_eig_below_vs_eigvalsh(n, seed, frac, method):
+    A, lam, sigma = _sym_case(n, seed, frac)
+    w, V, log = zo.eig_below(A, sigma, method=method, seed=seed)
+    exact = np.linalg.eigvalsh(A)
+    below = exact[exact < sigma]
+    assert log.rk == below.size and V.shape == (n, below.size)
+    assert np.allclose(w, below, atol=1e-13)
+    assert np.linalg.norm(A @ V - V * w) / np.linalg.norm(A) < 1e-13
+    assert np.linalg.norm(V.T @ V - np.eye(below.size)) < 1e-13
+    # span(V) = the exact invariant subspace: sin of the largest principal angle
+    Wx = np.linalg.eigh(A)[1][:, exact < sigma]
+    sines = np.linalg.svd(V - Wx @ (Wx.T @ V), compute_uv=False)
+    assert sines.max() < 1e-12
+    assert log.err[-1] <= 4.0 * 10.0 * EPS / 2.0 * 50
+
+
+def test_pqdwhsubit_outputs():
+    """On exit Q = V orthogonal and E = V^T A V has its off-diagonal block at
+    the accepted level; Rk = ||C||_F^2 exactly for an exact sign function."""
+    n = 50
+    A, lam, sigma = _sym_case(n, 7, 0.5)
+    W = np.linalg.eigh(A)[1]
+    ev = np.linalg.eigvalsh(A)
+    Q = (W * np.sign(ev - sigma)) @ W.T        # exact polar factor of A - sigma I
+    V, E, log = zo.pqdwhsubit(A, float(np.linalg.norm(A)), Q, np.random.default_rng(0))
+    assert log.rk == int((ev < sigma).sum())
+    assert np.linalg.norm(V.T @ V - np.eye(n)) < 1e-13
+    assert np.linalg.norm(E - V.T @ A @ V) < 1e-13
+    # one iteration leaves err at a few eps (the reference's 4 TOL1 = 20 eps is
+    # tight), the second always passes it
+    assert log.iters <= 2 and log.err[0] < 1e-13 and log.err[-1] <= 4.0 * 10.0 * EPS / 2.0
+
+
+def test_mutation_projector_sign_is_caught():
+    """(Q + I)/2 projects onto the eigenvalues above sigma: the count pin fails."""
+    n = 40
+    A, lam, sigma = _sym_case(n, 11, 0.3)
+    W = np.linalg.eigh(A)[1]
+    ev = np.linalg.eigvalsh(A)
+    Q = (W * np.sign(ev - sigma)) @ W.T
+    _, _, log = zo.pqdwhsubit(A, float(np.linalg.norm(A)), Q, np.random.default_rng(0), projector_sign=+1.0)
+    assert log.rk != int((ev < sigma).sum())
+
+
+def test_eig_below_no_eigenvalue_below():
+    n = 30
+    A, lam, _ = _sym_case(n, 5, 0.5)
+    w, V, log = zo.eig_below(A, float(lam[0]) - 0.5)
+    assert log.rk == 0 and w.size == 0 and V.shape == (n, 0)

@@ -506,6 +506,76 @@ int zolo_syevd_device(int64_t n, double* H, int64_t ldh, double* w, double* Z, i
   return syevd_run(n, H, ldh, w, Z, ldz, (hipStream_t)stream, phase_ms);
 }
 
+int zolo_subit_device(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err,
+                      int* iters, void* stream) {
+  if (n < 1 || n > 32768) return -1;
+  if (!A) return -2;
+  if (lda < n) return -3;
+  if (!(normA > 0.0)) return -4;
+  if (!Q) return -5;
+  if (ldq < n) return -6;
+  if (!rk) return -7;
+  ZOLO_CHECK(ensure_device());
+  return subit_run(n, A, lda, normA, Q, ldq, rk, err, iters, (hipStream_t)stream);
+}
+
+int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
+                    int* rk, double* err, int* iters, void* stream) {
+  if (n < 1 || n > 32768) return -1;
+  if (!Q) return -2;
+  if (ldq < n) return -3;
+  if (!As) return -4;
+  if (lda < n) return -5;
+  if (!w) return -6;
+  if (!V) return -7;
+  if (ldv < n) return -8;
+  if (!rk) return -9;
+  ZOLO_CHECK(ensure_device());
+  return zoloeig_run(n, Q, ldq, As, lda, w, V, ldv, rk, err, iters, (hipStream_t)stream);
+}
+
+int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
+                          double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
+                          void* stream) {
+  if (precision < 0 || precision > 2) return -1;
+  if (rk < 0 || rk > 8) return -2;
+  if (n < 1 || n > 32768) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!std::isfinite(sigma)) return -6;
+  if (!w) return -7;
+  if (!V) return -8;
+  if (ldv < n) return -9;
+  if (!count) return -10;
+  ZOLO_CHECK(ensure_device());
+  hipStream_t s = (hipStream_t)stream;
+  DevMat B, Q, As;
+  ZOLO_CHECK(B.alloc(n));
+  ZOLO_CHECK(Q.alloc(n));
+  ZOLO_CHECK(As.alloc(n));
+  // B = A - sigma I (pqdwhfacs with SIGMA, pqdwheig.f90:113-114); As keeps A
+  ZOLO_HIP(hipMemcpy2DAsync(As.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                            hipMemcpyDeviceToDevice, s));
+  ZOLO_HIP(hipMemcpy2DAsync(B.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                            hipMemcpyDeviceToDevice, s));
+  if (sigma != 0.0) ZOLO_CHECK(k_add_diag<double>(B.p, n, n, -sigma, s));
+  double alpha = 0.0, beta = 0.0;
+  {
+    const size_t bytes = bounds_bytes(n);
+    char* base = nullptr;
+    ZOLO_CHECK(workspace_acquire(bytes, &base));
+    Arena ar(base, bytes);
+    ZOLO_CHECK(bounds_run(0, n, B.p, n, &alpha, &beta, ar, s));
+  }
+  const int r = rk > 0 ? rk : choosem(1.0 / beta);
+  zolo_info_t local{};
+  // polar factor of the symmetric A - sigma I into Q
+  const int st = zolo_pd_run(precision, 'S', r, n, alpha, beta, B.p, n, Q.p, n, nullptr, n, info ? info : &local, s);
+  if (st) return st;
+  // PQDWHSUBIT, PDSYEVD on the Rk block, V1 VV1 (pqdwheig.f90:119-144 = ZoloEig.f90:80-103)
+  return zoloeig_run(n, Q.p, n, As.p, n, w, V, ldv, count, err, iters, s);
+}
+
 int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err) {
   if (n < 1) return -1;
   if (!(fone != 0.0)) return -2;

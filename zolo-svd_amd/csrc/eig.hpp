@@ -45,6 +45,17 @@ size_t syevd_bytes(int64_t n);
 int syevd_run(int64_t n, double* H, int64_t ldh, double* lambda, double* Z, int64_t ldz, hipStream_t s,
               double* phase_ms = nullptr);
 
+// Spectral division (eig_subit.cpp).  subit_run: PQDWHSUBIT -- on entry Q is
+// the polar factor of A - sigma I, on exit Q = V = [V1 V2] (n x n orthogonal,
+// V1 spanning the eigenvectors of the rk eigenvalues below sigma) and A =
+// V^T A V; err = ||E(1:rk, rk+1:n)||_F / normA, it = iterations (1 or 2).
+int subit_run(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err, int* it,
+              hipStream_t s);
+// zoloeig_run: ZoloEig -- subit_run, then w (rk eigenvalues below sigma,
+// ascending) and V (n x rk) their eigenvectors; Q and As are overwritten.
+int zoloeig_run(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
+                int* rk, double* err, int* it, hipStream_t s);
+
 // Apply Q = H_{k0} ... H_{k0+jb-1} (reflectors stored below the first subdiagonal
 // of A's columns k0.., heads at row k + 1, tau[k]) from the left to C (rows
 // k0+1 .. n-1, ncols columns): C <- Q C.  Used by the back-transformation.

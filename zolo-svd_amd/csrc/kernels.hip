@@ -259,6 +259,22 @@ __global__ __launch_bounds__(kT) void chol_leaf_kernel(T* A, int64_t lda, int64_
   if (t == 0 && bad && info) atomicCAS(&info[q], 0, (int)(col0 + bad));
 }
 
+// Uniform [0, 1) entries from a 64-bit hash of (seed, column-major index):
+// the SplitMix64 finaliser (S. Vigna), top 53 bits as the mantissa.  The
+// random start block of the subspace iteration (pqdwhsubit.f90).
+__global__ __launch_bounds__(kT) void fill_uniform_kernel(double* X, int64_t ld, int64_t m, int64_t n,
+                                                          unsigned long long seed) {
+  const int64_t total = m * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % m, j = e / m;
+    unsigned long long z = seed + (unsigned long long)(e + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    X[i + j * ld] = (double)(z >> 11) * 0x1.0p-53;
+  }
+}
+
 inline unsigned grid_for(int64_t total) {
   int64_t g = (total + kT - 1) / kT;
   if (g > 8192) g = 8192;
@@ -367,6 +383,12 @@ int k_chol_leaf(T* A, int64_t lda, int64_t strideA, T* Li, int64_t ldl, int64_t 
   }
   hipLaunchKernelGGL((chol_leaf_kernel<T, kCholLeaf>), dim3(batch), dim3(kT), 0, s, A, lda, strideA, Li, ldl, strideL,
                      nb, col0, info);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+int k_fill_uniform(double* X, int64_t ld, int64_t m, int64_t n, unsigned long long seed, hipStream_t s) {
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(m * n)), dim3(kT), 0, s, X, ld, m, n, seed);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

@@ -33,6 +33,8 @@ template <typename T>
 int k_symmetrize(T* A, int64_t lda, int64_t n, hipStream_t s);
 template <typename T>
 int k_add_diag(T* A, int64_t lda, int64_t n, double d, hipStream_t s);
+// X (m x n, ld) <- uniform [0, 1) numbers, a fixed function of (seed, index).
+int k_fill_uniform(double* X, int64_t ld, int64_t m, int64_t n, unsigned long long seed, hipStream_t s);
 template <typename T>
 int k_chol_leaf(T* A, int64_t lda, int64_t strideA, T* Li, int64_t ldl, int64_t strideL, int nb, int64_t col0,
                 int* info, int batch, hipStream_t s);

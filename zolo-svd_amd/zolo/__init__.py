@@ -11,6 +11,8 @@ Mirrors the reference's Fortran entry points on a 1x1 process grid:
 * ``compute_coeff`` / ``choosem`` / ``hfunc`` / ``mellipke`` / ``mellipj`` -- Auxil_Zolo.f90, hfunc.f90.
 * ``pd_batched_device`` -- PZOLOPD1 over many small FP32 matrices of uneven size in one hipGraph.
 * ``syevd_device`` / ``svd_device`` -- the Zolo-SVD tail: ELPA_PDSYEVD2's role and ZoloCHK1's assembly.
+* ``subit_device`` / ``eig_device`` / ``eig_below_device`` -- spectral division: PQDWHSUBIT
+  (pqdwhsubit.f90:2), ZoloEig (ZoloEig.f90:2) and the driver of pqdwheig.f90.
 
 Host arrays are numpy (column-major copies are made as needed); the
 ``*_device`` variants take torch CUDA tensors (column-major data in HBM, see
@@ -138,6 +140,12 @@ _sig("zolo_batched_tile", C.c_int, [_I64, C.c_int])
 _sig("zolo_syevd_device", C.c_int, [_I64, _P, _I64, _P, _P, _I64, _P, _P])
 _sig("zolo_svd_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _I64, _P, _I64, C.POINTER(_Info), _P,
                                   _P])
+_sig("zolo_subit_device", C.c_int, [_I64, _P, _I64, C.c_double, _P, _I64, C.POINTER(C.c_int), _D,
+                                    C.POINTER(C.c_int), _P])
+_sig("zolo_eig_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P, _I64, C.POINTER(C.c_int), _D, C.POINTER(C.c_int),
+                                  _P])
+_sig("zolo_eig_below_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, C.c_double, _P, _P, _I64,
+                                        C.POINTER(C.c_int), C.POINTER(_Info), _D, C.POINTER(C.c_int), _P])
 _sig("zolo_l0_param_count", C.c_int, [])
 _sig("zolo_l0_set_weights", C.c_int, [_P, C.c_int])
 _sig("zolo_l0_features_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P])
@@ -493,6 +501,53 @@ def svd_device(dA, precision: str = "f64", rk: int = 0):
                                                    ph.ctypes.data_as(C.c_void_p), _stream()))
     names = ("polar_h", "tridiagonal", "divide_conquer", "back_transform", "eig_total")
     return S, U, V, RunInfo.from_c(ci), dict(zip(names, ph.tolist()))
+
+
+def subit_device(dA, normA: float, dQ):
+    """PQDWHSUBIT (pqdwhsubit.f90:2): dQ the polar factor of A - sigma I on entry,
+    V = [V1 V2] on exit; dA overwritten by V^T A V.  Returns (rk, err, iterations)."""
+    n = dA.shape[0]
+    rk, it = C.c_int(0), C.c_int(0)
+    err = C.c_double(0.0)
+    _check("zolo_subit_device", _lib.zolo_subit_device(n, C.c_void_p(dA.data_ptr()), n, float(normA),
+                                                       C.c_void_p(dQ.data_ptr()), n, C.byref(rk), C.byref(err),
+                                                       C.byref(it), _stream()))
+    return rk.value, err.value, it.value
+
+
+def eig_device(dQ, dAs):
+    """ZoloEig (ZoloEig.f90:2): dQ the polar factor of As - sigma I, dAs the matrix
+    (both overwritten).  Returns (w, V, rk, err, iterations): the rk eigenvalues
+    below sigma (ascending) and V column-major n x rk (a torch (rk, n) tensor)."""
+    torch = _torch()
+    n = dQ.shape[0]
+    w = torch.zeros((n,), dtype=torch.float64, device="cuda")
+    V = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    rk, it = C.c_int(0), C.c_int(0)
+    err = C.c_double(0.0)
+    _check("zolo_eig_device", _lib.zolo_eig_device(n, C.c_void_p(dQ.data_ptr()), n, C.c_void_p(dAs.data_ptr()), n,
+                                                   C.c_void_p(w.data_ptr()), C.c_void_p(V.data_ptr()), n,
+                                                   C.byref(rk), C.byref(err), C.byref(it), _stream()))
+    return w[:rk.value], V[:rk.value], rk.value, err.value, it.value
+
+
+def eig_below_device(dA, sigma: float, precision: str = "f64", rk: int = 0):
+    """Eigenpairs of the symmetric A below sigma (pqdwheig.f90:113-144 with the Zolo
+    polar factor of A - sigma I).  dA is not modified.  Returns
+    (w, V, count, RunInfo, err, iterations) as ``eig_device``."""
+    torch = _torch()
+    n = dA.shape[0]
+    w = torch.zeros((n,), dtype=torch.float64, device="cuda")
+    V = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    cnt, it = C.c_int(0), C.c_int(0)
+    err = C.c_double(0.0)
+    ci = _Info()
+    _check("zolo_eig_below_device", _lib.zolo_eig_below_device(PREC[precision], rk, n, C.c_void_p(dA.data_ptr()), n,
+                                                               float(sigma), C.c_void_p(w.data_ptr()),
+                                                               C.c_void_p(V.data_ptr()), n, C.byref(cnt),
+                                                               C.byref(ci), C.byref(err), C.byref(it), _stream()))
+    k = cnt.value
+    return w[:k], V[:k], k, RunInfo.from_c(ci), err.value, it.value
 
 
 # ---------------------------------------------------------------------------
