@@ -212,6 +212,23 @@ int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, 
 int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
                           double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
                           void* stream);
+/* ---- MatrixMarket input (prepare_matrix.f90:154-321 prepare_real_GEM,
+ * :638-759 prepare_real_symm_MM, mmread :1076) ---------------------------------
+ * Coordinate files with field real, integer or pattern (value 1); complex and
+ * the dense 'array' form are refused.  zolo_mm_info: order, entries, mtype (0
+ * symmetric, 1 general, 2 skew-symmetric or hermitian, as prepare_real_GEM's
+ * mtype) and, when bandwidth is not NULL, max |i - j| + 1 over the entries
+ * (this reads the whole file).  zolo_mm_dense_host / _device: the square
+ * matrix into A (column-major, lda >= n; host or device memory), zeroed first,
+ * every entry stored in file order (a repeated position keeps its last value);
+ * symmetric files then A <- A + A^T and A(i, i) <- A(i, i)/2 - sigma; other
+ * types are left as stored (no shift).  The device form uploads the triplets
+ * and assembles on the GPU (same values bit for bit).  Returns 0, or -1
+ * cannot open, -2 banner/format, -3 size line, -4 entry line or early end,
+ * -5 index outside the matrix, -6 not square, -7 A NULL, -8 lda < n. */
+int zolo_mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth);
+int zolo_mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* mtype);
+int zolo_mm_dense_device(const char* path, double sigma, double* A, int64_t lda, int* mtype, void* stream);
 /* FormX2 (FormX2.f90:2) on host arrays: A <- (A + Z)/fone, Z <- the same, err = ||A_new - A_old||_F. */
 int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err);
 /* Host-array form of zolo_syevd_device (the Fortran ELPA_PDSYEVD2 drop-in calls it). */

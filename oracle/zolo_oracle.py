@@ -38,6 +38,10 @@ How each function is pinned (checked by ``tests/test_oracle.py``):
   eigenvalues, the residual ||A V - V diag(w)||, V^T V = I, and the angle
   between span(V) and the exact invariant subspace; a plant of the wrong
   projector sign ((Q + I)/2) fails the count.
+* ``mm_dense`` -- against scipy.io.mmread on files scipy.io.mmwrite wrote
+  (symmetric, general, integer, pattern), where prepare_real_GEM's A + A^T and
+  A(i,i)/2 - sigma must give scipy's full matrix with the diagonal shifted; a
+  plant of the diagonal without the halving fails it.
 """
 from __future__ import annotations
 
@@ -670,6 +674,42 @@ def eig_below(A: np.ndarray, sigma: float, rk: int = 0, method: str = "zolo", se
         r = rk if rk > 0 else choosem(1.0 / beta)
         Q, _ = pzolopd1(B, alpha, beta, r, sym="S")
     return zoloeig(Q, A, np.random.default_rng(seed))
+
+
+def mm_dense(path: str, sigma: float = 0.0, halve_diagonal: bool = True):
+    """Dense matrix of a MatrixMarket coordinate file as prepare_real_GEM
+    (prepare_matrix.f90:214-321) and prepare_real_symm_MM (:638-759) assemble
+    it: entries stored in file order (A(IPQ) = rval(I), :276-283), then for
+    'symmetric' files A <- A + A^T (PDTRAN, :292) and A(i,i) <- A(i,i)/2 -
+    sigma (:293-302); other types as stored.  mmread (:1076-1399) supplies the
+    entries; 'pattern' entries are 1 here.  ``halve_diagonal`` = False plants
+    a mistake for the tests.  Returns (A, mtype)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    head = lines[0].split()
+    symm = head[4].lower()
+    field = head[3].lower()
+    mtype = 0 if symm == "symmetric" else 1 if symm == "general" else 2
+    k = 1
+    while lines[k].startswith("%") or not lines[k].strip():
+        k += 1
+    rows, cols, nnz = (int(t) for t in lines[k].split()[:3])
+    A = np.zeros((rows, cols))
+    got = 0
+    for ln in lines[k + 1:]:
+        if got == nnz:
+            break
+        if not ln.strip() or ln.startswith("%"):
+            continue
+        t = ln.split()
+        v = 1.0 if field == "pattern" else float(t[2].replace("D", "e").replace("d", "e"))
+        A[int(t[0]) - 1, int(t[1]) - 1] = v
+        got += 1
+    if mtype == 0:
+        A = A + A.T
+        d = np.diag_indices(rows)
+        A[d] = (A[d] / 2.0 if halve_diagonal else A[d]) - sigma
+    return A, mtype
 
 
 def symmetric_matrix(n: int, lam: np.ndarray, seed: int = 0) -> tuple[np.ndarray, np.ndarray]:

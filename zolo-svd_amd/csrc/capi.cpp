@@ -576,6 +576,19 @@ int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int
   return zoloeig_run(n, Q.p, n, As.p, n, w, V, ldv, count, err, iters, s);
 }
 
+int zolo_mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth) {
+  return mm_info(path, rows, cols, nnz, mtype, bandwidth);
+}
+
+int zolo_mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* mtype) {
+  return mm_dense_host(path, sigma, A, lda, mtype);
+}
+
+int zolo_mm_dense_device(const char* path, double sigma, double* A, int64_t lda, int* mtype, void* stream) {
+  ZOLO_CHECK(ensure_device());
+  return mm_dense_device(path, sigma, A, lda, mtype, (hipStream_t)stream);
+}
+
 int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err) {
   if (n < 1) return -1;
   if (!(fone != 0.0)) return -2;

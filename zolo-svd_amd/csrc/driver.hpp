@@ -8,6 +8,10 @@ namespace zolo {
 int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A, int64_t lda,
                 double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s,
                 int max_it = 6, bool restart = false);
+// MatrixMarket input (mm_reader.cpp)
+int mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth);
+int mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* mtype);
+int mm_dense_device(const char* path, double sigma, double* A, int64_t lda, int* mtype, hipStream_t s);
 int qdwh_run(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu, double* H,
              int64_t ldh, zolo_info_t* info, hipStream_t s);
 int form_h(const Ctx& c, const double* U, int64_t ldu, const double* A, int64_t lda, int64_t n, double* H,

@@ -191,6 +191,44 @@ __global__ __launch_bounds__(kT) void add_diag_kernel(T* A, int64_t lda, int64_t
     A[i + i * lda] = (T)((double)A[i + i * lda] + d);
 }
 
+// A(ri[e], ci[e]) = v[e]; positions distinct (no two threads store to one)
+__global__ __launch_bounds__(kT) void scatter_kernel(double* A, int64_t lda, const int64_t* ri, const int64_t* ci,
+                                                     const double* v, int64_t z) {
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < z; e += (int64_t)gridDim.x * kT)
+    A[ri[e] + ci[e] * lda] = v[e];
+}
+
+// Symmetric MatrixMarket assembly: A <- A + A^T off the diagonal and
+// A(i, i) <- (A(i, i) + A(i, i))/2 - sigma on it.  One 32 x 32 tile pair per
+// workgroup (tile row bi >= tile column bj; the others exit): both tiles are
+// read and written along columns through LDS.
+constexpr int kSymTile = 32;
+__global__ __launch_bounds__(256) void mm_symmetrize_kernel(double* A, int64_t lda, int64_t n, double sigma) {
+  const int64_t bi = blockIdx.x, bj = blockIdx.y;
+  if (bi < bj) return;
+  __shared__ double t1[kSymTile][kSymTile + 1], t2[kSymTile][kSymTile + 1];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t r0 = bi * kSymTile, c0 = bj * kSymTile;
+  // t1[x][k] = A(r0 + x, c0 + k), t2[x][k] = A(c0 + x, r0 + k)
+  for (int k = ty; k < kSymTile; k += 8) {
+    t1[tx][k] = (r0 + tx < n && c0 + k < n) ? A[(r0 + tx) + (c0 + k) * lda] : 0.0;
+    t2[tx][k] = (c0 + tx < n && r0 + k < n) ? A[(c0 + tx) + (r0 + k) * lda] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < kSymTile; k += 8) {
+    const int64_t r = r0 + tx, c = c0 + k;
+    if (r < n && c < n) {  // A(r, c) + A(c, r)
+      const double v = (r == c) ? (t1[tx][k] + t1[tx][k]) / 2.0 - sigma : t1[tx][k] + t2[k][tx];
+      A[r + c * lda] = v;
+    }
+  }
+  if (bi == bj) return;  // the diagonal tile is complete
+  for (int k = ty; k < kSymTile; k += 8) {
+    const int64_t r = c0 + tx, c = r0 + k;
+    if (r < n && c < n) A[r + c * lda] = t1[k][tx] + t2[tx][k];  // A(c, r) + A(r, c)
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Cholesky + triangular inverse of one nb x nb diagonal block in LDS.
 // In: lower triangle of A (n x n block at A, ld lda).  Out: L in A (strict upper
@@ -383,6 +421,26 @@ int k_chol_leaf(T* A, int64_t lda, int64_t strideA, T* Li, int64_t ldl, int64_t 
   }
   hipLaunchKernelGGL((chol_leaf_kernel<T, kCholLeaf>), dim3(batch), dim3(kT), 0, s, A, lda, strideA, Li, ldl, strideL,
                      nb, col0, info);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+int k_scatter(double* A, int64_t lda, const int64_t* ri, const int64_t* ci, const double* v, int64_t z,
+              hipStream_t s) {
+  if (z <= 0) return kOk;
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(z)), dim3(kT), 0, s, A, lda, ri, ci, v, z);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+int k_mm_symmetrize(double* A, int64_t lda, int64_t n, double sigma, hipStream_t s) {
+  if (n <= 0) return kOk;
+  const int64_t nt = (n + kSymTile - 1) / kSymTile;
+  if (nt > 65535) {
+    set_error("symmetric assembly: order too large");
+    return kErrArg;
+  }
+  hipLaunchKernelGGL(mm_symmetrize_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, s, A, lda, n, sigma);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

@@ -11,6 +11,8 @@ Mirrors the reference's Fortran entry points on a 1x1 process grid:
 * ``compute_coeff`` / ``choosem`` / ``hfunc`` / ``mellipke`` / ``mellipj`` -- Auxil_Zolo.f90, hfunc.f90.
 * ``pd_batched_device`` -- PZOLOPD1 over many small FP32 matrices of uneven size in one hipGraph.
 * ``syevd_device`` / ``svd_device`` -- the Zolo-SVD tail: ELPA_PDSYEVD2's role and ZoloCHK1's assembly.
+* ``mm_info`` / ``mm_read`` / ``mm_read_device`` -- MatrixMarket input as prepare_real_GEM /
+  prepare_real_symm_MM (prepare_matrix.f90:154, :638) assemble it.
 * ``subit_device`` / ``eig_device`` / ``eig_below_device`` -- spectral division: PQDWHSUBIT
   (pqdwhsubit.f90:2), ZoloEig (ZoloEig.f90:2) and the driver of pqdwheig.f90.
 
@@ -146,6 +148,10 @@ _sig("zolo_eig_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P, _I64, C.POIN
                                   _P])
 _sig("zolo_eig_below_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, C.c_double, _P, _P, _I64,
                                         C.POINTER(C.c_int), C.POINTER(_Info), _D, C.POINTER(C.c_int), _P])
+_sig("zolo_mm_info", C.c_int, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
+                               C.POINTER(_I64)])
+_sig("zolo_mm_dense_host", C.c_int, [C.c_char_p, C.c_double, _D, _I64, C.POINTER(C.c_int)])
+_sig("zolo_mm_dense_device", C.c_int, [C.c_char_p, C.c_double, _P, _I64, C.POINTER(C.c_int), _P])
 _sig("zolo_l0_param_count", C.c_int, [])
 _sig("zolo_l0_set_weights", C.c_int, [_P, C.c_int])
 _sig("zolo_l0_features_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P])
@@ -501,6 +507,46 @@ def svd_device(dA, precision: str = "f64", rk: int = 0):
                                                    ph.ctypes.data_as(C.c_void_p), _stream()))
     names = ("polar_h", "tridiagonal", "divide_conquer", "back_transform", "eig_total")
     return S, U, V, RunInfo.from_c(ci), dict(zip(names, ph.tolist()))
+
+
+MM_TYPES = {0: "symmetric", 1: "general", 2: "other"}
+
+
+def mm_info(path: str, bandwidth: bool = False) -> dict:
+    """Header of a MatrixMarket coordinate file (and max |i - j| + 1 when asked: reads every entry)."""
+    r, c, z, bw = _I64(0), _I64(0), _I64(0), _I64(0)
+    t = C.c_int(0)
+    _check("zolo_mm_info", _lib.zolo_mm_info(os.fsencode(path), C.byref(r), C.byref(c), C.byref(z), C.byref(t),
+                                             C.byref(bw) if bandwidth else None))
+    out = dict(rows=r.value, cols=c.value, nnz=z.value, mtype=t.value, kind=MM_TYPES[t.value])
+    if bandwidth:
+        out["bandwidth"] = bw.value
+    return out
+
+
+def mm_read(path: str, sigma: float = 0.0):
+    """Dense matrix of a MatrixMarket file on the host, assembled as prepare_real_GEM: symmetric
+    files get each stored entry on both sides of the diagonal and the diagonal shifted by -sigma
+    (A <- A + A^T, then A(i, i)/2 - sigma); other types are kept as stored.
+    Returns (A Fortran-ordered, mtype)."""
+    n = mm_info(path)["rows"]
+    A = np.zeros((n, n), order="F")
+    t = C.c_int(0)
+    _check("zolo_mm_dense_host", _lib.zolo_mm_dense_host(os.fsencode(path), float(sigma), _dptr(A), max(n, 1),
+                                                         C.byref(t)))
+    return A, t.value
+
+
+def mm_read_device(path: str, sigma: float = 0.0):
+    """As ``mm_read``, assembled on the GPU from the uploaded entries; returns (column-major tensor, mtype)."""
+    torch = _torch()
+    n = mm_info(path)["rows"]
+    dA = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    t = C.c_int(0)
+    _check("zolo_mm_dense_device", _lib.zolo_mm_dense_device(os.fsencode(path), float(sigma),
+                                                             C.c_void_p(dA.data_ptr()), max(n, 1), C.byref(t),
+                                                             _stream()))
+    return dA, t.value
 
 
 def subit_device(dA, normA: float, dQ):
