@@ -212,6 +212,13 @@ int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, 
 int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
                           double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
                           void* stream);
+/* Host-array forms (the Fortran PQDWHSUBIT and ZoloEig drop-ins call them):
+ * zolo_subit_host overwrites A and Q as zolo_subit_device; zolo_eig_host leaves
+ * Q and As unchanged and returns w(1:rk) and V(:, 1:rk). */
+int zolo_subit_host(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err,
+                    int* iters);
+int zolo_eig_host(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
+                  int* rk);
 /* ---- MatrixMarket input (prepare_matrix.f90:154-321 prepare_real_GEM,
  * :638-759 prepare_real_symm_MM, mmread :1076) ---------------------------------
  * Coordinate files with field real, integer or pattern (value 1); complex and

@@ -12,19 +12,21 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "zolo-svd_amd")
 
 ENTRY = ["pzolopd1_", "pzolopd2_", "pqdwhfacs_", "zolochol_", "zoloqr_", "pdnormest_", "mpdgeqrf_", "mpdorgqr_",
-         "formx2_", "hfunc_"]
+         "formx2_", "hfunc_", "elpa_pdsyevd2_", "pqdwhsubit_", "zoloeig_"]
 
 
-def test_fortran_library_exports_entry_points():
+def test_fortran_library_exports_entry_points(zl):
+    # zl first: torch's HIP runtime must be the one libzolo.so binds to
     lib = ctypes.CDLL(os.path.join(PKG, "libzolo_f.so"))
     for name in ENTRY:
         assert hasattr(lib, name), name
     out = subprocess.run(["nm", "-D", os.path.join(PKG, "libzolo_f.so")], capture_output=True, text=True).stdout
-    for name in ["auxil_zolo", "compute_coeff", "choosem", "mellipke", "mellipj"]:
+    for name in ["auxil_zolo", "compute_coeff", "choosem", "mellipke", "mellipj", "mod_prepare_matrix",
+                 "prepare_real_symm_mm", "prepare_real_gem"]:
         assert name in out
 
 
-def test_fortran_hfunc_on_host():
+def test_fortran_hfunc_on_host(zl):
     lib = ctypes.CDLL(os.path.join(PKG, "libzolo_f.so"))
     lib.hfunc_.restype = ctypes.c_double
     x = ctypes.c_double(1.0)
@@ -36,6 +38,19 @@ def test_fortran_driver_on_gpu():
     exe = os.path.join(PKG, "test_pzolopd1")
     assert os.path.exists(exe), "build the Fortran driver first (make -C zolo-svd_amd)"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout)
+    print(r.stderr)
+    assert r.returncode == 0
+    assert "PASS" in r.stdout.splitlines()[-1]
+
+
+@pytest.mark.gpu
+def test_fortran_spectral_division_driver_on_gpu(tmp_path):
+    """pqdwheig.f90's flow through the Fortran drop-ins: prepare_real_symm_MM,
+    PQDWHFACS with a shift, PQDWHSUBIT, ELPA_PDSYEVD2 on the Rk block, ZoloEig."""
+    exe = os.path.join(PKG, "test_zoloeig")
+    assert os.path.exists(exe), "build the Fortran driver first (make -C zolo-svd_amd)"
+    r = subprocess.run([exe, os.path.join(str(tmp_path), "a.mtx")], capture_output=True, text=True, timeout=240)
     print(r.stdout)
     print(r.stderr)
     assert r.returncode == 0

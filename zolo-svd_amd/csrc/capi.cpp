@@ -576,6 +576,63 @@ int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int
   return zoloeig_run(n, Q.p, n, As.p, n, w, V, ldv, count, err, iters, s);
 }
 
+int zolo_subit_host(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err,
+                    int* iters) {
+  if (n < 1 || n > 32768) return -1;
+  if (!A) return -2;
+  if (lda < n) return -3;
+  if (!(normA > 0.0)) return -4;
+  if (!Q) return -5;
+  if (ldq < n) return -6;
+  if (!rk) return -7;
+  ZOLO_CHECK(ensure_device());
+  DevMat dA, dQ;
+  ZOLO_CHECK(dA.alloc(n));
+  ZOLO_CHECK(dQ.alloc(n));
+  ZOLO_CHECK(dA.upload(A, lda, n));
+  ZOLO_CHECK(dQ.upload(Q, ldq, n));
+  ZOLO_CHECK(subit_run(n, dA.p, n, normA, dQ.p, n, rk, err, iters, nullptr));
+  ZOLO_CHECK(dA.download(A, lda, n));
+  ZOLO_CHECK(dQ.download(Q, ldq, n));
+  return kOk;
+}
+
+int zolo_eig_host(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
+                  int* rk) {
+  if (n < 1 || n > 32768) return -1;
+  if (!Q) return -2;
+  if (ldq < n) return -3;
+  if (!As) return -4;
+  if (lda < n) return -5;
+  if (!w) return -6;
+  if (!V) return -7;
+  if (ldv < n) return -8;
+  if (!rk) return -9;
+  ZOLO_CHECK(ensure_device());
+  DevMat dQ, dA, dV;
+  ZOLO_CHECK(dQ.alloc(n));
+  ZOLO_CHECK(dA.alloc(n));
+  ZOLO_CHECK(dV.alloc(n));
+  double* dw = nullptr;
+  ZOLO_HIP(hipMalloc(&dw, sizeof(double) * n));
+  struct Free {
+    double* p;
+    ~Free() { (void)hipFree(p); }
+  } fw{dw};
+  ZOLO_CHECK(dQ.upload(Q, ldq, n));
+  ZOLO_CHECK(dA.upload(As, lda, n));
+  int k = 0;
+  ZOLO_CHECK(zoloeig_run(n, dQ.p, n, dA.p, n, dw, dV.p, n, &k, nullptr, nullptr, nullptr));
+  ZOLO_HIP(hipDeviceSynchronize());
+  if (k > 0) {
+    ZOLO_HIP(hipMemcpy(w, dw, sizeof(double) * k, hipMemcpyDeviceToHost));
+    ZOLO_HIP(hipMemcpy2D(V, sizeof(double) * ldv, dV.p, sizeof(double) * n, sizeof(double) * n, k,
+                         hipMemcpyDeviceToHost));
+  }
+  *rk = k;
+  return kOk;
+}
+
 int zolo_mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth) {
   return mm_info(path, rows, cols, nnz, mtype, bandwidth);
 }

@@ -3,8 +3,9 @@
 !> The external subroutines keep the reference's names and argument lists
 !> (pzolopd1.f90:2, pzolopd2.f90:2, pqdwhfacs.f90:2, zolochol.f90:2,
 !> zoloqr.f90:2, pdnormest.f90:2, mpdgeqrf.f:5, mpdorgqr.f:5, FormX2.f90:2,
-!> hfunc.f90:2, elpa_pdsyevd2.F90:1) and module auxil_zolo keeps
-!> Auxil_Zolo.f90's interface.  The
+!> hfunc.f90:2, elpa_pdsyevd2.F90:1, pqdwhsubit.f90:2, ZoloEig.f90:2), module
+!> auxil_zolo keeps Auxil_Zolo.f90's interface and module mod_prepare_matrix
+!> the MatrixMarket readers of prepare_matrix.f90.  The
 !> descriptors must describe a 1x1 process grid (RSRC = CSRC = 0, LLD >= M);
 !> the BLACS context is never dereferenced, so the library links no MPI or
 !> BLACS.  Work is done by libzolo.so through the ISO_C_BINDING interfaces of
@@ -28,6 +29,8 @@ module zolo_capi
   integer, save :: zolo_precision = 0
   !> record of the last PZOLOPD1 / PZOLOPD2 / PQDWHFACS call
   type(zolo_info_t), save, target :: zolo_last_info
+  !> Rk of the last ZoloEig call (the reference computes it but returns it to no caller)
+  integer, save :: zolo_eig_count = 0
 
   interface
     integer(c_int) function c_zolo_pd_host(precision, sym, rk, n, alpha, beta, a, lda, u, ldu, h, ldh, info) &
@@ -131,6 +134,35 @@ module zolo_capi
     real(c_double) function c_zolo_hfunc(x) bind(C, name='zolo_hfunc')
       import :: c_double
       real(c_double), value :: x
+    end function
+    integer(c_int) function c_zolo_subit_host(n, a, lda, norma, q, ldq, rk, err, iters) &
+        bind(C, name='zolo_subit_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n, lda, ldq
+      real(c_double), value :: norma
+      real(c_double) :: a(*), q(*)
+      integer(c_int) :: rk, iters
+      real(c_double) :: err
+    end function
+    integer(c_int) function c_zolo_eig_host(n, q, ldq, as, lda, w, v, ldv, rk) bind(C, name='zolo_eig_host')
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n, ldq, lda, ldv
+      real(c_double) :: q(*), as(*), w(*), v(*)
+      integer(c_int) :: rk
+    end function
+    integer(c_int) function c_zolo_mm_info(path, rows, cols, nnz, mtype, bw) bind(C, name='zolo_mm_info')
+      import :: c_int, c_int64_t, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int64_t) :: rows, cols, nnz, bw
+      integer(c_int) :: mtype
+    end function
+    integer(c_int) function c_zolo_mm_dense_host(path, sigma, a, lda, mtype) bind(C, name='zolo_mm_dense_host')
+      import :: c_int, c_int64_t, c_double, c_char
+      character(kind=c_char) :: path(*)
+      real(c_double), value :: sigma
+      real(c_double) :: a(*)
+      integer(c_int64_t), value :: lda
+      integer(c_int) :: mtype
     end function
     subroutine c_zolo_mellipke(alpha, tol, k, e) bind(C, name='zolo_mellipke')
       import :: c_double
@@ -551,3 +583,131 @@ double precision function hfunc(x)
   double precision, intent(in) :: x
   hfunc = c_zolo_hfunc(x)
 end function hfunc
+
+!> PQDWHSUBIT (pqdwhsubit.f90:2): Q, the polar factor of A - sigma I on entry,
+!> becomes V = [V1 V2] (V1 spanning the eigenvectors of the Rk eigenvalues
+!> below sigma) and A becomes V^T A V; NORMA = ||A||_F.  WORK, TAU and WORK2
+!> are accepted for the reference's argument list and unused.
+subroutine pqdwhsubit(n, a, ia, ja, desca, norma, q, iq, jq, descq, work, lwork, tau, work2, lwork2, rk, info)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: n, ia, ja, iq, jq, lwork, lwork2, desca(9), descq(9)
+  double precision, intent(in) :: norma
+  double precision, intent(inout) :: a(*), q(*), work(*), tau(*), work2(*)
+  integer, intent(out) :: rk, info
+  integer :: k, ioa, ioq
+  integer(c_int) :: crk, cit
+  real(c_double) :: err
+  info = 0
+  rk = 0
+  if (n == 0) return
+  if (n < 0) then
+    info = -1
+    return
+  end if
+  k = zolo_desc_check(desca, ia - 1 + n, ja - 1 + n)
+  if (k /= 0) then
+    info = -(500 + k)
+    return
+  end if
+  k = zolo_desc_check(descq, iq - 1 + n, jq - 1 + n)
+  if (k /= 0) then
+    info = -(1000 + k)
+    return
+  end if
+  if (.not. (norma > 0d0)) then
+    info = -6
+    return
+  end if
+  if (lwork == -12345 .or. lwork2 == -12345) tau(1) = work(1) + work2(1)  ! accepted, unused
+  ioa = ia + (ja - 1) * desca(9)
+  ioq = iq + (jq - 1) * descq(9)
+  info = c_zolo_subit_host(int(n, c_int64_t), a(ioa), int(desca(9), c_int64_t), norma, q(ioq), &
+                           int(descq(9), c_int64_t), crk, err, cit)
+  rk = crk
+end subroutine pqdwhsubit
+
+!> ZoloEig (ZoloEig.f90:2) on a 1x1 grid: Q (N x N, leading dimension N) the
+!> polar factor of As - sigma I, As the matrix.  On exit TAU(1:Rk) holds the
+!> eigenvalues below sigma (ascending) and WORK(1:N*Rk) their eigenvectors
+!> (N x Rk, leading dimension N), as the reference leaves them; Rk is in
+!> zolo_eig_count of module zolo_capi.  Q and As are left unchanged; WORK2
+!> is accepted and unused.  LWORK < N*N: INFO = -8.
+subroutine zoloeig(n, nblk, q, as, contxt, mpi_newcomm, work, lwork, work2, lwork2, tau, info)
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: n, nblk, contxt, mpi_newcomm, lwork, lwork2
+  double precision, intent(inout) :: q(*), as(*), work(*), work2(*), tau(*)
+  integer, intent(out) :: info
+  integer(c_int) :: crk
+  info = 0
+  zolo_eig_count = 0
+  if (n == 0) return
+  if (n < 0) then
+    info = -1
+    return
+  end if
+  if (lwork < n * n) then
+    info = -8
+    return
+  end if
+  if (nblk == -12345 .or. contxt == -12345 .or. mpi_newcomm == -12345 .or. lwork2 == -12345) work2(1) = 0d0
+  info = c_zolo_eig_host(int(n, c_int64_t), q, int(n, c_int64_t), as, int(n, c_int64_t), tau, work, &
+                         int(n, c_int64_t), crk)
+  zolo_eig_count = crk
+end subroutine zoloeig
+
+!> prepare_matrix.f90's MatrixMarket readers on a 1x1 grid (the descriptor's
+!> LLD is the leading dimension of A): the file's entries stored in order,
+!> symmetric files then A <- A + A^T and A(i,i) <- A(i,i)/2 - SIGMA.  As in the
+!> reference, SIGMA is set to zero on entry (prepare_matrix.f90:214, :651),
+!> so no shift is applied, and NNZ returns the number of entries.  Z is
+!> accepted and unused (the transpose is formed in place).
+module mod_prepare_matrix
+  use zolo_capi
+  use iso_c_binding
+  implicit none
+contains
+  subroutine prepare_real_symm_mm(filename, na, nnz, myid, myrow, mycol, nprow, npcol, sc_desc, a, z, sigma)
+    character(len=*), intent(in) :: filename
+    integer, intent(in) :: na, myid, myrow, mycol, nprow, npcol, sc_desc(*)
+    integer, intent(inout) :: nnz
+    double precision, intent(inout) :: a(*), z(*), sigma
+    character :: sym
+    call prepare_real_gem(sym, filename, na, nnz, myid, myrow, mycol, nprow, npcol, sc_desc, a, z, sigma)
+  end subroutine prepare_real_symm_mm
+
+  !> SYM returns 'S' (symmetric file), 'G' (general) or 'O' (other), as the
+  !> reference's 'S', 'Gen' and 'Other' read into one character.  A file that
+  !> cannot be read, is not square or not of order NA stops the program, as
+  !> the reference's mmread does.
+  subroutine prepare_real_gem(sym, filename, na, nnz, myid, myrow, mycol, nprow, npcol, sc_desc, a, z, sigma)
+    character, intent(out) :: sym
+    character(len=*), intent(in) :: filename
+    integer, intent(in) :: na, myid, myrow, mycol, nprow, npcol, sc_desc(*)
+    integer, intent(inout) :: nnz
+    double precision, intent(inout) :: a(*), z(*), sigma
+    integer(c_int64_t) :: rows, cols, cnnz, bw
+    integer(c_int) :: mtype, st
+    character(kind=c_char, len=len_trim(filename) + 1) :: cpath
+    cpath = trim(filename) // c_null_char
+    sigma = 0d0
+    if (myid < 0 .or. myrow /= 0 .or. mycol /= 0 .or. nprow /= 1 .or. npcol /= 1) stop 'prepare_matrix: 1x1 grid only'
+    if (na == -12345) z(1) = 0d0
+    st = c_zolo_mm_info(cpath, rows, cols, cnnz, mtype, bw)
+    if (st /= 0) stop 'prepare_matrix: cannot read the MatrixMarket file'
+    if (rows /= na .or. cols /= na) stop 'prepare_matrix: the file is not of order NA'
+    st = c_zolo_mm_dense_host(cpath, sigma, a, int(sc_desc(9), c_int64_t), mtype)
+    if (st /= 0) stop 'prepare_matrix: bad MatrixMarket file'
+    nnz = int(cnnz)
+    if (mtype == 0) then
+      sym = 'S'
+    else if (mtype == 1) then
+      sym = 'G'
+    else
+      sym = 'O'
+    end if
+  end subroutine prepare_real_gem
+end module mod_prepare_matrix
