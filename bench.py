@@ -260,6 +260,51 @@ def svd_tail(torch, zl, n, seed):
     return out
 
 
+def eig_below(torch, zl, n, seed):
+    """Spectral division (PQDWHSUBIT + ZoloEig): the eigenpairs below sigma of a
+    symmetric A = W diag(lam) W^T, lam uniform on [-1, 1] with no eigenvalue
+    within 0.01 of sigma = median, next to the full eigensolver on the same A."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed + 7)
+    Q, R = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    W = Q * torch.sign(torch.diagonal(R))
+    lam = torch.sort(torch.rand(n, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0).values
+    sigma = float(torch.median(lam))
+    lam = torch.where((lam - sigma).abs() < 0.01, lam + torch.where(lam >= sigma, 0.01, -0.01), lam)
+    A = (W * lam) @ W.T
+    A = 0.5 * (A + A.T)
+    dA = A.T.contiguous()
+    nrm = torch.linalg.norm(A).item()
+    exact = torch.sort(lam).values
+    below = exact[exact < sigma]
+    out = {"n": n, "sigma": sigma, "count_exact": int(below.numel())}
+    for prec in ("f64", "bf16x3"):
+        zl.eig_below_device(dA, sigma, precision=prec)     # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        w, V, count, info, err, iters = zl.eig_below_device(dA, sigma, precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        Vc = V.T
+        res = (torch.linalg.norm(A @ Vc - Vc * w) / nrm).item()
+        orth = torch.linalg.norm(Vc.T @ Vc - torch.eye(count, dtype=torch.float64, device="cuda")).item()
+        werr = (torch.max(torch.abs(w - below)).item()) if count == below.numel() else float("nan")
+        out[prec] = {"s": dt, "count": count, "subspace_iters": iters, "offdiag": err, "schedule": info.schedule,
+                     "eig_max_abs_err": werr, "residual": res, "orth": orth}
+        log(f"eig below sigma n={n} {prec}: {dt:.3f} s, {out[prec]}")
+        del w, V
+    H = dA.clone()
+    zl.syevd_device(H)
+    H = dA.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    zl.syevd_device(H)
+    torch.cuda.synchronize()
+    out["full_syevd_s"] = time.perf_counter() - t0
+    log(f"full eigensolver n={n}: {out['full_syevd_s']:.3f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,7 +318,8 @@ def main():
     ap.add_argument("--small-count", type=int, default=4096, help="matrices of the batched small-PD workload")
     ap.add_argument("--no-small", action="store_true", help="skip the batched small-PD workload")
     ap.add_argument("--svd", type=int, default=8192, help="also time the Zolo-SVD tail at this n (0: skip)")
-    ap.add_argument("--part", default="all", choices=["all", "small", "svd"],
+    ap.add_argument("--eig", type=int, default=0, help="also time the eigenpairs below sigma at this n (0: skip)")
+    ap.add_argument("--part", default="all", choices=["all", "small", "svd", "eig"],
                     help="run only the batched small-PD or the Zolo-SVD section (for profiling)")
     ap.add_argument("--factors", default="haar", choices=["haar", "reflector"],
                     help="singular-vector factors of the n=16384 workload: Haar (torch QR) or products of two "
@@ -302,6 +348,8 @@ def main():
         out = {"part": args.part}
         if args.part == "small":
             out["small_pds"] = small_pds(torch, zl, args.small_count, args.seed)
+        elif args.part == "eig":
+            out["eig_below"] = eig_below(torch, zl, args.eig or 8192, args.seed)
         else:
             out["zolo_svd"] = svd_tail(torch, zl, args.svd, args.seed)
         print(json.dumps(out), flush=True)
@@ -363,6 +411,9 @@ def main():
     svd = None
     if args.svd > 0:
         svd = svd_tail(torch, zl, args.svd, args.seed)
+    eig = None
+    if args.eig > 0:
+        eig = eig_below(torch, zl, args.eig, args.seed)
 
     out = {
         "metric": METRIC,
@@ -392,6 +443,8 @@ def main():
         out["small_pds"] = small
     if svd:
         out["zolo_svd"] = svd
+    if eig:
+        out["eig_below"] = eig
     if "f64" in compare:
         out["speedup_vs_fp64_gpu"] = compare["f64"]["s"] / sec
     if cpu:
