@@ -14,8 +14,9 @@
 //    eigenvectors V1 VV1.
 //
 // All products are the library's FP64 MFMA GEMMs, the QR factorisations the
-// blocked Householder GEQRF with its block reflectors applied to the identity
-// (the full Q of PDORGQR(N, N, kk)), the eigensolver syevd_run.
+// blocked Householder GEQRF of the n x kk block with its block reflectors
+// applied to [I; 0] (PDORGQR(N, KK, KK) and PDORGQR(N, N, KK)), the
+// eigensolver syevd_run.
 #include <algorithm>
 #include <cmath>
 
@@ -54,9 +55,10 @@ int fro2(const double* A, int64_t lda, int64_t m, int64_t n, double* part, doubl
   return kOk;
 }
 
-// V = the full n x n orthogonal factor of the QR of X (n x n, overwritten by
-// the reflectors): Householder GEQRF, then its block reflectors applied to I.
-int qr_full_q(const Ctx& c, double* X, int64_t n, double* V) {
+// Q (n x m) = the first m columns of the orthogonal factor of the QR of the
+// n x k block X (overwritten by the reflectors): GEQRF of k columns, then
+// their block reflectors applied to [I; 0] (PDGEQRF + PDORGQR(N, M, K)).
+int qr_q(const Ctx& c, double* X, int64_t n, int64_t k, double* Q, int64_t m) {
   const int64_t nout = (n + 255) / 256;
   const size_t bytes = dense_qr_bytes<double>(n) + sizeof(double) * ((size_t)n + (size_t)nout * 256 * 256) + 8192;
   char* base = nullptr;
@@ -66,12 +68,11 @@ int qr_full_q(const Ctx& c, double* X, int64_t n, double* V) {
   double* Tall = ar.take<double>((size_t)nout * 256 * 256);
   ZOLO_ARENA_OK(ar);
   const size_t mark = ar.used();
-  ZOLO_CHECK(dense_geqrf<double>(c, X, n, 0, n, 1, tau, Tall, ar));
-  // the factorisation's scratch is free again for the application
+  ZOLO_CHECK(dense_geqrf_cols<double>(c, X, n, n, k, tau, Tall, ar));
+  // the factorisation's scratch is free again for forming Q
   ar.reset();
   ar.take<char>(mark);
-  ZOLO_CHECK(k_set_identity_top<double>(V, n, 0, n, n, 1, c.s));
-  ZOLO_CHECK(dense_apply_q<double>(c, X, n, 0, n, 1, Tall, V, n, 0, ar));
+  ZOLO_CHECK(dense_orgqr_cols<double>(c, X, n, n, k, Tall, Q, n, m, ar));
   return kOk;
 }
 
@@ -105,26 +106,23 @@ int subit_run(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_
   const int64_t rk = std::min<int64_t>(n, std::max<int64_t>(0, (int64_t)std::llround(h)));
   int64_t kk = std::min<int64_t>(rk + 3, n);  // oversampling 3
   const double tol1 = 10.0 * dlamch_p() / 2.0;
-  // random start block in the first kk columns, zeros after them
-  ZOLO_CHECK(k_fill<double>(X.p, n, 0, n, n, 0.0, 1, s));
+  // random n x kk start block
   ZOLO_CHECK(k_fill_uniform(X.p, n, n, kk, 0x5eedull + (unsigned long long)n, s));
   double err = 0.0;
   int it = 1;
   for (;;) {
-    // X1 = orthonormal basis of X's kk columns (the zero columns leave tau = 0)
-    ZOLO_CHECK(qr_full_q(c, X.p, n, V.p));
-    // Y = C X1, then V = [V1 V2] from the QR of Y
-    ZOLO_CHECK(k_fill<double>(X.p, n, 0, n, n, 0.0, 1, s));
+    // X1 (n x kk) = orthonormal basis of X (pqdwhsubit.f90:266-267)
+    ZOLO_CHECK(qr_q(c, X.p, n, kk, V.p, kk));
+    // Y = C X1 (:269), then V = [V1 V2] (n x n) from the QR of Y (:271-272)
     ZOLO_CHECK(gemm_nn(c, 0, n, kk, n, Q, ldq, V.p, n, X.p, n));
-    ZOLO_CHECK(qr_full_q(c, X.p, n, V.p));
+    ZOLO_CHECK(qr_q(c, X.p, n, kk, V.p, n));
     // E = V^T A V into X
     ZOLO_CHECK(gemm_nn(c, 0, n, n, n, A, lda, V.p, n, W.p, n));
     ZOLO_CHECK(gemm_nn(c, 1, n, n, n, V.p, n, W.p, n, X.p, n));
     ZOLO_CHECK(fro2(X.p + rk * n, n, rk, n - rk, red.p, red.p + 256, &h, s));
     err = std::sqrt(h) / normA;
-    if (err <= 4.0 * tol1 || it == 2) break;
-    // one more iteration from X = C V1 (kk = Rk)
-    ZOLO_CHECK(k_fill<double>(W.p, n, 0, n, n, 0.0, 1, s));
+    if (err <= 4.0 * tol1 || it == 2 || rk == 0) break;
+    // one more iteration from X = C V1 (kk = Rk, :305-311)
     ZOLO_CHECK(gemm_nn(c, 0, n, rk, n, Q, ldq, V.p, n, W.p, n));
     std::swap(X.p, W.p);
     kk = rk;

@@ -110,5 +110,13 @@ int dense_geqrf(const Ctx& c, T* A, int64_t lda, int64_t sA, int64_t n, int batc
 template <typename T>
 int dense_apply_q(const Ctx& c, const T* A, int64_t lda, int64_t sA, int64_t n, int batch, const T* Tall, T* M,
                   int64_t ldm, int64_t sM, Arena& ar);
+// Tall n x k Householder QR (k <= n; the same storage as dense_geqrf, bytes
+// dense_qr_bytes(n)) and Q (n x m) = the first m columns of its orthogonal
+// factor (m <= n; PDORGQR(N, M, K)).
+template <typename T>
+int dense_geqrf_cols(const Ctx& c, T* A, int64_t lda, int64_t n, int64_t k, T* tau, T* Tall, Arena& ar);
+template <typename T>
+int dense_orgqr_cols(const Ctx& c, const T* A, int64_t lda, int64_t n, int64_t k, const T* Tall, T* Q, int64_t ldq,
+                     int64_t m, Arena& ar);
 
 }  // namespace zolo

@@ -143,7 +143,9 @@ int apply_block_reflector(const Ctx& c, const T* V, int64_t ldv, int64_t sV, con
 
 template <typename T>
 int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch, T* tau, T* Tall, Arena& ar,
-               QrShape sh) {
+               QrShape sh, int64_t kc = -1) {
+  // kc: columns to factor (the Dense shape may stop early: an n x kc matrix)
+  if (kc < 0 || sh != QrShape::Dense) kc = n;
   // small stacks: every inner panel in one workgroup (narrower panels in FP64)
   const bool fused = sh == QrShape::Stack && n + kQrPanelWgCols<T> <= kQrPanelWgRows;
   const int nb = fused ? kQrPanelWgCols<T> : kQrNb, nbo = kQrOuter;
@@ -159,8 +161,8 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
   const int64_t sV = mr * nbo, sG = (int64_t)nbo * nbo, sW = (int64_t)nbo * n;
   const int64_t nout = (n + nbo - 1) / nbo;
   const int64_t sTall = nout * sG;
-  for (int64_t P = 0; P < n; P += nbo) {
-    const int JB = (int)std::min<int64_t>(nbo, n - P);
+  for (int64_t P = 0; P < kc; P += nbo) {
+    const int JB = (int)std::min<int64_t>(nbo, kc - P);
     T* Tb = Tall + (P / nbo) * sG;
     ZOLO_CHECK(k_fill<T>(Tb, nbo, sTall, nbo, nbo, 0.0, batch, c.s));
     for (int64_t p = P; p < P + JB; p += nb) {
@@ -202,7 +204,7 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
       ZOLO_CHECK(gemm_t<T>(c, d));
       ZOLO_CHECK(combine_t<T>(c, Tb, sTall, G, sG, (T*)q.tmp, sG, 0, JB, batch, nb));
     }
-    const int64_t ncols = n - P - JB;
+    const int64_t ncols = kc - P - JB;
     if (ncols > 0)
       ZOLO_CHECK(apply_block_reflector<T>(c, V, ro.m(), sV, Tb, nbo, sTall, JB, S + P + (P + JB) * lds, lds, sS,
                                           ro.len1, n - P, ro.len2, P, ncols, true, (T*)q.W, (T*)q.W2, sW, batch));
@@ -276,7 +278,55 @@ int apply_q_dense_impl(const Ctx& c, const T* S, int64_t lds, int64_t sS, const 
   return kOk;
 }
 
+// Q (n x m) = the first m columns of H_0 ... H_{k-1} from a dense n x k GEQRF:
+// Q = [I; 0], then the outer blocks from last to first, each on columns P ..
+// m-1 only (columns before P are still unit vectors above its rows).
+template <typename T>
+int orgqr_dense_cols_impl(const Ctx& c, const T* A, int64_t lda, int64_t n, int64_t k, const T* Tall, T* Q,
+                          int64_t ldq, int64_t m, Arena& ar) {
+  const int nbo = kQrOuter;
+  QrScratch q;
+  ZOLO_CHECK(take_scratch<T>(ar, n, 1, q));
+  T* V = (T*)q.V;
+  const int64_t mr = n + nbo;
+  const int64_t sV = mr * nbo, sG = (int64_t)nbo * nbo, sW = (int64_t)nbo * n;
+  const int64_t nout = (k + nbo - 1) / nbo;
+  ZOLO_CHECK(k_set_identity_top<T>(Q, ldq, 0, n, m, 1, c.s));
+  for (int64_t ob = nout - 1; ob >= 0; --ob) {
+    const int64_t P = ob * nbo;
+    if (P >= m) continue;
+    const int JB = (int)std::min<int64_t>(nbo, k - P);
+    const Rows ro = rows_of(QrShape::Dense, n, P, JB);
+    ZOLO_CHECK(k_qr_extract_v<T>(A, lda, 0, V, ro.m(), sV, P, JB, n, ro.r1, ro.r2, 1, c.s));
+    ZOLO_CHECK(apply_block_reflector<T>(c, V, ro.m(), sV, Tall + ob * sG, nbo, nout * sG, JB, Q + P + P * ldq, ldq,
+                                        0, ro.len1, n - P, 0, P, m - P, false, (T*)q.W, (T*)q.W2, sW, 1));
+  }
+  return kOk;
+}
+
 }  // namespace
+
+template <typename T>
+int dense_geqrf_cols(const Ctx& c, T* A, int64_t lda, int64_t n, int64_t k, T* tau, T* Tall, Arena& ar) {
+  if (k < 1 || k > n) {
+    set_error("dense QR: column count out of range");
+    return kErrArg;
+  }
+  return geqrf_impl<T>(c, A, lda, 0, n, 1, tau, Tall, ar, QrShape::Dense, k);
+}
+
+template <typename T>
+int dense_orgqr_cols(const Ctx& c, const T* A, int64_t lda, int64_t n, int64_t k, const T* Tall, T* Q, int64_t ldq,
+                     int64_t m, Arena& ar) {
+  if (k < 1 || k > n || m < 1 || m > n) {
+    set_error("dense ORGQR: column counts out of range");
+    return kErrArg;
+  }
+  return orgqr_dense_cols_impl<T>(c, A, lda, n, k, Tall, Q, ldq, m, ar);
+}
+template int dense_geqrf_cols<double>(const Ctx&, double*, int64_t, int64_t, int64_t, double*, double*, Arena&);
+template int dense_orgqr_cols<double>(const Ctx&, const double*, int64_t, int64_t, int64_t, const double*, double*,
+                                      int64_t, int64_t, Arena&);
 
 int apply_trd_block(const Ctx& c, const double* A, int64_t lda, const double* tau, int64_t k0, int jb, int64_t n,
                     double* C, int64_t ldc, int64_t ncols, Arena& ar) {
