@@ -318,7 +318,7 @@ def main():
     ap.add_argument("--small-count", type=int, default=4096, help="matrices of the batched small-PD workload")
     ap.add_argument("--no-small", action="store_true", help="skip the batched small-PD workload")
     ap.add_argument("--svd", type=int, default=8192, help="also time the Zolo-SVD tail at this n (0: skip)")
-    ap.add_argument("--eig", type=int, default=0, help="also time the eigenpairs below sigma at this n (0: skip)")
+    ap.add_argument("--eig", type=int, default=8192, help="also time the eigenpairs below sigma at this n (0: skip)")
     ap.add_argument("--part", default="all", choices=["all", "small", "svd", "eig"],
                     help="run only the batched small-PD or the Zolo-SVD section (for profiling)")
     ap.add_argument("--factors", default="haar", choices=["haar", "reflector"],
@@ -349,7 +349,7 @@ def main():
         if args.part == "small":
             out["small_pds"] = small_pds(torch, zl, args.small_count, args.seed)
         elif args.part == "eig":
-            out["eig_below"] = eig_below(torch, zl, args.eig or 8192, args.seed)
+            out["eig_below"] = eig_below(torch, zl, args.eig, args.seed)
         else:
             out["zolo_svd"] = svd_tail(torch, zl, args.svd, args.seed)
         print(json.dumps(out), flush=True)
@@ -413,7 +413,11 @@ def main():
         svd = svd_tail(torch, zl, args.svd, args.seed)
     eig = None
     if args.eig > 0:
-        eig = eig_below(torch, zl, args.eig, args.seed)
+        try:
+            eig = eig_below(torch, zl, args.eig, args.seed)
+        except zl.ZoloError as e:  # an auxiliary section: report it, keep the headline line
+            log(f"eig below sigma failed: {e}")
+            eig = {"n": args.eig, "error": str(e)}
 
     out = {
         "metric": METRIC,
