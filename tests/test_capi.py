@@ -65,3 +65,35 @@ def test_argument_validation_without_gpu(zl):
     assert e.value.status == -1
     with pytest.raises(zl.ZoloError):
         zl.compute_coeff(2.0, 1e-16, 9)
+
+
+def test_spectral_division_argument_codes_without_gpu(zl):
+    """The spectral-division entry points check their arguments before any HIP call."""
+    import ctypes as C
+    L = zl.LIB
+    buf = (C.c_double * 16)()
+    rk, it = C.c_int(0), C.c_int(0)
+    err = C.c_double(0.0)
+    cnt = C.c_int(0)
+    p = C.cast(buf, C.c_void_p)
+    # zolo_subit_device(n, A, lda, normA, Q, ldq, rk, err, iters, stream)
+    assert L.zolo_subit_device(0, p, 1, 1.0, p, 1, C.byref(rk), C.byref(err), C.byref(it), None) == -1
+    assert L.zolo_subit_device(4, None, 4, 1.0, p, 4, C.byref(rk), C.byref(err), C.byref(it), None) == -2
+    assert L.zolo_subit_device(4, p, 3, 1.0, p, 4, C.byref(rk), C.byref(err), C.byref(it), None) == -3
+    assert L.zolo_subit_device(4, p, 4, 0.0, p, 4, C.byref(rk), C.byref(err), C.byref(it), None) == -4
+    assert L.zolo_subit_device(4, p, 4, 1.0, p, 3, C.byref(rk), C.byref(err), C.byref(it), None) == -6
+    assert L.zolo_subit_device(4, p, 4, 1.0, p, 4, None, C.byref(err), C.byref(it), None) == -7
+    # zolo_eig_device(n, Q, ldq, As, lda, w, V, ldv, rk, err, iters, stream)
+    assert L.zolo_eig_device(4, p, 4, p, 4, p, p, 3, C.byref(rk), None, None, None) == -8
+    assert L.zolo_eig_device(4, p, 4, p, 4, None, p, 4, C.byref(rk), None, None, None) == -6
+    # zolo_eig_below_device(precision, rk, n, A, lda, sigma, w, V, ldv, count, info, err, iters, stream)
+    assert L.zolo_eig_below_device(3, 0, 4, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -1
+    assert L.zolo_eig_below_device(0, 9, 4, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -2
+    assert L.zolo_eig_below_device(0, 0, 0, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -3
+    assert L.zolo_eig_below_device(0, 0, 4, p, 4, float("nan"), p, p, 4, C.byref(cnt), None, None, None,
+                                   None) == -6
+    assert L.zolo_eig_below_device(0, 0, 4, p, 4, 0.0, p, p, 4, None, None, None, None, None) == -10
+    # host forms
+    d = C.cast(buf, C.POINTER(C.c_double))
+    assert L.zolo_subit_host(4, d, 4, -1.0, d, 4, C.byref(rk), None, None) == -4
+    assert L.zolo_eig_host(4, d, 4, d, 4, d, d, 4, None) == -9

@@ -148,6 +148,8 @@ _sig("zolo_eig_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P, _I64, C.POIN
                                   _P])
 _sig("zolo_eig_below_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, C.c_double, _P, _P, _I64,
                                         C.POINTER(C.c_int), C.POINTER(_Info), _D, C.POINTER(C.c_int), _P])
+_sig("zolo_subit_host", C.c_int, [_I64, _D, _I64, C.c_double, _D, _I64, C.POINTER(C.c_int), _D, C.POINTER(C.c_int)])
+_sig("zolo_eig_host", C.c_int, [_I64, _D, _I64, _D, _I64, _D, _D, _I64, C.POINTER(C.c_int)])
 _sig("zolo_mm_info", C.c_int, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
                                C.POINTER(_I64)])
 _sig("zolo_mm_dense_host", C.c_int, [C.c_char_p, C.c_double, _D, _I64, C.POINTER(C.c_int)])
