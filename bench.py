@@ -78,8 +78,14 @@ def make_workload(torch, n, seed, factors="haar"):
     return A, dA, U, V
 
 
-def run_pd(zl, dA, prec, dU, dH):
+def run_pd(zl, dA, prec, dU, dH, bounds="qr"):
+    """One polar decomposition with its bounds: 'qr' takes alpha, beta from the QR
+    of A/alpha that the first QR step then reuses (zolo_pd_auto_device); 'lu'
+    computes them first by LU + condition estimate as the reference's driver."""
     import torch
+    if bounds == "qr":
+        _, _, info = zl.pzolopd_auto_device(dA, precision=prec, dU=dU, dH=dH)
+        return info
     t0 = time.perf_counter()
     al, be = zl.bounds_device(dA, "zolo")
     torch.cuda.synchronize()
@@ -318,6 +324,8 @@ def main():
     ap.add_argument("--small-count", type=int, default=4096, help="matrices of the batched small-PD workload")
     ap.add_argument("--no-small", action="store_true", help="skip the batched small-PD workload")
     ap.add_argument("--svd", type=int, default=8192, help="also time the Zolo-SVD tail at this n (0: skip)")
+    ap.add_argument("--bounds", default="qr", choices=["qr", "lu"],
+                    help="alpha/beta from the QR the first step reuses, or by LU + condition estimate first")
     ap.add_argument("--eig", type=int, default=8192, help="also time the eigenpairs below sigma at this n (0: skip)")
     ap.add_argument("--part", default="all", choices=["all", "small", "svd", "eig"],
                     help="run only the batched small-PD or the Zolo-SVD section (for profiling)")
@@ -362,7 +370,7 @@ def main():
     torch.cuda.synchronize()
 
     for i in range(args.warmup):
-        info = run_pd(zl, dA, args.precision, dU, dH)
+        info = run_pd(zl, dA, args.precision, dU, dH, args.bounds)
         torch.cuda.synchronize()
         log(f"warmup {i}: {info.time_ms / 1e3:.3f} s schedule {info.schedule}")
     zl.profile_enable(True)
@@ -370,7 +378,7 @@ def main():
     t0 = time.perf_counter()
     infos = []
     for i in range(args.steps):
-        infos.append(run_pd(zl, dA, args.precision, dU, dH))
+        infos.append(run_pd(zl, dA, args.precision, dU, dH, args.bounds))
         log(f"step {i}: {infos[-1].time_ms / 1e3:.3f} s")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -389,7 +397,7 @@ def main():
                 continue
             torch.cuda.synchronize()
             c0 = time.perf_counter()
-            ci = run_pd(zl, dA, prec, dU, dH)
+            ci = run_pd(zl, dA, prec, dU, dH, args.bounds)
             torch.cuda.synchronize()
             c1 = time.perf_counter()
             compare[prec] = {"s": c1 - c0, "schedule": ci.schedule, "errors": errors(torch, A, dU, dH, Ug, Vg),
@@ -435,7 +443,8 @@ def main():
                 "s geometric 1..1e-8 with 5% at 1e-8)",
         "config": {"workload": f"Zolo-PD {args.precision} n={n}, kappa=1e8 with 5% cluster at 1e-8",
                    "n": n, "r": info.r, "precision": args.precision, "schedule": info.schedule,
-                   "polish_passes": info.polish_passes, "cg_iters": info.cg_iters, "parallelism": "single GPU"},
+                   "polish_passes": info.polish_passes, "cg_iters": info.cg_iters, "bounds": args.bounds,
+                   "beta": info.beta, "parallelism": "single GPU"},
         "errors": err,
         "phase_ms": {k: round(v, 1) for k, v in info.phase_ms.items() if v > 0},
         "compare": compare,

@@ -75,6 +75,18 @@ void zolo_mellipj(double u, double alpha, double tol, double* sn, double* cn, do
  * Device pointers; A is read-only. */
 int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                    int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream);
+/* zolo_pd_device with the bounds computed inside (no counterpart in the
+ * reference, whose driver computes alpha, beta by LU + condition estimate,
+ * test_zolodrv1.f90:179-189): alpha = ||A||_F, A/alpha = Q_X R by the same
+ * Householder QR the first QR step of the iteration needs, and beta =
+ * 1/(||R^{-1}||_1 estimate * sqrt(n)) with Higham's estimator (DTRCON's);
+ * sigma_min(A/alpha) = 1/||R^{-1}||_2 >= beta as for the LU estimate.  The
+ * first QR step then starts from that factorisation.  rk = 0 picks
+ * choosem(1/beta); info->alpha, info->beta, info->r report the choice.
+ * Errors: -6 A NULL, -8 U NULL, the rest as zolo_pd_device; a numerically
+ * singular A (no positive bound) is kErrArg. */
+int zolo_pd_auto_device(int precision, char sym, int rk, int64_t n, const double* A, int64_t lda, double* U,
+                        int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream);
 /* Same with host arrays (copied to and from the device). */
 int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                  int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info);

@@ -31,7 +31,8 @@ How each function is pinned (checked by ``tests/test_oracle.py``):
   singular values fill [beta, 1]: the result is orthogonal to the FP32 unit
   roundoff after the last step and not before it.
 * ``lacn2_norm1_inv`` -- against LAPACK dgecon (scipy) and the exact
-  ||X^{-1}||_1 on small matrices.
+  ||X^{-1}||_1 on small matrices; ``lacn2_norm1_inv_r`` / ``bounds_from_qr``
+  against LAPACK dtrcon on R, the exact ||R^{-1}||_1, and beta <= sigma_min.
 * ``normest`` -- against numpy's exact 2-norm.
 * ``pqdwhsubit`` / ``zoloeig`` / ``eig_below`` -- against LAPACK's dsyevd
   (numpy.linalg.eigvalsh): the count of eigenvalues below sigma, those
@@ -306,6 +307,23 @@ def lacn2_norm1_inv(lu_piv) -> tuple[float, int]:
         y = sla.solve_triangular(lu, b, lower=False, trans=1)   # inv(U^T) b
         return sla.solve_triangular(lu, y, lower=True, unit_diagonal=True, trans=1)
 
+    return _lacn2(solve, n)
+
+
+def lacn2_norm1_inv_r(R: np.ndarray) -> tuple[float, int]:
+    """The same estimator on the upper triangle of R (DTRCON('1', 'U', 'N')'s use
+    of dlacn2): an estimate of ||R^{-1}||_1.  Returns (estimate, solves)."""
+    Ru = np.triu(R)
+
+    def solve(b, trans):
+        return sla.solve_triangular(Ru, b, lower=False, trans=trans)
+
+    return _lacn2(solve, Ru.shape[0])
+
+
+def _lacn2(solve, n: int) -> tuple[float, int]:
+    """dlacn2's reverse-communication loop, unrolled; solve(b, trans) applies
+    M^{-1} (trans 0) or M^{-T} (trans 1)."""
     nsolve = 0
     x = solve(np.full(n, 1.0 / n), 0); nsolve += 1
     if n == 1:
@@ -356,6 +374,18 @@ def bounds_estimate(A: np.ndarray, convention: str = "zolo") -> tuple[float, flo
     if convention == "qdwh":
         beta *= float(np.abs(X).sum(axis=0).max())
     return alpha, beta
+
+
+def bounds_from_qr(A: np.ndarray) -> tuple[float, float]:
+    """alpha = ||A||_F and beta = 1 / (||R^{-1}||_1 estimate * sqrt(n)) for
+    A/alpha = Q R: test_zolodrv1.f90:179-189's bound with the estimate taken on
+    R (||R^{-1}||_2 = ||(A/alpha)^{-1}||_2 <= sqrt(n) ||R^{-1}||_1, so beta is a
+    lower bound on sigma_min(A/alpha) exactly as the LU form)."""
+    n = A.shape[0]
+    alpha = float(np.linalg.norm(A))
+    R = np.linalg.qr(A / alpha, mode="r")
+    est, _ = lacn2_norm1_inv_r(R)
+    return alpha, min(1.0, 1.0 / (est * math.sqrt(n)))
 
 
 def normest(A: np.ndarray, tol: float = 1e-6, maxiter: int = 100) -> tuple[float, int]:

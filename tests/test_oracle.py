@@ -228,6 +228,30 @@ def test_lacn2_matches_dgecon(n):
     assert est <= np.abs(np.linalg.inv(X)).sum(axis=0).max() * (1 + 1e-12)
 
 
+@pytest.mark.parametrize("n", [2, 9, 60, 200])
+def test_lacn2_r_matches_dtrcon(n):
+    rng = np.random.default_rng(100 + n)
+    R = np.linalg.qr(rng.standard_normal((n, n)), mode="r")
+    est, _ = zo.lacn2_norm1_inv_r(R)
+    rc, info = sla.lapack.dtrcon(R, norm="1", uplo="U", diag="N")
+    assert info == 0
+    assert est == pytest.approx(1.0 / (rc * np.abs(R).sum(axis=0).max()), rel=1e-10)
+    assert est <= np.abs(np.linalg.inv(R)).sum(axis=0).max() * (1 + 1e-12)
+
+
+@pytest.mark.parametrize("kind,kappa", [("geometric", 1e7), ("cluster", 1e10), ("sine", 1.0)])
+def test_beta_from_qr_is_lower_bound(kind, kappa):
+    A, U, s, V = zo.make_matrix(80, kind, kappa, 5)
+    al, be = zo.bounds_from_qr(A)
+    assert al == pytest.approx(np.linalg.norm(A))
+    smin = s.min() / al
+    assert be <= smin * (1 + 1e-12)
+    assert be >= smin / (10 * math.sqrt(80))
+    # the estimate on R and the one on the LU factors agree to within the 1-norm's sqrt(n) slack
+    _, be_lu = zo.bounds_estimate(A)
+    assert be_lu / math.sqrt(80) <= be <= be_lu * math.sqrt(80)
+
+
 def test_beta_is_lower_bound():
     A, U, s, V = zo.make_matrix(50, "geometric", 1e7, 4)
     al, be = zo.bounds_estimate(A)

@@ -614,6 +614,56 @@ int trsv(const Ctx& c, const double* M, int64_t ld, int64_t n, int trans, int lo
   return kOk;
 }
 
+// Higham's estimate of ||M^{-1}||_1 (LAPACK dlacn2 as xGECON / xTRCON drive
+// it, with the final alternating-sign vector test); solve(h, trans) replaces
+// the host vector h by M^{-1} h (trans = 0) or M^{-T} h (trans = 1).
+template <typename Solve>
+int lacn2_inv(int64_t n, Solve&& solve, double* est_out) {
+  std::vector<double> h(n), sgn(n);
+  auto asum = [&]() { double t = 0; for (int64_t i = 0; i < n; ++i) t += std::fabs(h[i]); return t; };
+  auto iamax = [&]() { int64_t b = 0; for (int64_t i = 1; i < n; ++i) if (std::fabs(h[i]) > std::fabs(h[b])) b = i; return b; };
+  double est;
+  std::fill(h.begin(), h.end(), 1.0 / n);
+  ZOLO_CHECK(solve(h, 0));
+  if (n == 1) {
+    est = std::fabs(h[0]);
+  } else {
+    est = asum();
+    for (int64_t i = 0; i < n; ++i) sgn[i] = h[i] >= 0 ? 1.0 : -1.0;
+    h = sgn;
+    ZOLO_CHECK(solve(h, 1));
+    int64_t j = iamax();
+    int iter = 2;
+    while (true) {
+      std::fill(h.begin(), h.end(), 0.0);
+      h[j] = 1.0;
+      ZOLO_CHECK(solve(h, 0));
+      const double estold = est;
+      est = asum();
+      bool same = true;
+      for (int64_t i = 0; i < n; ++i)
+        if ((h[i] >= 0 ? 1.0 : -1.0) != sgn[i]) { same = false; break; }
+      if (same || est <= estold) break;
+      for (int64_t i = 0; i < n; ++i) sgn[i] = h[i] >= 0 ? 1.0 : -1.0;
+      h = sgn;
+      ZOLO_CHECK(solve(h, 1));
+      const int64_t jlast = j;
+      j = iamax();
+      if (h[jlast] != std::fabs(h[j]) && iter < 5) {
+        ++iter;
+        continue;
+      }
+      break;
+    }
+    for (int64_t i = 0; i < n; ++i) h[i] = ((i % 2 == 0) ? 1.0 : -1.0) * (1.0 + double(i) / double(n - 1));
+    ZOLO_CHECK(solve(h, 0));
+    const double temp = 2.0 * asum() / (3.0 * n);
+    if (temp > est) est = temp;
+  }
+  *est_out = est;
+  return kOk;
+}
+
 }  // namespace
 
 size_t bounds_bytes(int64_t n) {
@@ -670,8 +720,7 @@ int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* 
   hipLaunchKernelGGL(lu_block_inv_kernel, dim3((unsigned)nblk), dim3(kTb), 0, s, LU, n, n, Linv, Uinv);
   ZOLO_HIP(hipGetLastError());
   // dlacn2 on inv(U) inv(L) (xGECON applies no permutation).
-  std::vector<double> h(n), sgn(n);
-  auto solve = [&](int trans) -> int {
+  auto solve = [&](std::vector<double>& h, int trans) -> int {
     ZOLO_HIP(hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     if (!trans) {
       ZOLO_CHECK(trsv(c, LU, n, n, 0, 1, Linv, Uinv, x, y));   // L (unit lower)
@@ -684,50 +733,39 @@ int bounds_run(int convention, int64_t n, const double* A, int64_t lda, double* 
     ZOLO_HIP(hipStreamSynchronize(s));
     return kOk;
   };
-  auto asum = [&]() { double t = 0; for (int64_t i = 0; i < n; ++i) t += std::fabs(h[i]); return t; };
-  auto iamax = [&]() { int64_t b = 0; for (int64_t i = 1; i < n; ++i) if (std::fabs(h[i]) > std::fabs(h[b])) b = i; return b; };
-  double est;
-  std::fill(h.begin(), h.end(), 1.0 / n);
-  ZOLO_CHECK(solve(0));
-  if (n == 1) {
-    est = std::fabs(h[0]);
-  } else {
-    est = asum();
-    for (int64_t i = 0; i < n; ++i) sgn[i] = h[i] >= 0 ? 1.0 : -1.0;
-    h = sgn;
-    ZOLO_CHECK(solve(1));
-    int64_t j = iamax();
-    int iter = 2;
-    while (true) {
-      std::fill(h.begin(), h.end(), 0.0);
-      h[j] = 1.0;
-      ZOLO_CHECK(solve(0));
-      const double estold = est;
-      est = asum();
-      bool same = true;
-      for (int64_t i = 0; i < n; ++i)
-        if ((h[i] >= 0 ? 1.0 : -1.0) != sgn[i]) { same = false; break; }
-      if (same || est <= estold) break;
-      for (int64_t i = 0; i < n; ++i) sgn[i] = h[i] >= 0 ? 1.0 : -1.0;
-      h = sgn;
-      ZOLO_CHECK(solve(1));
-      const int64_t jlast = j;
-      j = iamax();
-      if (h[jlast] != std::fabs(h[j]) && iter < 5) {
-        ++iter;
-        continue;
-      }
-      break;
-    }
-    for (int64_t i = 0; i < n; ++i) h[i] = ((i % 2 == 0) ? 1.0 : -1.0) * (1.0 + double(i) / double(n - 1));
-    ZOLO_CHECK(solve(0));
-    const double temp = 2.0 * asum() / (3.0 * n);
-    if (temp > est) est = temp;
-  }
+  double est = 0.0;
+  ZOLO_CHECK(lacn2_inv(n, solve, &est));
   const double rcond = 1.0 / est;
   *alpha = al;
   *beta = xnorm1 * rcond / std::sqrt((double)n);
   return kOk;
+}
+
+size_t rbound_bytes(int64_t n) {
+  const int64_t nb = (n + kTb - 1) / kTb;
+  return sizeof(double) * (size_t)(3 * n + 2 * nb * kTb * kTb) + 8 * 256;
+}
+
+int rbound_run(int64_t n, const double* R, int64_t ldr, double* est, Arena& ar, hipStream_t s) {
+  Ctx c;
+  c.s = s;
+  const int64_t nblk = (n + kTb - 1) / kTb;
+  double* x = ar.take<double>(n);
+  double* y = ar.take<double>(n);
+  double* Linv = ar.take<double>((size_t)nblk * kTb * kTb);  // formed from the reflectors, unused
+  double* Uinv = ar.take<double>((size_t)nblk * kTb * kTb);
+  ZOLO_ARENA_OK(ar);
+  hipLaunchKernelGGL(lu_block_inv_kernel, dim3((unsigned)nblk), dim3(kTb), 0, s, R, ldr, n, Linv, Uinv);
+  ZOLO_HIP(hipGetLastError());
+  // dlacn2 on inv(R) (as DTRCON('1', 'U', 'N') drives it)
+  auto solve = [&](std::vector<double>& h, int trans) -> int {
+    ZOLO_HIP(hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    ZOLO_CHECK(trsv(c, R, ldr, n, trans, trans ? 1 : 0, Linv, Uinv, x, y));  // R, or R^T (lower)
+    ZOLO_HIP(hipMemcpyAsync(h.data(), y, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    ZOLO_HIP(hipStreamSynchronize(s));
+    return kOk;
+  };
+  return lacn2_inv(n, solve, est);
 }
 
 size_t normest_bytes(int64_t m, int64_t n) {

@@ -93,13 +93,25 @@ void zolo_mellipj(double u, double alpha, double tol, double* sn, double* cn, do
 
 int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                    int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream) {
+  if (!(alpha > 0.0)) return -5;  // (alpha, beta) = (0, 0) is zolo_pd_auto_device's mode, not this one's
+  if (!(beta > 0.0) || !(beta <= 1.0)) return -6;
   ZOLO_CHECK(ensure_device());
   return zolo_pd_run(precision, sym, rk, n, alpha, beta, A, lda, U, ldu, H, ldh, info, (hipStream_t)stream);
+}
+
+int zolo_pd_auto_device(int precision, char sym, int rk, int64_t n, const double* A, int64_t lda, double* U,
+                        int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream) {
+  if (!A) return -6;
+  if (!U) return -8;
+  ZOLO_CHECK(ensure_device());
+  return zolo_pd_run(precision, sym, rk, n, 0.0, 0.0, A, lda, U, ldu, H, ldh, info, (hipStream_t)stream);
 }
 
 int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                  int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) {
   if (n < 1) return -4;
+  if (!(alpha > 0.0)) return -5;
+  if (!(beta > 0.0) || !(beta <= 1.0)) return -6;
   if (lda < n) return -8;
   if (ldu < n) return -10;
   if (H && ldh < n) return -12;
@@ -120,6 +132,8 @@ int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, doubl
 int zolo_pd2_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                   int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) {
   if (n < 1) return -4;
+  if (!(alpha > 0.0)) return -5;
+  if (!(beta > 0.0) || !(beta <= 1.0)) return -6;
   if (lda < n) return -8;
   if (ldu < n) return -10;
   if (H && ldh < n) return -12;

@@ -8,6 +8,10 @@ namespace zolo {
 int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A, int64_t lda,
                 double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s,
                 int max_it = 6, bool restart = false);
+// est = Higham's estimate of ||R^{-1}||_1 for the upper triangle of R (n x n,
+// ld ldr; what lies below the diagonal is ignored), as DTRCON('1', 'U', 'N')
+size_t rbound_bytes(int64_t n);
+int rbound_run(int64_t n, const double* R, int64_t ldr, double* est, Arena& ar, hipStream_t s);
 // MatrixMarket input (mm_reader.cpp)
 int mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth);
 int mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* mtype);

@@ -191,6 +191,15 @@ __global__ __launch_bounds__(kT) void add_diag_kernel(T* A, int64_t lda, int64_t
     A[i + i * lda] = (T)((double)A[i + i * lda] + d);
 }
 
+// A(i, j) *= f for i <= j (the upper triangle; the rest is left alone)
+__global__ __launch_bounds__(kT) void scale_upper_kernel(double* A, int64_t lda, int64_t n, double f) {
+  const int64_t total = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
+    const int64_t i = e % n, j = e / n;
+    if (i <= j) A[i + j * lda] *= f;
+  }
+}
+
 // A(ri[e], ci[e]) = v[e]; positions distinct (no two threads store to one)
 __global__ __launch_bounds__(kT) void scatter_kernel(double* A, int64_t lda, const int64_t* ri, const int64_t* ci,
                                                      const double* v, int64_t z) {
@@ -421,6 +430,13 @@ int k_chol_leaf(T* A, int64_t lda, int64_t strideA, T* Li, int64_t ldl, int64_t 
   }
   hipLaunchKernelGGL((chol_leaf_kernel<T, kCholLeaf>), dim3(batch), dim3(kT), 0, s, A, lda, strideA, Li, ldl, strideL,
                      nb, col0, info);
+  ZOLO_HIP(hipGetLastError());
+  return kOk;
+}
+
+int k_scale_upper(double* A, int64_t lda, int64_t n, double f, hipStream_t s) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(scale_upper_kernel, dim3(grid_for(n * n)), dim3(kT), 0, s, A, lda, n, f);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

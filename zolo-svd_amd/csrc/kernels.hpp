@@ -35,6 +35,8 @@ template <typename T>
 int k_add_diag(T* A, int64_t lda, int64_t n, double d, hipStream_t s);
 // X (m x n, ld) <- uniform [0, 1) numbers, a fixed function of (seed, index).
 int k_fill_uniform(double* X, int64_t ld, int64_t m, int64_t n, unsigned long long seed, hipStream_t s);
+// A(i, j) *= f on the upper triangle (i <= j) of an n x n block
+int k_scale_upper(double* A, int64_t lda, int64_t n, double f, hipStream_t s);
 // A(ri[e], ci[e]) = v[e] for e < z (0-based, distinct positions, all inside A)
 int k_scatter(double* A, int64_t lda, const int64_t* ri, const int64_t* ci, const double* v, int64_t z,
               hipStream_t s);

@@ -204,7 +204,7 @@ size_t qr_step_bytes(int64_t n, int r) {
 
 template <typename T>
 int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const double* d_sdiag, const double* d_alpha,
-            T* Z, int64_t ldz, int64_t sZ, Arena& ar, int* nz) {
+            T* Z, int64_t ldz, int64_t sZ, Arena& ar, int* nz, const QrPre<T>* pre) {
   const int64_t ls = 2 * n, ss = 2 * n * n;
   T* S = ar.take<T>((size_t)r * ss);
   T* Q = ar.take<T>((size_t)r * ss);
@@ -224,14 +224,22 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
     *nz = r;
     return kOk;
   }
-  // X = Q_X R (X itself is kept: it is the iterate FormX2 needs)
+  // X = Q_X R (X itself is kept: it is the iterate FormX2 needs), or the
+  // caller's factorisation of it
   const int64_t nout = (n + 255) / 256;
-  T* Xw = ar.take<T>((size_t)n * n);
-  T* taux = ar.take<T>((size_t)n);
-  T* Tx = ar.take<T>((size_t)nout * 256 * 256);
-  ZOLO_ARENA_OK(ar);
-  ZOLO_CHECK(k_scale_copy<T, T>(X, ldx, Xw, n, n, n, 1.0, c.s));
-  ZOLO_CHECK(dense_geqrf<T>(c, Xw, n, 0, n, 1, taux, Tx, ar));
+  T* Xw = nullptr;
+  T* Tx = nullptr;
+  if (pre) {
+    Xw = pre->Xw;
+    Tx = pre->Tx;
+  } else {
+    Xw = ar.take<T>((size_t)n * n);
+    T* taux = ar.take<T>((size_t)n);
+    Tx = ar.take<T>((size_t)nout * 256 * 256);
+    ZOLO_ARENA_OK(ar);
+    ZOLO_CHECK(k_scale_copy<T, T>(X, ldx, Xw, n, n, n, 1.0, c.s));
+    ZOLO_CHECK(dense_geqrf<T>(c, Xw, n, 0, n, 1, taux, Tx, ar));
+  }
   // [R; s_q I] = [Q~1_q; Q~2_q] R~_q for every shift; rows sorted large to small
   // (R on top) keep the Householder QR row-wise accurate for tiny s_q
   ZOLO_CHECK(k_build_stack<T>(Xw, n, S, ls, ss, n, d_sdiag, r, c.s, true));
@@ -264,7 +272,7 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
                             const double*, T*, int64_t, int64_t, Arena&, int*, bool);                                \
   template size_t qr_step_bytes<T>(int64_t, int);                                                                \
   template int qr_step<T>(const Ctx&, const T*, int64_t, int64_t, int, const double*, const double*, T*, int64_t, \
-                          int64_t, Arena&, int*);
+                          int64_t, Arena&, int*, const QrPre<T>*);
 
 ZOLO_LA_INST(double)
 ZOLO_LA_INST(float)

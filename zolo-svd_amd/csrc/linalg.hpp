@@ -79,9 +79,17 @@ int chol_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const dou
 // returned in slot 0 and *nz = 1; otherwise the r slots hold Z_q and *nz = r.
 template <typename T>
 size_t qr_step_bytes(int64_t n, int r);
+// A QR of the iterate computed beforehand for the factored form: Xw holds R
+// (upper triangle, already scaled to the iterate) and the reflectors below it,
+// Tx the outer T factors (dense_geqrf's storage).
+template <typename T>
+struct QrPre {
+  T* Xw;
+  T* Tx;
+};
 template <typename T>
 int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const double* d_sdiag, const double* d_alpha,
-            T* Z, int64_t ldz, int64_t sZ, Arena& ar, int* nz);
+            T* Z, int64_t ldz, int64_t sZ, Arena& ar, int* nz, const QrPre<T>* pre = nullptr);
 
 // Row structure of a blocked Householder QR: Stack = [X; s I] (mpdgeqrf.f),
 // Dense = one n x n matrix, Tri = [R; s I] with R upper triangular.

@@ -99,12 +99,16 @@ int form_h(const Ctx& c, const double* U, int64_t ldu, const double* A, int64_t 
 int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A, int64_t lda,
                 double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, hipStream_t s, int max_it,
                 bool restart) {
+  // alpha = beta = 0: the bounds come from the QR of A/alpha that the first
+  // QR step needs anyway (see below); rk = 0 then picks choosem(1/beta)
+  const bool autob = alpha == 0.0 && beta == 0.0;
   if (precision < 0 || precision > 2) return -1;
   if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -2;
-  if (rk < 1 || rk > 8) return -3;
+  if (rk < (autob ? 0 : 1) || rk > 8) return -3;
   if (n < 1) return -4;
-  if (!(alpha > 0.0)) return -5;
-  if (!(beta > 0.0) || !(beta <= 1.0)) return -6;
+  if (!autob && !(alpha > 0.0)) return -5;
+  if (!autob && (!(beta > 0.0) || !(beta <= 1.0))) return -6;
+  if (autob && restart) return -6;
   if (lda < n) return -8;
   if (ldu < n) return -10;
   if (H && ldh < n) return -12;
@@ -113,32 +117,41 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
   zolo_info_t local;
   if (!info) info = &local;
   std::memset(info, 0, sizeof(*info));
-  info->r = rk;
-  info->alpha = alpha;
-  info->beta = beta;
+  const int rcap = rk > 0 ? rk : 8;  // shifts the workspace is sized for
 
   const int64_t nn = n * n;
-  // Workspace: scalars | X64 | Z64 (r) | [mixed: X32, Z32 (r), Xref64, polish] | step scratch
-  size_t step64 = std::max(qr_step_bytes<double>(n, rk), chol_step_bytes<double>(n, rk));
+  const int64_t nout = (n + 255) / 256;
+  // Workspace: scalars | X64 | Z64 (r) | [mixed: X32, Z32 (r), Xref64, polish] |
+  // [auto bounds: the QR of A/alpha] | step scratch
+  size_t step64 = std::max(qr_step_bytes<double>(n, rcap), chol_step_bytes<double>(n, rcap));
   if (restart) step64 = std::max(step64, bounds_bytes(n));
-  size_t bytes = kScalarBytes + sizeof(double) * (size_t)(1 + rk) * nn + step64 + 4096;
-  if (mixed) bytes += sizeof(float) * (size_t)(1 + rk) * nn + sizeof(double) * nn + polish_bytes(n) +
-                      chol_step_bytes<float>(n, rk);
+  if (autob) step64 = std::max(step64, dense_qr_bytes<double>(n) + rbound_bytes(n) + 4096);
+  size_t bytes = kScalarBytes + sizeof(double) * (size_t)(1 + rcap) * nn + step64 + 4096;
+  if (mixed) bytes += sizeof(float) * (size_t)(1 + rcap) * nn + sizeof(double) * nn + polish_bytes(n) +
+                      chol_step_bytes<float>(n, rcap);
+  if (autob) bytes += sizeof(double) * ((size_t)nn + (size_t)n + (size_t)nout * 256 * 256) + 3 * 256;
   char* base = nullptr;
   ZOLO_CHECK(workspace_acquire(bytes, &base));
   Arena ar(base, bytes);
   Scalars sc = take_scalars(ar);
   double* X = ar.take<double>(nn);
-  double* Z = ar.take<double>((size_t)rk * nn);
+  double* Z = ar.take<double>((size_t)rcap * nn);
   float* X32 = nullptr;
   float* Z32 = nullptr;
   double* Xref = nullptr;
   char* polish_ws = nullptr;
   if (mixed) {
     X32 = ar.take<float>(nn);
-    Z32 = ar.take<float>((size_t)rk * nn);
+    Z32 = ar.take<float>((size_t)rcap * nn);
     Xref = ar.take<double>(nn);
     polish_ws = ar.take<char>(polish_bytes(n));
+  }
+  QrPre<double> pre{nullptr, nullptr};
+  double* pre_tau = nullptr;
+  if (autob) {
+    pre.Xw = ar.take<double>(nn);
+    pre_tau = ar.take<double>(n);
+    pre.Tx = ar.take<double>((size_t)nout * 256 * 256);
   }
   const size_t mark = ar.used();
   ZOLO_ARENA_OK(ar);
@@ -148,6 +161,49 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
   c.low = (precision == 2) ? Prec::BF16X3 : Prec::F32;
   Timer tm(s);
   tm.mark(kPhOther);
+
+  if (autob) {
+    // alpha = ||A||_F; A/alpha = Q_X R; beta = 1/(||R^{-1}||_1 est * sqrt(n)).
+    // This is test_zolodrv1.f90:179-189's bound (rcond/sqrt(n) with ANORM =
+    // 1) with the 1-norm estimate taken on R instead of on the LU factors:
+    // sigma_min(A/alpha) = 1/||R^{-1}||_2 >= 1/(sqrt(n) ||R^{-1}||_1), so it
+    // is a lower bound by the same argument.  The factorisation is the one the
+    // first (QR) step of the iteration starts from, scaled by 1/beta.
+    tm.mark(kPhBounds);
+    ZOLO_CHECK(k_sumsq<double>(A, lda, n, n, sc.partial, kPartials, sc.out, s));
+    double a2 = 0.0;
+    ZOLO_CHECK(fetch_double(sc.out, &a2, s));
+    alpha = std::sqrt(a2);
+    if (!(alpha > 0.0) || !std::isfinite(alpha)) {
+      set_error("zolo_pd: zero or non-finite matrix");
+      return kErrArg;
+    }
+    ZOLO_CHECK(k_scale_copy<double, double>(A, lda, pre.Xw, n, n, n, 1.0 / alpha, s));
+    tm.mark(kPhQr);
+    ZOLO_CHECK(dense_geqrf<double>(c, pre.Xw, n, 0, n, 1, pre_tau, pre.Tx, ar));
+    ar.reset();
+    ar.take<char>(mark);
+    tm.mark(kPhBounds);
+    double est = 0.0;
+    ZOLO_CHECK(rbound_run(n, pre.Xw, n, &est, ar, s));
+    ar.reset();
+    ar.take<char>(mark);
+    if (!(est > 0.0) || !std::isfinite(est)) {
+      set_error("zolo_pd: matrix is numerically singular (no positive lower bound from its R factor)");
+      return kErrArg;
+    }
+    beta = std::min(1.0, 1.0 / (est * std::sqrt((double)n)));
+    if (!(beta > 0.0)) {
+      set_error("zolo_pd: lower bound underflows");
+      return kErrArg;
+    }
+    if (rk == 0) rk = choosem(1.0 / beta);
+    ZOLO_CHECK(k_scale_upper(pre.Xw, n, n, 1.0 / beta, s));
+    tm.mark(kPhOther);
+  }
+  info->r = rk;
+  info->alpha = alpha;
+  info->beta = beta;
 
   ZOLO_HIP(hipMemsetAsync(sc.info, 0, 8 * sizeof(int), s));
   // X0 = A / (alpha*beta), CON = 1/beta  (pzolopd1.f90:286-291)
@@ -205,7 +261,9 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
       ZOLO_HIP(hipMemcpyAsync(sc.shift, hs, rk * sizeof(double), hipMemcpyHostToDevice, s));
       ZOLO_HIP(hipMemcpyAsync(sc.alpha, ha, rk * sizeof(double), hipMemcpyHostToDevice, s));
       tm.mark(kPhQr);
-      ZOLO_CHECK(qr_step<double>(c, X, n, n, rk, sc.shift, sc.alpha, Z, n, nn, ar, &nz));
+      // the first step may start from the factorisation of the bounds
+      const QrPre<double>* pq = (autob && it == 0) ? &pre : nullptr;
+      ZOLO_CHECK(qr_step<double>(c, X, n, n, rk, sc.shift, sc.alpha, Z, n, nn, ar, &nz, pq));
       if (it < 16) info->schedule[it] = 'Q';
     } else {
       for (int q = 0; q < rs; ++q) {

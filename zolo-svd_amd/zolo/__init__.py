@@ -142,6 +142,8 @@ _sig("zolo_batched_tile", C.c_int, [_I64, C.c_int])
 _sig("zolo_syevd_device", C.c_int, [_I64, _P, _I64, _P, _P, _I64, _P, _P])
 _sig("zolo_svd_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _I64, _P, _I64, C.POINTER(_Info), _P,
                                   _P])
+_sig("zolo_pd_auto_device", C.c_int, [C.c_int, C.c_char, C.c_int, _I64, _P, _I64, _P, _I64, _P, _I64,
+                                      C.POINTER(_Info), _P])
 _sig("zolo_subit_device", C.c_int, [_I64, _P, _I64, C.c_double, _P, _I64, C.POINTER(C.c_int), _D,
                                     C.POINTER(C.c_int), _P])
 _sig("zolo_eig_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P, _I64, C.POINTER(C.c_int), _D, C.POINTER(C.c_int),
@@ -344,6 +346,26 @@ def pzolopd1_device(dA, alpha: float, beta: float, rk: int, sym: str = "N", pola
                              C.byref(ci), _stream())
     _check("zolo_pd_device", st)
     return dU, (dH if polar else None), RunInfo.from_c(ci)
+
+
+def pzolopd_auto_device(dA, precision: str = "f64", rk: int = 0, sym: str = "N", polar: bool = True, dU=None,
+                        dH=None):
+    """Zolo-PD with alpha, beta from the QR of A/alpha that the first QR step reuses
+    (zolo_pd_auto_device).  Returns (dU, dH, RunInfo); RunInfo has alpha, beta, r."""
+    torch = _torch()
+    n = dA.shape[0]
+    if dU is None:
+        dU = torch.empty_like(dA)
+    if polar and dH is None:
+        dH = torch.empty_like(dA)
+    if not polar:
+        dH = None
+    ci = _Info()
+    st = _lib.zolo_pd_auto_device(PREC[precision], sym.encode()[:1], rk, n, C.c_void_p(dA.data_ptr()), n,
+                                  C.c_void_p(dU.data_ptr()), n, C.c_void_p(dH.data_ptr()) if polar else None, n,
+                                  C.byref(ci), _stream())
+    _check("zolo_pd_auto_device", st)
+    return dU, dH, RunInfo.from_c(ci)
 
 
 def pqdwhfacs_device(dA, sigma: float = 0.0, polar: bool = True, sym: str = "N"):
