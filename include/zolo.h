@@ -192,9 +192,10 @@ int zolo_batched_tile(int64_t n, int precision);
  * divide and conquer on the tridiagonal (Gu-Eisenstat eigenvectors, MFMA
  * eigenvector updates), blocked back-transformation.  phase_ms (3 values,
  * may be NULL): tridiagonalisation, divide and conquer, back-transformation.
- * zolo_svd_device is the flow of ZoloCHK1.f90:80-91 on one GPU: bounds
- * (test_zolodrv1.f90:179-189), polar factor U_p by Zolo-PD (precision as
- * zolo_pd_device; rk = 0 picks choosem), H = (U_p^T A + A^T U_p)/2,
+ * zolo_svd_device is the flow of ZoloCHK1.f90:80-91 on one GPU: polar factor
+ * U_p by Zolo-PD with its bounds from the QR its first step reuses (as
+ * zolo_pd_auto_device; precision as zolo_pd_device; rk = 0 picks choosem;
+ * info->alpha/beta report them), H = (U_p^T A + A^T U_p)/2,
  * H = V diag(S) V^T (S ascending), U = U_p V; A = U diag(S) V^T.
  * phase_ms (5 values, may be NULL): polar + H, tridiagonalisation, divide and
  * conquer, back-transformation, eigensolve + U. */
@@ -215,7 +216,7 @@ int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t l
  * eigenvalues below sigma, ascending, and V (n x Rk) their eigenvectors.
  * zolo_eig_below_device is the driver pqdwheig.f90:113-144 with the Zolo
  * polar factor: the polar factor of A - sigma I (precision and rk as
- * zolo_pd_device, sym = 'S'), then zolo_eig_device; A is not modified,
+ * zolo_pd_auto_device, sym = 'S'), then zolo_eig_device; A is not modified,
  * *count = Rk, info (may be NULL) describes the polar decomposition. */
 int zolo_subit_device(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err,
                       int* iters, void* stream);

@@ -573,18 +573,10 @@ int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int
   ZOLO_HIP(hipMemcpy2DAsync(B.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
                             hipMemcpyDeviceToDevice, s));
   if (sigma != 0.0) ZOLO_CHECK(k_add_diag<double>(B.p, n, n, -sigma, s));
-  double alpha = 0.0, beta = 0.0;
-  {
-    const size_t bytes = bounds_bytes(n);
-    char* base = nullptr;
-    ZOLO_CHECK(workspace_acquire(bytes, &base));
-    Arena ar(base, bytes);
-    ZOLO_CHECK(bounds_run(0, n, B.p, n, &alpha, &beta, ar, s));
-  }
-  const int r = rk > 0 ? rk : choosem(1.0 / beta);
   zolo_info_t local{};
-  // polar factor of the symmetric A - sigma I into Q
-  const int st = zolo_pd_run(precision, 'S', r, n, alpha, beta, B.p, n, Q.p, n, nullptr, n, info ? info : &local, s);
+  // polar factor of the symmetric A - sigma I into Q, its bounds from the QR
+  // the first step reuses (rk = 0: choosem)
+  const int st = zolo_pd_run(precision, 'S', rk, n, 0.0, 0.0, B.p, n, Q.p, n, nullptr, n, info ? info : &local, s);
   if (st) return st;
   // PQDWHSUBIT, PDSYEVD on the Rk block, V1 VV1 (pqdwheig.f90:119-144 = ZoloEig.f90:80-103)
   return zoloeig_run(n, Q.p, n, As.p, n, w, V, ldv, count, err, iters, s);
@@ -732,21 +724,13 @@ int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t l
   hipStream_t s = (hipStream_t)stream;
   using clk = std::chrono::steady_clock;
   const auto t0 = clk::now();
-  double alpha = 0.0, beta = 0.0;
-  {
-    const size_t bytes = bounds_bytes(n);
-    char* base = nullptr;
-    ZOLO_CHECK(workspace_acquire(bytes, &base));
-    Arena ar(base, bytes);
-    ZOLO_CHECK(bounds_run(0, n, A, lda, &alpha, &beta, ar, s));
-  }
-  const int r = rk > 0 ? rk : choosem(1.0 / beta);
   DevMat Ht;  // H, then U_p V
   ZOLO_CHECK(Ht.alloc(n));
   zolo_info_t local{};
   zolo_info_t* inf = info ? info : &local;
-  // polar factor U_p into U, H = (U_p^T A + A^T U_p)/2 into Ht (ZoloCHK1.f90:80-83)
-  const int st = zolo_pd_run(precision, 'N', r, n, alpha, beta, A, lda, U, ldu, Ht.p, n, inf, s);
+  // polar factor U_p into U, H = (U_p^T A + A^T U_p)/2 into Ht (ZoloCHK1.f90:80-83);
+  // alpha, beta from the QR its first step reuses (rk = 0: choosem)
+  const int st = zolo_pd_run(precision, 'N', rk, n, 0.0, 0.0, A, lda, U, ldu, Ht.p, n, inf, s);
   if (st) return st;
   ZOLO_HIP(hipStreamSynchronize(s));
   const auto t1 = clk::now();
