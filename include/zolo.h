@@ -82,7 +82,10 @@ int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, dou
  * 1/(||R^{-1}||_1 estimate * sqrt(n)) with Higham's estimator (DTRCON's);
  * sigma_min(A/alpha) = 1/||R^{-1}||_2 >= beta as for the LU estimate.  The
  * first QR step then starts from that factorisation.  rk = 0 picks
- * choosem(1/beta); info->alpha, info->beta, info->r report the choice.
+ * choosem(1/beta); info->alpha, info->beta, info->r report the choice.  For
+ * a well-conditioned A whose first step is a Cholesky step (largest shift
+ * coefficient <= 300, pzolopd1.f90:405) the QR is not reused, and
+ * zolo_bounds_device + zolo_pd_device is the cheaper route.
  * Errors: -6 A NULL, -8 U NULL, the rest as zolo_pd_device; a numerically
  * singular A (no positive bound) is kErrArg. */
 int zolo_pd_auto_device(int precision, char sym, int rk, int64_t n, const double* A, int64_t lda, double* U,
