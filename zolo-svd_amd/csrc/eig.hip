@@ -44,21 +44,49 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n
 // Column i (panel-local j): A[i:, i] -= A[i:, k0:i] W[i, 0:j]^T + W[i:, 0:j] A[i, k0:i]^T
 // (the earlier reflectors of the panel), partial sums of squares of
 // A[i+2:, i] for the reflector, and sc[0] = the updated A[i+1][i].
+// fix: column j - 1 of W still lacks dlatrd's last correction W[:, j-1] +=
+// alpha v (alpha = -tau[i-1]/2 w^T v from wpart, v = reflector i - 1); it is
+// applied here to rows > i, which this launch owns, and to the row-i values
+// it reads (row i of W is read by nothing else: the later columns read rows
+// > i, the rank-2k update rows >= k0 + the panel width).
 __global__ __launch_bounds__(kT) void trd_col_kernel(double* A, int64_t ld, int64_t n, int64_t k0, int64_t i,
-                                                     const double* W, double* part, double* sc) {
+                                                     double* W, double* part, double* sc, const double* v,
+                                                     const double* wpart, int nwp, const double* tau, int fix) {
   __shared__ double arow[kTrdNb], wrow[kTrdNb];
   __shared__ double sh[4];
+  __shared__ double s_alpha;
   const int j = (int)(i - k0);
   const int t = threadIdx.x;
+  if (t == 0) {
+    double a = 0.0;
+    if (fix) {
+      double s = 0.0;
+      for (int k = 0; k < nwp; ++k) s += wpart[k];
+      a = -0.5 * tau[i - 1] * s;
+    }
+    s_alpha = a;
+  }
+  __syncthreads();
+  const double alpha = s_alpha;
   if (t < j) {
     arow[t] = A[i + (k0 + t) * ld];
-    wrow[t] = W[i + (int64_t)t * n];
+    double wv = W[i + (int64_t)t * n];
+    if (fix && t == j - 1) wv += alpha * v[i];
+    wrow[t] = wv;
   }
   __syncthreads();
   double ss = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * kT + t + i; r < n; r += (int64_t)gridDim.x * kT) {
     double a = A[r + i * ld];
-    for (int c = 0; c < j; ++c) a -= A[r + (k0 + c) * ld] * wrow[c] + W[r + (int64_t)c * n] * arow[c];
+    double wlast = 0.0;  // W[r][j-1] after the correction
+    if (fix) {
+      wlast = W[r + (int64_t)(j - 1) * n] + alpha * v[r];
+      if (r > i) W[r + (int64_t)(j - 1) * n] = wlast;
+    }
+    for (int c = 0; c < j; ++c) {
+      const double wrc = (fix && c == j - 1) ? wlast : W[r + (int64_t)c * n];
+      a -= A[r + (k0 + c) * ld] * wrow[c] + wrc * arow[c];
+    }
     A[r + i * ld] = a;
     if (r >= i + 2) ss += a * a;
     if (r == i + 1) sc[0] = a;
@@ -603,10 +631,14 @@ int ke_fill(double* A, int64_t ld, int64_t m, int64_t n, double val, hipStream_t
 }
 
 int ke_trd_column(double* A, int64_t ld, int64_t n, int64_t k0, int64_t i, double* W, double* v, double* tau, double* e,
-                  TrdScratch& w, hipStream_t s) {
+                  TrdScratch& w, bool last_in_panel, hipStream_t s) {
   const int64_t m = n - i;
   const int nwg = (int)std::min<int64_t>(64, std::max<int64_t>(1, (m + kT - 1) / kT));
-  hipLaunchKernelGGL(trd_col_kernel, dim3(nwg), dim3(kT), 0, s, A, ld, n, k0, i, W, w.part, w.sc);
+  // the previous column of the panel left its W correction to this launch
+  const int fix = (i > k0) ? 1 : 0;
+  const int nwg_prev = (int)std::min<int64_t>(64, std::max<int64_t>(1, (m + 1 + kT - 1) / kT));
+  hipLaunchKernelGGL(trd_col_kernel, dim3(nwg), dim3(kT), 0, s, A, ld, n, k0, i, W, w.part, w.sc, (const double*)v,
+                     (const double*)w.wpart, nwg_prev, (const double*)tau, fix);
   if (i + 1 >= n) {
     ZOLO_HIP(hipGetLastError());
     return kOk;
@@ -618,7 +650,10 @@ int ke_trd_column(double* A, int64_t ld, int64_t n, int64_t k0, int64_t i, doubl
   hipLaunchKernelGGL(trd_symv_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(kT), 0, s, A, ld, n, i, v, w.ypart);
   hipLaunchKernelGGL(trd_w_kernel, dim3(nwg), dim3(kT), 0, s, A, ld, n, k0, i, W, v, w.ypart, (int)T, w.dpart, nwg, tau,
                      w.wpart);
-  hipLaunchKernelGGL(trd_wfix_kernel, dim3(nwg), dim3(kT), 0, s, W, n, (int)(i - k0), v, w.wpart, nwg, tau, i);
+  // the last column of a panel corrects its W column itself (the rank-2k
+  // update follows); the others leave it to the next column's trd_col
+  if (last_in_panel)
+    hipLaunchKernelGGL(trd_wfix_kernel, dim3(nwg), dim3(kT), 0, s, W, n, (int)(i - k0), v, w.wpart, nwg, tau, i);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

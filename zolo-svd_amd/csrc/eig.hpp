@@ -31,8 +31,10 @@ struct DcLevel {
 };
 
 int ke_fill(double* A, int64_t ld, int64_t m, int64_t n, double val, hipStream_t s);
+// last_in_panel: the column applies its own final W correction; otherwise the
+// next column's launch does (the columns of one panel run in order)
 int ke_trd_column(double* A, int64_t ld, int64_t n, int64_t k0, int64_t i, double* W, double* v, double* tau, double* e,
-                  TrdScratch& w, hipStream_t s);
+                  TrdScratch& w, bool last_in_panel, hipStream_t s);
 int ke_trd_panel_end(double* A, int64_t ld, int64_t k0, int jb, int64_t n, double* d, hipStream_t s);
 int ke_trd_extract_v(const double* A, int64_t ld, int64_t k0, int64_t r0, int jb, int64_t mrows, double* V,
                      int64_t ldv, hipStream_t s);

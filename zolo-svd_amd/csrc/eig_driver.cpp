@@ -99,7 +99,8 @@ int syevd_run(int64_t n, double* H, int64_t ldh, double* lambda, double* Z, int6
   ZOLO_CHECK(ke_fill(e, n, n, 1, 0.0, s));
   for (int64_t k0 = 0; k0 < n; k0 += kTrdNb) {
     const int jb = (int)std::min<int64_t>(kTrdNb, n - k0);
-    for (int64_t i = k0; i < k0 + jb; ++i) ZOLO_CHECK(ke_trd_column(H, ldh, n, k0, i, W, v, tau, e, tw, s));
+    for (int64_t i = k0; i < k0 + jb; ++i)
+      ZOLO_CHECK(ke_trd_column(H, ldh, n, k0, i, W, v, tau, e, tw, i + 1 == k0 + jb, s));
     ZOLO_CHECK(ke_trd_panel_end(H, ldh, k0, jb, n, d, s));
     const int64_t r = k0 + jb, n2 = n - r;
     if (n2 <= 0) continue;
