@@ -55,3 +55,35 @@ def test_fortran_spectral_division_driver_on_gpu(tmp_path):
     print(r.stderr)
     assert r.returncode == 0
     assert "PASS" in r.stdout.splitlines()[-1]
+
+
+def test_fortran_spectral_division_info_codes(zl):
+    """PQDWHSUBIT and ZoloEig check their arguments before any GPU work."""
+    lib = ctypes.CDLL(os.path.join(PKG, "libzolo_f.so"))
+    I = ctypes.c_int
+    D = ctypes.c_double
+    buf = (D * 64)()
+    desc = (I * 9)(1, 0, 4, 4, 4, 4, 0, 0, 4)
+    bad = (I * 9)(1, 0, 4, 4, 4, 4, 1, 0, 4)   # RSRC /= 0
+    info, rk = I(99), I(99)
+
+    def subit(n, da, dq, norma):
+        lib.pqdwhsubit_(ctypes.byref(I(n)), buf, ctypes.byref(I(1)), ctypes.byref(I(1)), da, ctypes.byref(D(norma)),
+                        buf, ctypes.byref(I(1)), ctypes.byref(I(1)), dq, buf, ctypes.byref(I(16)), buf, buf,
+                        ctypes.byref(I(16)), ctypes.byref(rk), ctypes.byref(info))
+        return info.value
+
+    assert subit(-1, desc, desc, 1.0) == -1
+    assert subit(4, bad, desc, 1.0) == -507
+    assert subit(4, desc, bad, 1.0) == -1007
+    assert subit(4, desc, desc, 0.0) == -6
+    assert subit(0, desc, desc, 1.0) == 0 and rk.value == 0
+
+    def eig(n, lwork):
+        lib.zoloeig_(ctypes.byref(I(n)), ctypes.byref(I(4)), buf, buf, ctypes.byref(I(0)), ctypes.byref(I(0)), buf,
+                     ctypes.byref(I(lwork)), buf, ctypes.byref(I(16)), buf, ctypes.byref(info))
+        return info.value
+
+    assert eig(-2, 64) == -1
+    assert eig(4, 15) == -8
+    assert eig(0, 0) == 0
