@@ -107,40 +107,88 @@ def errors(torch, A, dU, dH, Ug, Vg):
     return {"backward": bwd, "orth_sqrt_n": orth, "orth_n": orth / math.sqrt(n), "dist_UVt": dist}
 
 
-def flops_schedule_lapack(n, r, schedule):
-    """FP64 flops of the reference's dense-LAPACK route for an executed schedule."""
-    f = (2.0 / 3.0) * n ** 3 + 2.0 * n ** 3           # bounds LU + H
-    for step in schedule:
-        if step == "Q":
-            f += r * (10.0 / 3 + 10.0 / 3 + 2.0) * n ** 3   # dgeqrf + dorgqr on 2n x n + Q1 Q2^T
-        else:
-            f += r * (2.0 + 1.0 / 3 + 2.0) * n ** 3          # Gram + dpotrf + dpotrs (n rhs)
-    return f
-
-
-def cpu_baseline(n, schedule, r):
-    """Time one shifted Cholesky solve (Gram, POTRF, POTRS) of the oracle on host cores."""
-    import numpy as np
-    import scipy.linalg as sla
-    from oracle import zolo_oracle as zo
+def _blas_threads():
     try:
         from threadpoolctl import threadpool_info
-        cores = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+        return max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
     except Exception:  # pragma: no cover
-        cores = os.cpu_count() or 1
-    ns = n
-    rng = np.random.default_rng(0)
-    X = rng.standard_normal((ns, ns))
+        return os.cpu_count() or 1
+
+
+def _cpu_step_times(zo, n, r, seed):
+    """Measured seconds of one of each piece of the reference schedule at order n on the
+    host (numpy/scipy FP64 LAPACK, all BLAS threads): the alpha/beta bounds (LU + 1-norm
+    estimate, test_zolodrv1.f90:179-189), one shifted QR solve (LAPACK dgeqrf + dorgqr on the
+    2n x n stack and the Q1 Q2^T product, zoloqr.f90:99-112), one shifted Cholesky solve
+    (Gram, dpotrf, dpotrs, zolochol.f90:87-113) and H = (U^T A + A^T U)/2."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, n))
     coeff = zo.compute_coeff(1.5, zo.DLAMCH_P, r)
+    qcoeff = zo.compute_coeff(1e13, zo.DLAMCH_P, r)
+    t = {}
+    t0 = time.perf_counter()
+    zo.bounds_estimate(X)
+    t["bounds"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    zo.zoloqr_step_lapack(X, qcoeff, r, 1)
+    t["qr"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     zo.zolochol_step(X, coeff, r, 1)
-    dt = time.perf_counter() - t0
-    rate = (2.0 + 1.0 / 3 + 2.0) * ns ** 3 / dt
-    est = flops_schedule_lapack(n, r, schedule) / rate
-    return {"value": est, "unit": "s", "cores": cores, "kind": "port",
-            "sample": f"one shifted Cholesky solve (Gram + POTRF + POTRS, n={ns}, numpy/scipy FP64) timed "
-                      f"{dt:.2f} s = {rate / 1e9:.0f} GFLOP/s, extrapolated to the executed schedule "
-                      f"{schedule} with r={r} using dense LAPACK flop counts"}
+    t["chol"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    zo.polar_h(X, X)
+    t["h"] = time.perf_counter() - t0
+    return t
+
+
+def _cpu_model(t, r, schedule):
+    """Seconds of a whole CPU polar decomposition from measured step times: bounds, r
+    shifted solves per step of the executed schedule, H."""
+    return t["bounds"] + sum(r * (t["qr"] if x == "Q" else t["chol"]) for x in schedule) + t["h"]
+
+
+def cpu_baseline(n, schedule, r, validate_n=0, seed=20260103):
+    """CPU baseline on the host cores, measured.
+
+    At the headline order n every piece of the reference schedule is timed once
+    (bounds, one shifted QR solve, one shifted Cholesky solve, H) and the whole
+    decomposition is their sum over the executed schedule (r shifts per step,
+    as the reference's shift groups run them one after another on one process).
+    At validate_n the same sum is compared with one whole CPU polar
+    decomposition of the headline spectrum timed end to end (oracle pzolopd1
+    with LAPACK QR steps, bounds and H included)."""
+    from oracle import zolo_oracle as zo
+    cores = _blas_threads()
+    t = _cpu_step_times(zo, n, r, seed)
+    est = _cpu_model(t, r, schedule)
+    out = {"value": est, "unit": "s", "cores": cores, "kind": "port",
+           "steps_s": {k: round(v, 3) for k, v in t.items()}}
+    sample = (f"numpy/scipy FP64 LAPACK, {cores} BLAS threads; at n={n} one of each step measured "
+              f"(bounds {t['bounds']:.2f} s, one shifted QR solve {t['qr']:.2f} s, one shifted Cholesky solve "
+              f"{t['chol']:.2f} s, H {t['h']:.2f} s) and summed over the schedule {schedule} with r={r}")
+    if validate_n:
+        m = validate_n
+        g = np.random.default_rng(seed)
+        U0, V0 = zo.haar(m, g), zo.haar(m, g)
+        k = m - m // 20
+        sv = np.full(m, 1e-8)
+        sv[:k] = np.geomspace(1.0, 1e-8, k)
+        A = (U0 * sv) @ V0.T
+        tv = _cpu_step_times(zo, m, r, seed)
+        t0 = time.perf_counter()
+        al, be = zo.bounds_estimate(A)
+        rk = zo.choosem(1.0 / be)
+        Q, lg = zo.pzolopd1(A, al, be, rk, lapack_qr=True)
+        zo.polar_h(Q, A)
+        meas = time.perf_counter() - t0
+        sched = "".join("Q" if x == "QR" else "C" for x in lg.schedule)
+        model = _cpu_model(tv, rk, sched)
+        out["validation"] = {"n": m, "schedule": sched, "r": rk, "measured_s": round(meas, 3),
+                             "model_s": round(model, 3), "model_over_measured": round(model / meas, 3)}
+        sample += (f"; check at n={m}: a whole CPU polar decomposition ({sched}, r={rk}) took {meas:.2f} s "
+                   f"end to end, the step-sum model {model:.2f} s")
+    out["sample"] = sample
+    return out
 
 
 def gemm_classes(zl):
@@ -207,7 +255,14 @@ def small_pds(torch, zl, count, seed):
         _, info = zl.pd_batched_device(mats, l0=src)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        res[src] = {"pds_per_sec": count / dt, "s": dt, "orth_max": float(info.orth.max()),
+        # roofline of the whole call: the algorithmic flops of the executed
+        # schedules (FP32 steps, FP64 bounds and first QR steps) against the time
+        # the MFMA peaks would need for them
+        ideal = info.flops_f32 / (PEAK_TFLOPS[1] * 1e12) + info.flops_f64 / (PEAK_TFLOPS[0] * 1e12)
+        rl = {"bound": "mfma", "flops_f32": info.flops_f32, "flops_f64": info.flops_f64,
+              "achieved": (info.flops_f32 + info.flops_f64) / dt / 1e12, "unit": "TFLOP/s",
+              "peak": "FP32 157.3 / FP64 78.6", "frac": ideal / dt}
+        res[src] = {"pds_per_sec": count / dt, "s": dt, "orth_max": float(info.orth.max()), "roofline": rl,
                     "failed": int(info.failed.sum()), "fallback_rate": float(info.fallback.mean()),
                     "iters_mean": float(info.iters.mean()), "iters_hist": np.bincount(info.iters).tolist(),
                     "graph_nodes": info.graph_nodes, "phase_ms": {k: round(v, 1) for k, v in info.phase_ms.items()}}
@@ -242,7 +297,10 @@ def svd_tail(torch, zl, n, seed):
     eye = torch.eye(n, dtype=torch.float64, device="cuda")
     bwd = (torch.linalg.norm(A - (U * S) @ V.T) / torch.linalg.norm(A)).item()
     sig_err = (torch.max(torch.abs(S - torch.sort(s).values))).item()
-    out = {"n": n, "s": dt, "schedule": info.schedule, "backward": bwd, "sigma_max_abs_err": sig_err,
+    # the reference's own residual ||A V - U diag(S)||_F / max(S) (check_routines.f90:70-86)
+    resid = (torch.linalg.norm(A @ V - U * S) / S.max()).item()
+    out = {"n": n, "s": dt, "schedule": info.schedule, "backward": bwd, "residual_ref": resid,
+           "sigma_max_abs_err": sig_err,
            "orth_U": (torch.linalg.norm(U.T @ U - eye) / math.sqrt(n)).item(),
            "orth_V": (torch.linalg.norm(V.T @ V - eye) / math.sqrt(n)).item(),
            "phase_ms": {k: round(v, 1) for k, v in ph.items()}}
@@ -258,6 +316,7 @@ def svd_tail(torch, zl, n, seed):
     U, V = dU.T, dV.T
     out["mixed"] = {"s": dt, "schedule": info.schedule,
                     "backward": (torch.linalg.norm(A - (U * S) @ V.T) / torch.linalg.norm(A)).item(),
+                    "residual_ref": (torch.linalg.norm(A @ V - U * S) / S.max()).item(),
                     "sigma_max_abs_err": (torch.max(torch.abs(S - torch.sort(s).values))).item(),
                     "orth_U": (torch.linalg.norm(U.T @ U - eye) / math.sqrt(n)).item(),
                     "orth_V": (torch.linalg.norm(V.T @ V - eye) / math.sqrt(n)).item(),
@@ -320,6 +379,9 @@ def main():
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f32", "f64"])
     ap.add_argument("--seed", type=int, default=20260103)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-validate", type=int, default=0,
+                    help="order of a whole CPU polar decomposition timed end to end to check the CPU step-sum "
+                         "model (0: skip; 4096 takes about 80 s on 16 threads)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 / FP32 comparison runs")
     ap.add_argument("--small-count", type=int, default=4096, help="matrices of the batched small-PD workload")
     ap.add_argument("--no-small", action="store_true", help="skip the batched small-PD workload")
@@ -392,7 +454,8 @@ def main():
 
     compare = {}
     if not args.no_compare:
-        for prec in ("f64", "f32"):
+        extra = ("f32", f"{args.precision}_refsched") if args.precision != "f64" else ("f32",)
+        for prec in ("f64",) + extra:
             if prec == args.precision:
                 continue
             torch.cuda.synchronize()
@@ -410,7 +473,7 @@ def main():
         # mixed run without it, the QR step followed by two Cholesky steps
         ref_sched = compare["f64"]["schedule"] if "f64" in compare else (
             info.schedule if args.precision == "f64" else "QCC")
-        cpu = cpu_baseline(n, ref_sched.upper(), info.r)
+        cpu = cpu_baseline(n, ref_sched.upper(), info.r, args.cpu_validate, args.seed)
         log(f"cpu baseline {cpu['value']:.1f} s ({cpu['sample']})")
 
     small = None

@@ -31,6 +31,14 @@ extern "C" {
 #define ZOLO_PREC_F64 0     /* FP64 throughout (the reference's arithmetic)               */
 #define ZOLO_PREC_F32 1     /* FP32 Cholesky iterations, FP64 QR step + polish            */
 #define ZOLO_PREC_BF16X3 2  /* BF16x3 MFMA Cholesky iterations, FP64 QR step + polish     */
+/* The low-precision Cholesky steps above take the fewest shifts whose
+ * Zolotarev bound reaches FP32 working accuracy in that step and stop once
+ * CON - 1 is below the FP32 unit roundoff.  These two keep the reference's
+ * schedule instead (all r shifts every step, the loop ends on the ERR test of
+ * pzolopd1.f90:397), then polish in FP64 the same way; they separate what the
+ * precision buys from what the shorter schedule buys. */
+#define ZOLO_PREC_F32_REFSCHED 3
+#define ZOLO_PREC_BF16X3_REFSCHED 4
 
 /* Per-run record: iteration trajectory and phase times. */
 typedef struct zolo_info {
@@ -72,7 +80,8 @@ void zolo_mellipj(double u, double alpha, double tol, double* sn, double* cn, do
  * U = polar factor of A (n x n), H = (U^T A + A^T U)/2 if H != NULL.
  * sym = 'S' or 'N' (zolochol.f90:99), rk = number of shifts (1..8),
  * alpha, beta as in pzolopd1: X0 = A/(alpha*beta), CON = 1/beta.
- * Device pointers; A is read-only. */
+ * Device pointers; A is read-only.  Every argument is checked before any
+ * device work: -i for argument i (A NULL -7, lda < n -8, U NULL -9, ...). */
 int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
                    int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream);
 /* zolo_pd_device with the bounds computed inside (no counterpart in the
@@ -86,8 +95,10 @@ int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, dou
  * a well-conditioned A whose first step is a Cholesky step (largest shift
  * coefficient <= 300, pzolopd1.f90:405) the QR is not reused, and
  * zolo_bounds_device + zolo_pd_device is the cheaper route.
- * Errors: -6 A NULL, -8 U NULL, the rest as zolo_pd_device; a numerically
- * singular A (no positive bound) is kErrArg. */
+ * Errors by this function's own argument positions (precision -1, sym -2,
+ * rk -3, n -4, A NULL -5, lda -6, U NULL -7, ldu -8, ldh -10), all checked
+ * before any device work; a numerically singular A (no positive bound) is
+ * kErrArg. */
 int zolo_pd_auto_device(int precision, char sym, int rk, int64_t n, const double* A, int64_t lda, double* U,
                         int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream);
 /* Same with host arrays (copied to and from the device). */
@@ -180,6 +191,11 @@ typedef struct zolo_batch_info {
   double time_ms;        /* wall time of the call                                 */
   double phase_ms[8];    /* 0 alpha + beta, 1 plan, 2 tile choice, 3 iterations,
                             4 fallback and restarts, 5 learned features + network */
+  double flops_f32;      /* algorithmic flops of the executed schedules at each
+                            matrix's own order n: per Cholesky step (7/3) n^3 per
+                            shift + n^3, per FP32 QR step 5 n^3 per shift        */
+  double flops_f64;      /* the same for the FP64 work: LU bounds (2/3) n^3 and
+                            first QR steps at CON > 1e8 (5 n^3 per shift)        */
 } zolo_batch_info_t;
 int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* const* U,
                            const int64_t* ldu, int precision, int l0_source, const double* beta_in, double orth_tol,

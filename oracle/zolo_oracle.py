@@ -20,7 +20,9 @@ How each function is pinned (checked by ``tests/test_oracle.py``):
 * ``structured_geqrf`` / ``structured_orgqr`` -- against LAPACK dgeqrf/dorgqr
   (numpy) on the full stacked matrix, and the invariants Q^T Q = I, QR = S.
 * ``zoloqr_step`` / ``zolochol_step`` -- against X (X^T X + c I)^{-1} computed
-  by a direct LAPACK solve, and against each other.
+  by a direct LAPACK solve, and against each other; ``zoloqr_step_lapack``
+  (and ``pzolopd1(lapack_qr=True)``) against ``zoloqr_step`` (same schedule,
+  same polar factor).
 * ``pzolopd1`` / ``pqdwhfacs`` -- against the closed form U V^T of the
   generator A = U diag(s) V^T and against scipy.linalg.polar.
 * ``pzolopd2`` -- the same closed form, on a numerically singular matrix
@@ -261,6 +263,25 @@ def zoloqr_step(X: np.ndarray, coeff: np.ndarray, rk: int, ip: int) -> np.ndarra
     return akk * (Q[:n] @ Q[n:].T)
 
 
+def zoloqr_step_lapack(X: np.ndarray, coeff: np.ndarray, rk: int, ip: int) -> np.ndarray:
+    """zoloqr_step with LAPACK's blocked dgeqrf + dorgqr on the whole stack.
+
+    The structure of [X; sqrt(c) I] is only exploited by the column loop of
+    structured_geqrf; LAPACK's QR of the same stack gives the same Q up to
+    column signs, which cancel in Q1 Q2^T.  This is the form the CPU baseline
+    times (level-3 BLAS, all host cores).
+    """
+    n = X.shape[0]
+    c = coeff[2 * ip - 2]
+    S = np.empty((2 * n, n))
+    S[:n] = X
+    S[n:] = 0.0
+    S[n:][np.diag_indices(n)] = math.sqrt(c)
+    Q = sla.qr(S, mode="economic", overwrite_a=True, check_finite=False)[0]
+    akk = -coeff[2 * rk + ip - 1] / math.sqrt(c)
+    return akk * (Q[:n] @ Q[n:].T)
+
+
 def zolochol_step(X: np.ndarray, coeff: np.ndarray, rk: int, ip: int, sym: str = "N") -> np.ndarray:
     """Z_ip = -a_ip X (X^T X + c_{2ip-1} I)^{-1} (zolochol.f90:87-113).
 
@@ -279,8 +300,11 @@ def zolochol_step(X: np.ndarray, coeff: np.ndarray, rk: int, ip: int, sym: str =
 
 
 def formx2(X: np.ndarray, Z: np.ndarray, fone: float) -> tuple[np.ndarray, float]:
-    """X+ = (X + Z)/FONE, ERR = ||X+ - X||_F (FormX2.f90:73-89)."""
-    Xp = (X + Z) / fone
+    """X+ = (X + Z)/FONE, ERR = ||X+ - X||_F (FormX2.f90:73-89).
+
+    PDLASCL('General', FONE, ONE) at FormX2.f90:74 scales by MUL = CTO/CFROM
+    = 1/FONE (xLASCL's single-step case), so X+ is (X + Z) * (1/FONE)."""
+    Xp = (X + Z) * (1.0 / fone)
     return Xp, float(np.linalg.norm(Xp - X))
 
 
@@ -429,11 +453,13 @@ class ZoloLog:
 
 
 def pzolopd1(A: np.ndarray, alpha: float, beta: float, rk: int, sym: str = "N",
-             max_it: int = 6) -> tuple[np.ndarray, ZoloLog]:
+             max_it: int = 6, lapack_qr: bool = False) -> tuple[np.ndarray, ZoloLog]:
     """Zolo-PD (pzolopd1.f90:280-487) on one process, r shifts done in sequence.
 
-    Returns the polar factor U_p and the iteration log.
+    Returns the polar factor U_p and the iteration log.  lapack_qr = True
+    takes the QR steps with zoloqr_step_lapack (the timed CPU baseline).
     """
+    qr_step = zoloqr_step_lapack if lapack_qr else zoloqr_step
     n = A.shape[0]
     X = A / (alpha * beta)
     con = 1.0 / beta
@@ -451,7 +477,7 @@ def pzolopd1(A: np.ndarray, alpha: float, beta: float, rk: int, sym: str = "N",
         if (it == 0 or con > 1e3) and maxc > 3e2:
             log.schedule.append("QR")
             for ip in range(1, rk + 1):
-                Z += zoloqr_step(X, coeff, rk, ip)
+                Z += qr_step(X, coeff, rk, ip)
         else:
             log.schedule.append("CHOL")
             for ip in range(1, rk + 1):
@@ -763,6 +789,13 @@ def orthogonality(U) -> tuple[float, float]:
     n = U.shape[1]
     e = float(np.linalg.norm(U.T @ U - np.eye(n)))
     return e / math.sqrt(n), e / n
+
+
+def svd_residual(A, U, S, V) -> float:
+    """||A V - U diag(S)||_F / max(S) (CheckResidual_SVD, check_routines.f90:70-86;
+    ZoloCHK1.f90:103-134)."""
+    S = np.asarray(S)
+    return float(np.linalg.norm(A @ V - U * S) / S.max())
 
 
 def haar(n: int, rng: np.random.Generator) -> np.ndarray:

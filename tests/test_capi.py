@@ -87,7 +87,7 @@ def test_spectral_division_argument_codes_without_gpu(zl):
     assert L.zolo_eig_device(4, p, 4, p, 4, p, p, 3, C.byref(rk), None, None, None) == -8
     assert L.zolo_eig_device(4, p, 4, p, 4, None, p, 4, C.byref(rk), None, None, None) == -6
     # zolo_eig_below_device(precision, rk, n, A, lda, sigma, w, V, ldv, count, info, err, iters, stream)
-    assert L.zolo_eig_below_device(3, 0, 4, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -1
+    assert L.zolo_eig_below_device(5, 0, 4, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -1
     assert L.zolo_eig_below_device(0, 9, 4, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -2
     assert L.zolo_eig_below_device(0, 0, 0, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -3
     assert L.zolo_eig_below_device(0, 0, 4, p, 4, float("nan"), p, p, 4, C.byref(cnt), None, None, None,
@@ -97,3 +97,55 @@ def test_spectral_division_argument_codes_without_gpu(zl):
     d = C.cast(buf, C.POINTER(C.c_double))
     assert L.zolo_subit_host(4, d, 4, -1.0, d, 4, C.byref(rk), None, None) == -4
     assert L.zolo_eig_host(4, d, 4, d, 4, d, d, 4, None) == -9
+
+
+def test_pd_argument_codes_without_gpu(zl):
+    """The Zolo-PD and QDWH device entry points return their argument codes
+    before any HIP call (a NULL A or U never reaches a kernel)."""
+    import ctypes as C
+    L = zl.LIB
+    buf = (C.c_double * 16)()
+    p = C.cast(buf, C.c_void_p)
+    # zolo_pd_device(precision, sym, rk, n, alpha, beta, A, lda, U, ldu, H, ldh, info, stream)
+    pd = L.zolo_pd_device
+    assert pd(5, b"N", 2, 4, 1.0, 0.5, p, 4, p, 4, None, 4, None, None) == -1
+    assert pd(0, b"X", 2, 4, 1.0, 0.5, p, 4, p, 4, None, 4, None, None) == -2
+    assert pd(0, b"N", 9, 4, 1.0, 0.5, p, 4, p, 4, None, 4, None, None) == -3
+    assert pd(0, b"N", 2, 0, 1.0, 0.5, p, 4, p, 4, None, 4, None, None) == -4
+    assert pd(0, b"N", 2, 4, 0.0, 0.5, p, 4, p, 4, None, 4, None, None) == -5
+    assert pd(0, b"N", 2, 4, 1.0, 2.0, p, 4, p, 4, None, 4, None, None) == -6
+    assert pd(0, b"N", 2, 4, 1.0, 0.5, None, 4, p, 4, None, 4, None, None) == -7
+    assert pd(0, b"N", 2, 4, 1.0, 0.5, p, 3, p, 4, None, 4, None, None) == -8
+    assert pd(0, b"N", 2, 4, 1.0, 0.5, p, 4, None, 4, None, 4, None, None) == -9
+    assert pd(0, b"N", 2, 4, 1.0, 0.5, p, 4, p, 3, None, 4, None, None) == -10
+    assert pd(0, b"N", 2, 4, 1.0, 0.5, p, 4, p, 4, p, 3, None, None) == -12
+    # zolo_pd_auto_device(precision, sym, rk, n, A, lda, U, ldu, H, ldh, info, stream): its own positions
+    au = L.zolo_pd_auto_device
+    assert au(5, b"N", 0, 4, p, 4, p, 4, None, 4, None, None) == -1
+    assert au(0, b"N", -1, 4, p, 4, p, 4, None, 4, None, None) == -3
+    assert au(0, b"N", 0, 0, p, 4, p, 4, None, 4, None, None) == -4
+    assert au(0, b"N", 0, 4, None, 4, p, 4, None, 4, None, None) == -5
+    assert au(0, b"N", 0, 4, p, 3, p, 4, None, 4, None, None) == -6
+    assert au(0, b"N", 0, 4, p, 4, None, 4, None, 4, None, None) == -7
+    assert au(0, b"N", 0, 4, p, 4, p, 3, None, 4, None, None) == -8
+    assert au(0, b"N", 0, 4, p, 4, p, 4, p, 3, None, None) == -10
+    # zolo_qdwh_device(sym, n, sigma, A, lda, U, ldu, H, ldh, info, stream)
+    qd = L.zolo_qdwh_device
+    assert qd(b"N", 4, 0.0, None, 4, p, 4, None, 4, None, None) == -4
+    assert qd(b"N", 4, 0.0, p, 4, None, 4, None, 4, None, None) == -6
+    assert qd(b"N", 4, float("inf"), p, 4, p, 4, None, 4, None, None) == -3
+
+
+def test_mm_size_line_beyond_the_file_is_refused(zl, tmp_path):
+    """A size line that claims more entries than the file can hold is an
+    argument error (-3), not an allocation of that many entries."""
+    f = tmp_path / "huge.mtx"
+    f.write_text("%%MatrixMarket matrix coordinate real general\n3 3 9000000000000000000\n1 1 2.0\n")
+    with pytest.raises(zl.ZoloError) as e:
+        zl.mm_read(str(f))
+    assert e.value.status == -3
+    g = tmp_path / "short.mtx"
+    g.write_text("%%MatrixMarket matrix coordinate real general\n3 3 5\n1 1 2.0\n2 2 3.0\n")
+    with pytest.raises(zl.ZoloError) as e:
+        zl.mm_read(str(g))
+    assert e.value.status in (-3, -4)

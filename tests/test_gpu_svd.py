@@ -70,7 +70,11 @@ def test_zolo_svd_c4_closed_form(zl, torch_gpu, n):
     V = zl.from_device(dV)
     assert np.max(np.abs(S - np.sort(s))) <= 1e-13
     assert np.linalg.norm(A - (U * S) @ V.T) / np.linalg.norm(A) <= 1e-13
+    # the reference's own residual (check_routines.f90:70-86), within 10x of
+    # the oracle's on the same matrix
+    Sref, Uref, Vref = zo.zolo_svd(A)
+    res, res_o = zo.svd_residual(A, U, S, V), zo.svd_residual(A, Uref, Sref, Vref)
+    assert res <= 10 * max(res_o, 1e-15), (res, res_o)
     assert np.linalg.norm(U.T @ U - np.eye(n)) / math.sqrt(n) <= 1e-13
     assert np.linalg.norm(V.T @ V - np.eye(n)) / math.sqrt(n) <= 1e-13
-    Sref, _, _ = zo.zolo_svd(A)
     assert np.max(np.abs(S - Sref)) <= 1e-13

@@ -177,6 +177,27 @@ def test_shift_steps_closed_form(con, rk):
         assert np.linalg.norm(zq - ref) <= max(1e-12, 1e-14 * con) * np.linalg.norm(ref)
 
 
+def test_lapack_qr_step_matches_structured():
+    """The LAPACK-QR form of the QR step (the timed CPU baseline) gives the
+    structured step's Z, and the driver built on it the same schedule and
+    polar factor."""
+    n, rk = 24, 4
+    rng = np.random.default_rng(11)
+    X = zo.haar(n, rng) @ np.diag(np.geomspace(1, 1e5, n)) @ zo.haar(n, rng).T
+    coeff = zo.compute_coeff(1e5, EPS, rk)
+    for ip in range(1, rk + 1):
+        a = zo.zoloqr_step(X, coeff, rk, ip)
+        b = zo.zoloqr_step_lapack(X, coeff, rk, ip)
+        assert np.linalg.norm(a - b) <= 1e-14 * 1e5 * np.linalg.norm(a)   # the QR route: u * kappa(X)
+    A, U, s, V = zo.make_matrix(40, "loguniform", 1e12, 9)
+    al, be = zo.bounds_estimate(A)
+    rk = zo.choosem(1 / be)
+    Q1, l1 = zo.pzolopd1(A, al, be, rk)
+    Q2, l2 = zo.pzolopd1(A, al, be, rk, lapack_qr=True)
+    assert l1.schedule == l2.schedule and l1.iters == l2.iters
+    assert np.linalg.norm(Q1 - Q2) < 1e-13 * 1e12 * 40 / 1e4
+
+
 # ---------------------------------------------------------------- drivers vs closed form
 @pytest.mark.parametrize("n,kind,kappa", [(40, "geometric", 1e6), (48, "loguniform", 1e12), (40, "cluster", 1e8),
                                           (32, "sine", 0)])
@@ -459,3 +480,19 @@ def test_eig_below_no_eigenvalue_below():
     A, lam, _ = _sym_case(n, 5, 0.5)
     w, V, log = zo.eig_below(A, float(lam[0]) - 0.5)
     assert log.rk == 0 and w.size == 0 and V.shape == (n, 0)
+
+
+def test_svd_residual_metric():
+    """check_routines.f90:70-86: zero (to rounding) for an exact SVD, and a
+    change of delta in one singular value gives exactly delta / max(S)."""
+    rng = np.random.default_rng(4)
+    n = 30
+    U, V = zo.haar(n, rng), zo.haar(n, rng)
+    s = np.geomspace(3.0, 0.1, n)
+    A = (U * s) @ V.T
+    assert zo.svd_residual(A, U, s, V) < 1e-14
+    s2 = s.copy()
+    s2[7] += 1e-3
+    assert math.isclose(zo.svd_residual(A, U, s2, V), 1e-3 / 3.0, rel_tol=1e-9)
+    # the residual divides by the largest singular value, not the Frobenius norm
+    assert math.isclose(zo.svd_residual(2 * A, U, 2 * s2, V), 1e-3 / 3.0, rel_tol=1e-9)

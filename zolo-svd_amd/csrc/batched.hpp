@@ -65,6 +65,11 @@ struct BatchResult {
   int buckets = 0;
   double phase_ms[8] = {};         // 0 alpha + beta, 1 plan, 2 tile choice, 3 iterations, 4 fallback
   double time_ms = 0.0;
+  // algorithmic flops of the executed work at each matrix's own order n (not
+  // the padded bucket order): FP64 LU bound (2/3) n^3; per QR step 5 n^3 per
+  // shift (FP64 for a first step at CON > 1e8); per Cholesky step (7/3) n^3
+  // per shift + n^3 (the shared Gram matrix); fallback and restart calls added
+  double flops_f32 = 0.0, flops_f64 = 0.0;
 };
 
 int pd_batched_run(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* const* U,

@@ -563,11 +563,18 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
     for (auto& th : pool) th.join();
   }
   for (int i = 0; i < count; ++i) {
+    const double n3 = (double)n[i] * (double)n[i] * (double)n[i];
+    if (opt.l0_source == 0) res.flops_f64 += (2.0 / 3.0) * n3;
     if (!ok[i]) {
       res.failed[i] = 1;
     } else {
       res.iters[i] = plans[i].steps;
       res.rk[i] = plans[i].r;
+      const Plan& p = plans[i];
+      for (int k = 0; k < p.steps; ++k) {
+        const double f = p.type[k] == 'Q' ? 5.0 * p.r * n3 : (7.0 / 3.0) * p.r * n3 + n3;
+        (p.f64[k] ? res.flops_f64 : res.flops_f32) += f;
+      }
     }
   }
   Blob blob;
@@ -734,6 +741,8 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
         res.continued[i] = r2.continued[k];
       }
       res.graph_nodes += r2.graph_nodes;
+      res.flops_f32 += r2.flops_f32;
+      res.flops_f64 += r2.flops_f64;
     }
   }
   // ---- phase 5: continuation --------------------------------------------------
@@ -769,6 +778,8 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
       res.iters[i] += r3.iters[k];
     }
     res.graph_nodes += r3.graph_nodes;
+    res.flops_f32 += r3.flops_f32;
+    res.flops_f64 += r3.flops_f64;
   }
   for (int i = 0; i < count; ++i) res.failed[i] = (res.orth[i] <= opt.orth_tol) ? 0 : 1;
   res.phase_ms[4] = ms_since(t5);

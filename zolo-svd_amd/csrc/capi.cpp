@@ -3,6 +3,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -57,31 +59,55 @@ int ensure_device() {
   return kOk;
 }
 
+// Every extern "C" entry point below is a function-try-block: an exception
+// from the host side (a failed host allocation, a library error) becomes a
+// status code instead of leaving the C ABI.
+int exception_status() {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return kErrAlloc;
+  } catch (const std::exception& e) {
+    set_error(std::string("internal error: ") + e.what());
+    return kErrInternal;
+  } catch (...) {
+    set_error("internal error");
+    return kErrInternal;
+  }
+}
+
 }  // namespace
 
 extern "C" {
 
-int zolo_init(int device) {
+int zolo_init(int device) try {
   if (device >= 0) ZOLO_HIP(hipSetDevice(device));
   ZOLO_HIP(hipGetDevice(&g_device));
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_finalize(void) {
+int zolo_finalize(void) try {
   workspace_release_all();
   l0_release();
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 const char* zolo_last_error(void) { return last_error(); }
 
 int zolo_version(void) { return 100; }
 
-int zolo_compute_coeff(double con, double tol, int rk, double* coeff) {
+int zolo_compute_coeff(double con, double tol, int rk, double* coeff) try {
   if (!(con >= 1.0)) return -1;
   if (rk < 1 || rk > 8) return -3;
   compute_coeff(con, tol, rk, coeff);
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_choosem(double con) { return choosem(con); }
@@ -91,24 +117,46 @@ void zolo_mellipj(double u, double alpha, double tol, double* sn, double* cn, do
   mellipj(u, alpha, tol, sn, cn, dn);
 }
 
+// Argument checks of the device-pointer Zolo-PD entry points, in the argument
+// positions of zolo_pd_device; they run before any HIP call.
+static int pd_device_args(int precision, char sym, int rk, int64_t n, const double* A, int64_t lda, const double* U,
+                          int64_t ldu, const double* H, int64_t ldh, bool autob) {
+  if (precision < 0 || precision > 4) return -1;
+  if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -2;
+  if (rk < (autob ? 0 : 1) || rk > 8) return -3;
+  if (n < 1) return -4;
+  if (!A) return -7;
+  if (lda < n) return -8;
+  if (!U) return -9;
+  if (ldu < n) return -10;
+  if (H && ldh < n) return -12;
+  return kOk;
+}
+
 int zolo_pd_device(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
-                   int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream) {
+                   int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream) try {
   if (!(alpha > 0.0)) return -5;  // (alpha, beta) = (0, 0) is zolo_pd_auto_device's mode, not this one's
   if (!(beta > 0.0) || !(beta <= 1.0)) return -6;
+  ZOLO_CHECK(pd_device_args(precision, sym, rk, n, A, lda, U, ldu, H, ldh, false));
   ZOLO_CHECK(ensure_device());
   return zolo_pd_run(precision, sym, rk, n, alpha, beta, A, lda, U, ldu, H, ldh, info, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_pd_auto_device(int precision, char sym, int rk, int64_t n, const double* A, int64_t lda, double* U,
-                        int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream) {
-  if (!A) return -6;
-  if (!U) return -8;
+                        int64_t ldu, double* H, int64_t ldh, zolo_info_t* info, void* stream) try {
+  // zolo_pd_device's positions 7..12 are this function's 5..10 (no alpha, beta)
+  const int st = pd_device_args(precision, sym, rk, n, A, lda, U, ldu, H, ldh, true);
+  if (st < 0) return st <= -7 ? st + 2 : st;
   ZOLO_CHECK(ensure_device());
   return zolo_pd_run(precision, sym, rk, n, 0.0, 0.0, A, lda, U, ldu, H, ldh, info, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
-                 int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) {
+                 int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) try {
   if (n < 1) return -4;
   if (!(alpha > 0.0)) return -5;
   if (!(beta > 0.0) || !(beta <= 1.0)) return -6;
@@ -127,10 +175,12 @@ int zolo_pd_host(int precision, char sym, int rk, int64_t n, double alpha, doubl
   ZOLO_CHECK(dU.download(U, ldu, n));
   if (H) ZOLO_CHECK(dH.download(H, ldh, n));
   return st;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_pd2_host(int precision, char sym, int rk, int64_t n, double alpha, double beta, const double* A,
-                  int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) {
+                  int64_t lda, double* U, int64_t ldu, double* H, int64_t ldh, zolo_info_t* info) try {
   if (n < 1) return -4;
   if (!(alpha > 0.0)) return -5;
   if (!(beta > 0.0) || !(beta <= 1.0)) return -6;
@@ -150,9 +200,11 @@ int zolo_pd2_host(int precision, char sym, int rk, int64_t n, double alpha, doub
   ZOLO_CHECK(dU.download(U, ldu, n));
   if (H) ZOLO_CHECK(dH.download(H, ldh, n));
   return st;
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_bounds_host(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta) {
+int zolo_bounds_host(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta) try {
   if (convention != 0 && convention != 1) return -1;
   if (n < 1) return -2;
   if (lda < n) return -4;
@@ -167,9 +219,11 @@ int zolo_bounds_host(int convention, int64_t n, const double* A, int64_t lda, do
   const int st = bounds_run(convention, n, dA.p, n, alpha, beta, ar, nullptr);
   ZOLO_HIP(hipDeviceSynchronize());
   return st;
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_normest_host(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt) {
+int zolo_normest_host(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt) try {
   if (m < 1) return -1;
   if (n < 1) return -2;
   if (lda < m) return -4;
@@ -197,6 +251,8 @@ int zolo_normest_host(int64_t m, int64_t n, const double* A, int64_t lda, double
   (void)hipDeviceSynchronize();
   (void)hipFree(dA);
   return st;
+} catch (...) {
+  return exception_status();
 }
 
 // Host stacked QR pieces: the 2n x n stack S (ld lds) is copied to the device.
@@ -246,12 +302,14 @@ static int stacked_host(int64_t n, double* S, int64_t lds, double* tau, bool geq
 }
 
 int zolo_mpdgeqrf_host(int64_t n, double* S, int64_t lds, double* tau) { return stacked_host(n, S, lds, tau, true); }
-int zolo_mpdorgqr_host(int64_t n, double* S, int64_t lds, const double* tau) {
+int zolo_mpdorgqr_host(int64_t n, double* S, int64_t lds, const double* tau) try {
   return stacked_host(n, S, lds, const_cast<double*>(tau), false);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_bounds_device(int convention, int64_t n, const double* A, int64_t lda, double* alpha, double* beta,
-                       void* stream) {
+                       void* stream) try {
   if (convention != 0 && convention != 1) return -1;
   if (n < 1) return -2;
   if (lda < n) return -4;
@@ -261,16 +319,28 @@ int zolo_bounds_device(int convention, int64_t n, const double* A, int64_t lda, 
   ZOLO_CHECK(workspace_acquire(bytes, &base));
   Arena ar(base, bytes);
   return bounds_run(convention, n, A, lda, alpha, beta, ar, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_qdwh_device(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu,
-                     double* H, int64_t ldh, zolo_info_t* info, void* stream) {
+                     double* H, int64_t ldh, zolo_info_t* info, void* stream) try {
+  if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -1;
+  if (n < 1) return -2;
+  if (!std::isfinite(sigma)) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!U) return -6;
+  if (ldu < n) return -7;
+  if (H && ldh < n) return -9;
   ZOLO_CHECK(ensure_device());
   return qdwh_run(sym, n, sigma, A, lda, U, ldu, H, ldh, info, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_qdwh_host(char sym, int64_t n, double sigma, const double* A, int64_t lda, double* U, int64_t ldu, double* H,
-                   int64_t ldh, zolo_info_t* info) {
+                   int64_t ldh, zolo_info_t* info) try {
   if (n < 1) return -2;
   if (lda < n) return -5;
   if (ldu < n) return -7;
@@ -287,10 +357,12 @@ int zolo_qdwh_host(char sym, int64_t n, double sigma, const double* A, int64_t l
   ZOLO_CHECK(dU.download(U, ldu, n));
   if (H) ZOLO_CHECK(dH.download(H, ldh, n));
   return st;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_chol_step_host(char sym, int64_t n, const double* X, int64_t ldx, double* Z, int64_t ldz, const double* coeff,
-                        int rk, int ip) {
+                        int rk, int ip) try {
   if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -1;
   if (n < 1) return -2;
   if (ldx < n) return -4;
@@ -322,10 +394,12 @@ int zolo_chol_step_host(char sym, int64_t n, const double* X, int64_t ldx, doubl
     return kErrNotSpd;
   }
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_qr_step_host(int64_t n, const double* X, int64_t ldx, double* Z, int64_t ldz, const double* coeff, int rk,
-                      int ip) {
+                      int ip) try {
   if (n < 1) return -1;
   if (ldx < n) return -3;
   if (ldz < n) return -5;
@@ -350,10 +424,12 @@ int zolo_qr_step_host(int64_t n, const double* X, int64_t ldx, double* Z, int64_
   ZOLO_HIP(hipDeviceSynchronize());
   ZOLO_CHECK(dZ.download(Z, ldz, n));
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_normest_device(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt,
-                        void* stream) {
+                        void* stream) try {
   if (m < 1) return -1;
   if (n < 1) return -2;
   if (lda < m) return -4;
@@ -363,11 +439,13 @@ int zolo_normest_device(int64_t m, int64_t n, const double* A, int64_t lda, doub
   ZOLO_CHECK(workspace_acquire(bytes, &base));
   Arena ar(base, bytes);
   return normest_run(m, n, A, lda, tol, est, cnt, ar, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
                      const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc,
-                     void* stream) {
+                     void* stream) try {
   if (precision < 0 || precision > 2) return -1;
   if (m < 0) return -4;
   if (n < 0) return -5;
@@ -379,19 +457,25 @@ int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n
   d.transA = transa; d.transB = transb; d.M = m; d.N = n; d.K = k;
   d.A = A; d.lda = lda; d.B = B; d.ldb = ldb; d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
   return gemm((Prec)precision, d, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_profile_enable(int on) {
+int zolo_profile_enable(int on) try {
   ledger_enable(on != 0);
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_profile_gemm(int precision, int transa, int transb, int tile_m, int tile_n, int64_t* launches, double* ms,
-                      double* flops) {
+                      double* flops) try {
   return ledger_query(precision, transa, transb, tile_m, tile_n, launches, ms, flops);
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t ldq, double* tau, void* stream) {
+int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t ldq, double* tau, void* stream) try {
   if (n < 1) return -1;
   if (lds < 2 * n) return -3;
   if (Q && ldq < 2 * n) return -5;
@@ -403,9 +487,11 @@ int zolo_stacked_qr_device(int64_t n, double* S, int64_t lds, double* Q, int64_t
   Ctx c;
   c.s = (hipStream_t)stream;
   return stacked_qr<double>(c, S, lds, 0, Q, ldq, 0, n, 1, tau, ar);
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* Li, int64_t ldl, void* stream) {
+int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* Li, int64_t ldl, void* stream) try {
   if (precision < 0 || precision > 2) return -1;
   if (n < 1) return -2;
   if (lda < n) return -4;
@@ -438,12 +524,14 @@ int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* L
     return kErrNotSpd;
   }
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* const* U,
                            const int64_t* ldu, int precision, int l0_source, const double* beta_in, double orth_tol,
                            int rk, int tile, int use_graph, double* beta_out, double* orth_out, int* iters_out,
-                           int* flags_out, zolo_batch_info_t* info, void* stream) {
+                           int* flags_out, zolo_batch_info_t* info, void* stream) try {
   if (count < 0) return -1;
   if (count > 0 && (!n || !A || !lda || !U || !ldu)) return -2;
   if (precision != 1) {
@@ -502,14 +590,18 @@ int zolo_pd_batched_device(int count, const int64_t* n, const float* const* A, c
     for (int k = 0; k < 5; ++k) info->phase_ms[k] = res.phase_ms[k];
     info->phase_ms[5] = learned_ms;
     info->time_ms = res.time_ms + learned_ms;
+    info->flops_f32 = res.flops_f32;
+    info->flops_f64 = res.flops_f64;
   }
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_batched_tile(int64_t n, int precision) { return batched_tile_of(n, precision); }
 
 int zolo_syevd_device(int64_t n, double* H, int64_t ldh, double* w, double* Z, int64_t ldz, double* phase_ms,
-                      void* stream) {
+                      void* stream) try {
   if (n < 1 || n > 32768) return -1;
   if (!H) return -2;
   if (ldh < n) return -3;
@@ -518,10 +610,12 @@ int zolo_syevd_device(int64_t n, double* H, int64_t ldh, double* w, double* Z, i
   if (ldz < n) return -6;
   ZOLO_CHECK(ensure_device());
   return syevd_run(n, H, ldh, w, Z, ldz, (hipStream_t)stream, phase_ms);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_subit_device(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err,
-                      int* iters, void* stream) {
+                      int* iters, void* stream) try {
   if (n < 1 || n > 32768) return -1;
   if (!A) return -2;
   if (lda < n) return -3;
@@ -531,10 +625,12 @@ int zolo_subit_device(int64_t n, double* A, int64_t lda, double normA, double* Q
   if (!rk) return -7;
   ZOLO_CHECK(ensure_device());
   return subit_run(n, A, lda, normA, Q, ldq, rk, err, iters, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
-                    int* rk, double* err, int* iters, void* stream) {
+                    int* rk, double* err, int* iters, void* stream) try {
   if (n < 1 || n > 32768) return -1;
   if (!Q) return -2;
   if (ldq < n) return -3;
@@ -546,12 +642,14 @@ int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, 
   if (!rk) return -9;
   ZOLO_CHECK(ensure_device());
   return zoloeig_run(n, Q, ldq, As, lda, w, V, ldv, rk, err, iters, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
                           double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
-                          void* stream) {
-  if (precision < 0 || precision > 2) return -1;
+                          void* stream) try {
+  if (precision < 0 || precision > 4) return -1;
   if (rk < 0 || rk > 8) return -2;
   if (n < 1 || n > 32768) return -3;
   if (!A) return -4;
@@ -580,10 +678,12 @@ int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int
   if (st) return st;
   // PQDWHSUBIT, PDSYEVD on the Rk block, V1 VV1 (pqdwheig.f90:119-144 = ZoloEig.f90:80-103)
   return zoloeig_run(n, Q.p, n, As.p, n, w, V, ldv, count, err, iters, s);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_subit_host(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_t ldq, int* rk, double* err,
-                    int* iters) {
+                    int* iters) try {
   if (n < 1 || n > 32768) return -1;
   if (!A) return -2;
   if (lda < n) return -3;
@@ -601,10 +701,12 @@ int zolo_subit_host(int64_t n, double* A, int64_t lda, double normA, double* Q, 
   ZOLO_CHECK(dA.download(A, lda, n));
   ZOLO_CHECK(dQ.download(Q, ldq, n));
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_eig_host(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
-                  int* rk) {
+                  int* rk) try {
   if (n < 1 || n > 32768) return -1;
   if (!Q) return -2;
   if (ldq < n) return -3;
@@ -637,22 +739,30 @@ int zolo_eig_host(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, do
   }
   *rk = k;
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth) {
+int zolo_mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth) try {
   return mm_info(path, rows, cols, nnz, mtype, bandwidth);
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* mtype) {
+int zolo_mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* mtype) try {
   return mm_dense_host(path, sigma, A, lda, mtype);
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_mm_dense_device(const char* path, double sigma, double* A, int64_t lda, int* mtype, void* stream) {
+int zolo_mm_dense_device(const char* path, double sigma, double* A, int64_t lda, int* mtype, void* stream) try {
   ZOLO_CHECK(ensure_device());
   return mm_dense_device(path, sigma, A, lda, mtype, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err) {
+int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, int64_t ldz, double* err) try {
   if (n < 1) return -1;
   if (!(fone != 0.0)) return -2;
   if (!A) return -3;
@@ -681,9 +791,11 @@ int zolo_formx2_host(int64_t n, double fone, double* A, int64_t lda, double* Z, 
   ZOLO_CHECK(dA.download(A, lda, n));
   ZOLO_CHECK(dA.download(Z, ldz, n));  // FormX2 leaves the new iterate in both arrays
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_syevd_host(int64_t n, double* H, int64_t ldh, double* w, double* Z, int64_t ldz) {
+int zolo_syevd_host(int64_t n, double* H, int64_t ldh, double* w, double* Z, int64_t ldz) try {
   if (n < 1 || n > 32768) return -1;
   if (!H) return -2;
   if (ldh < n) return -3;
@@ -706,11 +818,13 @@ int zolo_syevd_host(int64_t n, double* H, int64_t ldh, double* w, double* Z, int
   ZOLO_HIP(hipMemcpy(w, dw, sizeof(double) * n, hipMemcpyDeviceToHost));
   ZOLO_CHECK(dZ.download(Z, ldz, n));
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U, int64_t ldu,
-                    double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream) {
-  if (precision < 0 || precision > 2) return -1;
+                    double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream) try {
+  if (precision < 0 || precision > 4) return -1;
   if (rk < 0 || rk > 8) return -2;
   if (n < 1 || n > 32768) return -3;
   if (!A) return -4;
@@ -754,11 +868,13 @@ int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t l
     phase_ms[4] = std::chrono::duration<double, std::milli>(t2 - t1).count();   // whole eigensolve + U
   }
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_l0_param_count(void) { return kParamCount; }
 
-int zolo_l0_set_weights(const float* params, int nparams) {
+int zolo_l0_set_weights(const float* params, int nparams) try {
   if (!params) return -1;
   if (nparams != kParamCount) {
     set_error("zolo_l0_set_weights: expected " + std::to_string(kParamCount) + " parameters");
@@ -771,10 +887,12 @@ int zolo_l0_set_weights(const float* params, int nparams) {
     }
   ZOLO_CHECK(ensure_device());
   return l0_set_weights(params);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_l0_features_device(int count, const int64_t* n, const float* const* A, const int64_t* lda, float* feat,
-                            float* smax, void* stream) {
+                            float* smax, void* stream) try {
   if (count < 0) return -1;
   if (count > 0 && (!n || !A || !lda || !feat || !smax)) return -2;
   for (int i = 0; i < count; ++i) {
@@ -784,9 +902,11 @@ int zolo_l0_features_device(int count, const int64_t* n, const float* const* A, 
   if (count == 0) return kOk;
   ZOLO_CHECK(ensure_device());
   return l0_features(count, n, A, lda, feat, smax, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
-int zolo_l0_forward_device(const float* params, const float* feat, int N, float* y, void* stream) {
+int zolo_l0_forward_device(const float* params, const float* feat, int N, float* y, void* stream) try {
   if (!params) return -1;
   if (!feat) return -2;
   if (N < 0) return -3;
@@ -795,10 +915,12 @@ int zolo_l0_forward_device(const float* params, const float* feat, int N, float*
   ZOLO_CHECK(kl_forward(params, feat, N, nullptr, nullptr, y, (hipStream_t)stream));
   ZOLO_HIP(hipStreamSynchronize((hipStream_t)stream));
   return kOk;
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_l0_grad_device(const float* params, const float* feat, const float* target, int N, float tau, float* grad,
-                        float* loss, void* stream) {
+                        float* loss, void* stream) try {
   if (!params) return -1;
   if (!feat) return -2;
   if (!target) return -3;
@@ -807,10 +929,12 @@ int zolo_l0_grad_device(const float* params, const float* feat, const float* tar
   if (!grad) return -6;
   ZOLO_CHECK(ensure_device());
   return l0_grad(params, feat, target, N, tau, grad, loss, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 int zolo_l0_train_device(float* params, const float* feat, const float* target, int N, int steps, float lr, float tau,
-                         float* loss_hist, void* stream) {
+                         float* loss_hist, void* stream) try {
   if (!params) return -1;
   if (!feat) return -2;
   if (!target) return -3;
@@ -820,6 +944,8 @@ int zolo_l0_train_device(float* params, const float* feat, const float* target, 
   if (!(tau > 0.0f && tau < 1.0f)) return -7;
   ZOLO_CHECK(ensure_device());
   return l0_train(params, feat, target, N, steps, lr, tau, loss_hist, (hipStream_t)stream);
+} catch (...) {
+  return exception_status();
 }
 
 }  // extern "C"

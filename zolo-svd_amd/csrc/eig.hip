@@ -600,12 +600,12 @@ __global__ __launch_bounds__(kT) void dc_leaves_kernel(const double* d, const do
 // V (mrows x jb, ld ldv): column c = reflector of A column k0 + c, rows from r0:
 // unit at row k0 + c + 1, entries A[row][k0 + c] below, zero above.
 __global__ __launch_bounds__(kT) void trd_extract_v_kernel(const double* A, int64_t ld, int64_t k0, int64_t r0,
-                                                           int jb, int64_t mrows, double* V, int64_t ldv) {
+                                                           int jb, int64_t mrows, double* V, int64_t ldv, int off) {
   const int64_t total = mrows * jb;
   for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < total; e += (int64_t)gridDim.x * kT) {
     const int64_t i = e % mrows;
     const int c = (int)(e / mrows);
-    const int64_t row = r0 + i, head = k0 + c + 1;
+    const int64_t row = r0 + i, head = k0 + c + off;
     double x = 0.0;
     if (row == head) x = 1.0;
     else if (row > head) x = A[row + (k0 + c) * ld];
@@ -665,9 +665,9 @@ int ke_trd_panel_end(double* A, int64_t ld, int64_t k0, int jb, int64_t n, doubl
 }
 
 int ke_trd_extract_v(const double* A, int64_t ld, int64_t k0, int64_t r0, int jb, int64_t mrows, double* V,
-                     int64_t ldv, hipStream_t s) {
+                     int64_t ldv, int off, hipStream_t s) {
   hipLaunchKernelGGL(trd_extract_v_kernel, dim3((unsigned)std::min<int64_t>(8192, blocks_for(mrows * jb))), dim3(kT), 0,
-                     s, A, ld, k0, r0, jb, mrows, V, ldv);
+                     s, A, ld, k0, r0, jb, mrows, V, ldv, off);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }

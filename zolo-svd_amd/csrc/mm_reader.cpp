@@ -122,9 +122,19 @@ int mm_parse(const char* path, bool entries, MmFile& m) {
     return mm_error(-3, "bad size line '" + line + "'");
   m.rows = r; m.cols = c; m.nnz = z;
   if (!entries) return kOk;
-  m.i.resize((size_t)z);
-  m.j.resize((size_t)z);
-  m.v.resize((size_t)z);
+  // An entry line takes at least 4 bytes ("i j" and a newline), so a size line
+  // that claims more entries than the rest of the file can hold is refused
+  // before anything is allocated; storage then grows with the entries read.
+  const long here = std::ftell(f);
+  if (here >= 0 && std::fseek(f, 0, SEEK_END) == 0) {
+    const long end = std::ftell(f);
+    if (end < here || std::fseek(f, here, SEEK_SET) != 0) return mm_error(-3, "cannot size the file");
+    if (z > (long long)(end - here) / 4 + 1)
+      return mm_error(-3, "size line claims " + std::to_string(z) + " entries, more than the file holds");
+  }
+  m.i.reserve((size_t)std::min<long long>(z, 1 << 20));
+  m.j.reserve(m.i.capacity());
+  m.v.reserve(m.i.capacity());
   int64_t k = 0;
   while (k < z && get_line(f, line)) {
     if (blank(line) || line[0] == '%') continue;
@@ -145,9 +155,9 @@ int mm_parse(const char* path, bool entries, MmFile& m) {
     if (ii < 1 || ii > r || jj < 1 || jj > c)
       return mm_error(-5, "entry (" + std::to_string(ii) + ", " + std::to_string(jj) + ") outside " +
                               std::to_string(r) + " x " + std::to_string(c));
-    m.i[(size_t)k] = ii;
-    m.j[(size_t)k] = jj;
-    m.v[(size_t)k] = val;
+    m.i.push_back(ii);
+    m.j.push_back(jj);
+    m.v.push_back(val);
     ++k;
   }
   if (k < z) return mm_error(-4, "file ends after " + std::to_string(k) + " of " + std::to_string(z) + " entries");

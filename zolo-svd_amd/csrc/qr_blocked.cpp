@@ -329,13 +329,14 @@ template int dense_orgqr_cols<double>(const Ctx&, const double*, int64_t, int64_
                                       int64_t, int64_t, Arena&);
 
 int apply_trd_block(const Ctx& c, const double* A, int64_t lda, const double* tau, int64_t k0, int jb, int64_t n,
-                    double* C, int64_t ldc, int64_t ncols, Arena& ar) {
+                    double* C, int64_t ldc, int64_t ncols, Arena& ar, int off) {
   const int nbo = kQrOuter;
   if (jb <= 0 || jb > nbo) {
     set_error("apply_trd_block: block width out of range");
     return kErrArg;
   }
-  const int64_t r0 = k0 + 1, mrows = n - r0;
+  const int64_t r0 = k0 + off, mrows = n - r0;
+  if (mrows <= 0) return kOk;
   double* V = ar.take<double>((size_t)mrows * jb);
   double* G = ar.take<double>((size_t)nbo * nbo);
   double* Tm = ar.take<double>((size_t)nbo * nbo);
@@ -343,7 +344,7 @@ int apply_trd_block(const Ctx& c, const double* A, int64_t lda, const double* ta
   double* W = ar.take<double>((size_t)nbo * ncols);
   double* W2 = ar.take<double>((size_t)nbo * ncols);
   ZOLO_ARENA_OK(ar);
-  ZOLO_CHECK(ke_trd_extract_v(A, lda, k0, r0, jb, mrows, V, mrows, c.s));
+  ZOLO_CHECK(ke_trd_extract_v(A, lda, k0, r0, jb, mrows, V, mrows, off, c.s));
   GemmDesc d;  // G = V^T V
   d.transA = 1; d.M = jb; d.N = jb; d.K = mrows;
   d.A = V; d.lda = mrows; d.B = V; d.ldb = mrows; d.C = G; d.ldc = nbo;

@@ -102,7 +102,11 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
   // alpha = beta = 0: the bounds come from the QR of A/alpha that the first
   // QR step needs anyway (see below); rk = 0 then picks choosem(1/beta)
   const bool autob = alpha == 0.0 && beta == 0.0;
-  if (precision < 0 || precision > 2) return -1;
+  if (precision < 0 || precision > 4) return -1;
+  // 3 and 4: FP32 / BF16x3 Cholesky steps on the reference's schedule (every
+  // step takes all r shifts and the loop ends on the ERR test only)
+  const bool refsched = precision >= 3;
+  if (refsched) precision -= 2;
   if (sym != 'S' && sym != 's' && sym != 'N' && sym != 'n') return -2;
   if (rk < (autob ? 0 : 1) || rk > 8) return -3;
   if (n < 1) return -4;
@@ -234,7 +238,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
     // bound reaches FP32 convergence in this one step (the polish then works
     // to FP64 from that iterate), falling back to r.
     int rs = rk;
-    if (mixed && !use_qr) {
+    if (mixed && !refsched && !use_qr) {
       std::vector<double> cr(3 * rk);
       for (int r = 1; r < rk; ++r) {
         double f1 = 0.0, c1 = 0.0;
@@ -322,7 +326,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
     // has converged to its working precision (CON - 1 below the FP32 unit
     // roundoff), a further low-precision step cannot improve it; the FP64
     // polish takes over.  The FP64 path keeps the reference's ERR test.
-    if (low_active && con - 1.0 <= kLowConverged) break;
+    if (low_active && !refsched && con - 1.0 <= kLowConverged) break;
   }
   info->iters = it;
   int hinfo[8];

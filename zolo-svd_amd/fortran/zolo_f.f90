@@ -408,7 +408,6 @@ subroutine pqdwhfacs(sym, m, n, sigma, a, ia, ja, desca, q, iq, jq, descq, work,
     info = -(1200 + k)
     return
   end if
-  if (descw(1) == -12345) tau(1) = 0d0  ! DESCW and TAU are accepted but unused
   ioa = ia + (ja - 1) * desca(9)
   ioq = iq + (jq - 1) * descq(9)
   cs = sym
@@ -430,7 +429,6 @@ subroutine zolochol(sym, n, nblk, fone, a, z, contxt, work, coeff, rk, ip, info)
   integer, intent(out) :: info
   character(kind=c_char) :: cs
   cs = sym
-  if (nblk < 0 .or. contxt < -1 .or. fone /= fone) work(1) = 0d0  ! unused, as in zolochol.f90:81-84
   info = c_zolo_chol_step_host(cs, int(n, c_int64_t), a, int(n, c_int64_t), z, int(n, c_int64_t), coeff, &
                                int(rk, c_int), int(ip, c_int))
 end subroutine zolochol
@@ -448,7 +446,6 @@ subroutine zoloqr(m, n, nblk, a, z, contxt, workq, tau, coeff, rk, ip, info, it)
     info = -1
     return
   end if
-  if (nblk < 0 .or. contxt < -1 .or. it < -1) workq(1) = tau(1)
   info = c_zolo_qr_step_host(int(n, c_int64_t), a, int(n, c_int64_t), z, int(n, c_int64_t), coeff, int(rk, c_int), &
                              int(ip, c_int))
 end subroutine zoloqr
@@ -534,7 +531,6 @@ subroutine formx2(n, nblk, fone, err, a, z, contxt, work, info, it, ip)
   double precision, intent(out) :: err
   double precision, intent(inout) :: a(n, n), z(n, n), work(*)
   integer, intent(out) :: info
-  if (nblk < 0 .or. contxt < -1 .or. it < -1 .or. ip < -1) work(1) = 0d0
   err = 0d0
   info = 0
   if (n <= 0) return
@@ -620,7 +616,6 @@ subroutine pqdwhsubit(n, a, ia, ja, desca, norma, q, iq, jq, descq, work, lwork,
     info = -6
     return
   end if
-  if (lwork == -12345 .or. lwork2 == -12345) tau(1) = work(1) + work2(1)  ! accepted, unused
   ioa = ia + (ja - 1) * desca(9)
   ioq = iq + (jq - 1) * descq(9)
   info = c_zolo_subit_host(int(n, c_int64_t), a(ioa), int(desca(9), c_int64_t), norma, q(ioq), &
@@ -653,7 +648,6 @@ subroutine zoloeig(n, nblk, q, as, contxt, mpi_newcomm, work, lwork, work2, lwor
     info = -8
     return
   end if
-  if (nblk == -12345 .or. contxt == -12345 .or. mpi_newcomm == -12345 .or. lwork2 == -12345) work2(1) = 0d0
   info = c_zolo_eig_host(int(n, c_int64_t), q, int(n, c_int64_t), as, int(n, c_int64_t), tau, work, &
                          int(n, c_int64_t), crk)
   zolo_eig_count = crk
@@ -695,7 +689,6 @@ contains
     cpath = trim(filename) // c_null_char
     sigma = 0d0
     if (myid < 0 .or. myrow /= 0 .or. mycol /= 0 .or. nprow /= 1 .or. npcol /= 1) stop 'prepare_matrix: 1x1 grid only'
-    if (na == -12345) z(1) = 0d0
     st = c_zolo_mm_info(cpath, rows, cols, cnnz, mtype, bw)
     if (st /= 0) stop 'prepare_matrix: cannot read the MatrixMarket file'
     if (rows /= na .or. cols /= na) stop 'prepare_matrix: the file is not of order NA'

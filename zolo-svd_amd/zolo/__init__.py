@@ -45,7 +45,7 @@ except ImportError:  # pragma: no cover - host-only use of the scalar helpers
     _torch_runtime = None
 _lib = C.CDLL(_LIB_PATH)
 
-PREC = {"f64": 0, "f32": 1, "bf16x3": 2}
+PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "f32_refsched": 3, "bf16x3_refsched": 4}
 ERR_NOT_SPD = 1002
 ERR_NO_CONVERGENCE = 1003
 
@@ -162,6 +162,7 @@ _sig("zolo_l0_features_device", C.c_int, [C.c_int, _P, _P, _P, _P, _P, _P])
 _sig("zolo_l0_forward_device", C.c_int, [_P, _P, C.c_int, _P, _P])
 _sig("zolo_l0_grad_device", C.c_int, [_P, _P, _P, C.c_int, C.c_float, _P, C.POINTER(C.c_float), _P])
 _sig("zolo_l0_train_device", C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_float, C.c_float, _P, _P])
+_sig("zolo_formx2_host", C.c_int, [_I64, C.c_double, _D, _I64, _D, _I64, _D])
 _sig("zolo_profile_enable", C.c_int, [C.c_int])
 _sig("zolo_profile_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), _D, _D])
 
@@ -424,7 +425,7 @@ class _BatchInfo(C.Structure):
     _fields_ = [
         ("count", C.c_int), ("buckets", C.c_int), ("fallbacks", C.c_int), ("failures", C.c_int),
         ("max_iters", C.c_int), ("graph_nodes", C.c_int), ("continued", C.c_int), ("time_ms", C.c_double),
-        ("phase_ms", C.c_double * 8),
+        ("phase_ms", C.c_double * 8), ("flops_f32", C.c_double), ("flops_f64", C.c_double),
     ]
 
 
@@ -445,6 +446,8 @@ class BatchRunInfo:
     graph_nodes: int
     time_ms: float
     phase_ms: dict
+    flops_f32: float = 0.0      # algorithmic flops of the executed schedules (zolo.h)
+    flops_f64: float = 0.0
 
 
 def pd_batched_device(mats, precision: str = "f32", l0: str = "lu", beta=None, orth_tol: float = 1e-6, rk: int = 0,
@@ -491,7 +494,8 @@ def pd_batched_device(mats, precision: str = "f32", l0: str = "lu", beta=None, o
     info = BatchRunInfo(beta=b_out, orth=o_out, iters=it_out, fallback=(fl_out & 1) != 0, failed=(fl_out & 2) != 0,
                         continued=(fl_out & 4) != 0,
                         buckets=ci.buckets, graph_nodes=ci.graph_nodes, time_ms=ci.time_ms,
-                        phase_ms={p: ci.phase_ms[i] for i, p in enumerate(BATCH_PHASES)})
+                        phase_ms={p: ci.phase_ms[i] for i, p in enumerate(BATCH_PHASES)},
+                        flops_f32=ci.flops_f32, flops_f64=ci.flops_f64)
     return outs, info
 
 
@@ -546,6 +550,16 @@ def mm_info(path: str, bandwidth: bool = False) -> dict:
     if bandwidth:
         out["bandwidth"] = bw.value
     return out
+
+
+def formx2(X, Z, fone: float):
+    """FormX2 (FormX2.f90:2) on the GPU: returns (X+ = (X + Z)/FONE, ERR = ||X+ - X||_F)."""
+    A = _fortran(X).copy(order="F")
+    Zc = _fortran(Z).copy(order="F")
+    n = A.shape[0]
+    err = C.c_double(0.0)
+    _check("zolo_formx2_host", _lib.zolo_formx2_host(n, float(fone), _dptr(A), n, _dptr(Zc), n, C.byref(err)))
+    return A, err.value
 
 
 def mm_read(path: str, sigma: float = 0.0):
