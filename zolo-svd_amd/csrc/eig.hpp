@@ -36,8 +36,12 @@ int ke_fill(double* A, int64_t ld, int64_t m, int64_t n, double val, hipStream_t
 int ke_trd_column(double* A, int64_t ld, int64_t n, int64_t k0, int64_t i, double* W, double* v, double* tau, double* e,
                   TrdScratch& w, bool last_in_panel, hipStream_t s);
 int ke_trd_panel_end(double* A, int64_t ld, int64_t k0, int jb, int64_t n, double* d, hipStream_t s);
+// V (mrows x jb, ld ldv) = the reflectors of columns k0 .. k0+jb-1 of A, rows
+// r0 .. r0+mrows-1: column c has its unit head at row k0 + c + off and its
+// tail below it in A (off = 1 after the tridiagonalisation, the bandwidth
+// after the full -> band stage); zeros above the head.
 int ke_trd_extract_v(const double* A, int64_t ld, int64_t k0, int64_t r0, int jb, int64_t mrows, double* V,
-                     int64_t ldv, hipStream_t s);
+                     int64_t ldv, int off, hipStream_t s);
 int ke_dc_leaves(const double* d, const double* e, int64_t n, double* lam, double* Q, int64_t ldq, hipStream_t s);
 int ke_dc_level(const DcLevel& L, hipStream_t s);
 
@@ -58,10 +62,28 @@ int subit_run(int64_t n, double* A, int64_t lda, double normA, double* Q, int64_
 int zoloeig_run(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, double* w, double* V, int64_t ldv,
                 int* rk, double* err, int* it, hipStream_t s);
 
-// Apply Q = H_{k0} ... H_{k0+jb-1} (reflectors stored below the first subdiagonal
-// of A's columns k0.., heads at row k + 1, tau[k]) from the left to C (rows
-// k0+1 .. n-1, ncols columns): C <- Q C.  Used by the back-transformation.
+// Apply Q = H_{k0} ... H_{k0+jb-1} (reflectors stored in A's columns k0..,
+// heads at row k + off, tau[k]) from the left to C (rows k0+off .. n-1, ncols
+// columns): C <- Q C.  Used by the back-transformations (off = 1 for the
+// one-stage reduction, the bandwidth for the full -> band stage).
 int apply_trd_block(const Ctx& c, const double* A, int64_t lda, const double* tau, int64_t k0, int jb, int64_t n,
-                    double* C, int64_t ldc, int64_t ncols, Arena& ar);
+                    double* C, int64_t ldc, int64_t ncols, Arena& ar, int off = 1);
+
+// ---- two-stage reduction (sbr.hip) ----------------------------------------
+constexpr int kSbNb = 64;     // bandwidth of the full -> band stage
+constexpr int kSbGroup = 64;  // sweeps per block reflector of the bulge-chasing back-transformation
+// LB (ld ldb >= 2b, n columns) <- the lower band d <= b of H, zero beyond
+int ke_band_extract(const double* H, int64_t ldh, int64_t n, int b, double* LB, int64_t ldb, hipStream_t s);
+// d = diagonal, e = subdiagonal of the band store (e[n-1] = 0)
+int ke_band_diag(const double* LB, int64_t ldb, int64_t n, double* d, double* e, hipStream_t s);
+// Bulge chasing of the band store (bandwidth kSbNb) to tridiagonal form on nwg
+// workgroups; reflectors go to vstore / tstore in group layout (groups of g =
+// kSbGroup sweeps, kmax tasks per sweep at most), progress (n ints) and
+// status (1 int) must be zero on entry; status is 1 afterwards if a wait ran
+// out of its spin budget.
+int ke_sb2st(double* LB, int64_t ldb, int64_t n, int g, int kmax, double* vstore, double* tstore, int* progress,
+             int* status, int nwg, hipStream_t s);
+// T factors of the group block reflectors from G = V^T V (g x g per group)
+int ke_q2_form_t(const double* Gstore, const double* tstore, int ngroups, double* Tstore, hipStream_t s);
 
 }  // namespace zolo
