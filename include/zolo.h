@@ -138,6 +138,16 @@ int zolo_qr_step_host(int64_t n, const double* X, int64_t ldx, double* Z, int64_
 int zolo_normest_device(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt,
                         void* stream);
 int zolo_normest_host(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt);
+/* No counterpart in the reference.  zolo_normest_device for callers that
+ * cannot allocate device memory (graph capture, serving loops, harnesses that
+ * forbid hipMalloc): no allocation of any kind; the scratch (x, S x and the
+ * partial sums, 8 (m + n + m ceil(n/256) + 1040) bytes + alignment) lives in a
+ * fixed 16 MiB device array of the library, one per GPU, held until the
+ * process ends.  Larger m, n return -1 with a message.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (other streams or host
+ * threads) race; nothing in it needs to start at zero. */
+int zolo_normest_noalloc_device(int64_t m, int64_t n, const double* A, int64_t lda, double tol, double* est, int* cnt,
+                                void* stream);
 
 /* ---- Structured QR of the 2n x n stack [X; s I] on host arrays -----------------
  * MPDGEQRF (mpdgeqrf.f:5): S <- R + reflectors, tau out.

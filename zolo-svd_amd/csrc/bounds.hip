@@ -15,6 +15,7 @@
 // per-workgroup partials combined in a fixed order.
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "driver.hpp"
@@ -821,4 +822,50 @@ int normest_run(int64_t m, int64_t n, const double* A, int64_t lda, double tol, 
   return kOk;
 }
 
+// ---------------------------------------------------------------------------
+// PDNORMEST without allocation: the same power iteration with its scratch
+// (x, S x, the mat-vec partials and the norm partials) in a fixed device
+// array of this library instead of the workspace.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocBytes = size_t(16) << 20;  // 16 MiB per device
+__device__ __align__(256) char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+char* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each GPU
+}  // namespace
+
 }  // namespace zolo
+
+// No counterpart in the reference: zolo_normest_device (PDNORMEST,
+// pdnormest.f90:2) for callers that cannot allocate device memory.
+extern "C" int zolo_normest_noalloc_device(int64_t m, int64_t n, const double* A, int64_t lda, double tol,
+                                           double* est, int* cnt, void* stream) {
+  using namespace zolo;
+  if (m < 1) return -1;
+  if (n < 1) return -2;
+  if (!A) return -3;
+  if (lda < m) return -4;
+  if (!est) return -6;
+  if (!cnt) return -7;
+  if (normest_bytes(m, n) > kNoAllocBytes) {
+    set_error("zolo_normest_noalloc_device: m = " + std::to_string(m) + ", n = " + std::to_string(n) + " needs " +
+              std::to_string(normest_bytes(m, n)) + " bytes of scratch, more than the fixed " +
+              std::to_string(kNoAllocBytes));
+    return -1;
+  }
+  int dev = 0;
+  ZOLO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("zolo_normest_noalloc_device: device index out of range");
+    return kErrArg;
+  }
+  if (!g_scratch_ptr[dev]) {
+    // the array's own name is a host-side stand-in; kernels get this address
+    void* p = nullptr;
+    ZOLO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  Arena ar(g_scratch_ptr[dev], kNoAllocBytes);
+  return normest_run(m, n, A, lda, tol, est, cnt, ar, (hipStream_t)stream);
+}
+
