@@ -45,6 +45,19 @@ def test_fortran_driver_on_gpu():
 
 
 @pytest.mark.gpu
+def test_fortran_dropins_on_gpu():
+    """ZOLOCHOL, ZOLOQR, PDNORMEST, MPDGEQRF / MPDORGQR, FormX2, PZOLOPD2 and
+    PZOLOPD1 through their Fortran argument lists, submatrices at IA / JA > 1."""
+    exe = os.path.join(PKG, "test_dropins")
+    assert os.path.exists(exe), "build the Fortran driver first (make -C zolo-svd_amd)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout)
+    print(r.stderr)
+    assert r.returncode == 0
+    assert "PASS" in r.stdout.splitlines()[-1]
+
+
+@pytest.mark.gpu
 def test_fortran_spectral_division_driver_on_gpu(tmp_path):
     """pqdwheig.f90's flow through the Fortran drop-ins: prepare_real_symm_MM,
     PQDWHFACS with a shift, PQDWHSUBIT, ELPA_PDSYEVD2 on the Rk block, ZoloEig."""

@@ -55,6 +55,232 @@ std::vector<Level> plan_levels(int64_t n) {
   return lv;
 }
 
+// ---------------------------------------------------------------------------
+// Two-stage reduction (the ELPA 2-stage scheme of elpa_pdsyevd2.F90:199-200).
+// ---------------------------------------------------------------------------
+constexpr int64_t kTwoStageMin = 4 * kSbNb;  // smaller matrices take the one-stage reduction
+
+struct TwoStage {
+  int64_t n = 0, ldb = 0, ldzp = 0;
+  int J = 0, kmax = 0, ngroups = 0;
+  int64_t span = 0;
+  int64_t ncols1 = 0;                   // columns whose stage-1 reflectors are stored in H
+  double* tau1 = nullptr;               // stage-1 tau per column
+  double *LB = nullptr, *vstore = nullptr, *tstore = nullptr, *Gs = nullptr, *Ts = nullptr;
+  int *progress = nullptr, *status = nullptr;
+  double *Zp = nullptr, *W = nullptr, *W2 = nullptr;
+};
+
+size_t two_stage_bytes(int64_t n) {
+  const int64_t b = kSbNb, g = kSbGroup, span = b + g - 1;
+  const int64_t J = (n - 2 + g - 1) / g, kmax = (n - 2) / b + 1, ng = J * kmax;
+  size_t by = 0;
+  by += sizeof(double) * (size_t)n;                                   // tau1
+  by += sizeof(double) * (size_t)n * 2 * b;                           // LB
+  by += sizeof(double) * (size_t)ng * g * (span + 1 + 2 * g);         // vstore, tstore, G, T
+  by += sizeof(int) * ((size_t)n + 64);                               // progress, status
+  by += sizeof(double) * (size_t)(n + span + b + g) * n;              // padded eigenvectors
+  by += 2 * sizeof(double) * (size_t)J * g * n;                       // W, W2 of one wavefront
+  // stage 1: panel QR scratch and the block vectors of the two-sided update
+  by += dense_qr_bytes<double>(n) + sizeof(double) * (size_t)(3 * n * b + 4 * b * b + 256 * 256);
+  return by + 64 * 256;
+}
+
+// Full -> band (bandwidth b): for every panel of b columns, Householder QR of
+// the part below the band, then the two-sided update of the trailing matrix
+// A22 <- Q^T A22 Q with Q = I - V T V^T:  Y = A22 V T, M = V^T Y,
+// Y <- Y - V (T^T M)/2, A22 <- A22 - V Y^T - Y V^T.  The reflectors stay
+// below R in H; their tau in ts.tau1.
+int sy2sb(const Ctx& c, double* H, int64_t ldh, int64_t n, TwoStage& ts, Arena& ar) {
+  const int b = kSbNb;
+  const size_t mark = ar.used();
+  ts.ncols1 = 0;
+  for (int64_t k0 = 0; k0 + b + 1 < n; k0 += b) {
+    ar.reset();
+    ar.take<char>(mark);
+    const int64_t r0 = k0 + b, m = n - r0;
+    const int kk = (int)std::min<int64_t>(m, b);
+    double* P = H + r0 + k0 * ldh;
+    double* Tall = ar.take<double>(256 * 256);
+    double* V = ar.take<double>((size_t)m * b);
+    double* VT = ar.take<double>((size_t)m * b);
+    double* Y = ar.take<double>((size_t)m * b);
+    double* M = ar.take<double>((size_t)b * b);
+    double* Wm = ar.take<double>((size_t)b * b);
+    ZOLO_ARENA_OK(ar);
+    ZOLO_CHECK(dense_geqrf_cols<double>(c, P, ldh, m, kk, ts.tau1 + k0, Tall, ar));
+    ts.ncols1 = k0 + kk;
+    ZOLO_CHECK(ke_trd_extract_v(H, ldh, k0, r0, kk, m, V, m, b, c.s));
+    if (kk < b) {
+      // fewer rows than columns (the last panel): Q^T on the panel's other columns
+      GemmDesc d;  // Wm = V^T P2
+      d.transA = 1; d.M = kk; d.N = b - kk; d.K = m;
+      d.A = V; d.lda = m; d.B = P + kk * ldh; d.ldb = ldh; d.C = Wm; d.ldc = kk;
+      ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+      d = GemmDesc();  // M = T^T Wm
+      d.transA = 1; d.M = kk; d.N = b - kk; d.K = kk;
+      d.A = Tall; d.lda = 256; d.B = Wm; d.ldb = kk; d.C = M; d.ldc = kk; d.aHi = 0;
+      ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+      d = GemmDesc();  // P2 -= V M
+      d.M = m; d.N = b - kk; d.K = kk;
+      d.A = V; d.lda = m; d.B = M; d.ldb = kk; d.C = P + kk * ldh; d.ldc = ldh; d.alpha = -1.0; d.beta = 1.0;
+      ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    }
+    double* A22 = H + r0 + r0 * ldh;
+    GemmDesc d;  // VT = V T   (T upper: op(B)[k][j] = 0 for k > j)
+    d.M = m; d.N = kk; d.K = kk;
+    d.A = V; d.lda = m; d.B = Tall; d.ldb = 256; d.C = VT; d.ldc = m; d.bHi = 0;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    d = GemmDesc();  // Y = A22 VT
+    d.M = m; d.N = kk; d.K = m;
+    d.A = A22; d.lda = ldh; d.B = VT; d.ldb = m; d.C = Y; d.ldc = m;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    d = GemmDesc();  // M = V^T Y
+    d.transA = 1; d.M = kk; d.N = kk; d.K = m;
+    d.A = V; d.lda = m; d.B = Y; d.ldb = m; d.C = M; d.ldc = kk;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    d = GemmDesc();  // Wm = T^T M   (T^T lower: op(A)[i][k] = 0 for k > i)
+    d.transA = 1; d.M = kk; d.N = kk; d.K = kk;
+    d.A = Tall; d.lda = 256; d.B = M; d.ldb = kk; d.C = Wm; d.ldc = kk; d.aHi = 0;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    d = GemmDesc();  // Y -= V Wm / 2
+    d.M = m; d.N = kk; d.K = kk;
+    d.A = V; d.lda = m; d.B = Wm; d.ldb = kk; d.C = Y; d.ldc = m; d.alpha = -0.5; d.beta = 1.0;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    d = GemmDesc();  // A22 -= V Y^T
+    d.transB = 1; d.M = m; d.N = m; d.K = kk;
+    d.A = V; d.lda = m; d.B = Y; d.ldb = m; d.C = A22; d.ldc = ldh; d.alpha = -1.0; d.beta = 1.0;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    d.A = Y; d.B = V;  // A22 -= Y V^T
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+  }
+  ar.reset();
+  ar.take<char>(mark);
+  return kOk;
+}
+
+// Band -> tridiagonal (bulge chasing on the GPU), d and e out.
+int sb2st(const Ctx& c, const double* H, int64_t ldh, int64_t n, double* d, double* e, TwoStage& ts) {
+  const int b = kSbNb, g = kSbGroup;
+  ZOLO_CHECK(ke_band_extract(H, ldh, n, b, ts.LB, ts.ldb, c.s));
+  ZOLO_HIP(hipMemsetAsync(ts.progress, 0, sizeof(int) * ((size_t)n + 64), c.s));
+  ZOLO_HIP(hipMemsetAsync(ts.vstore, 0, sizeof(double) * (size_t)ts.ngroups * g * ts.span, c.s));
+  ZOLO_HIP(hipMemsetAsync(ts.tstore, 0, sizeof(double) * (size_t)ts.ngroups * g, c.s));
+  // concurrent sweeps: sweep i+1 trails sweep i by three tasks
+  const int nwg = (int)std::min<int64_t>(256, std::max<int64_t>(8, (n / b) / 3 + 8));
+  ZOLO_CHECK(ke_sb2st(ts.LB, ts.ldb, n, g, ts.kmax, ts.vstore, ts.tstore, ts.progress, ts.status, nwg, c.s));
+  int st = 0;
+  ZOLO_HIP(hipMemcpyAsync(&st, ts.status, sizeof(int), hipMemcpyDeviceToHost, c.s));
+  ZOLO_HIP(hipStreamSynchronize(c.s));
+  if (st != 0) {
+    set_error("sb2st: a sweep waited longer than its spin budget");
+    return kErrInternal;
+  }
+  return ke_band_diag(ts.LB, ts.ldb, n, d, e, c.s);
+}
+
+// Z <- Q1 Q2 Z (Z n x n, ld ldz): the bulge-chasing reflectors in block
+// reflectors of g sweeps at one task index, in anti-diagonal wavefronts of
+// mutually disjoint groups (one batched launch per GEMM and wavefront), then
+// the stage-1 reflectors in blocks of 256 columns, last block first.
+int back_two_stage(const Ctx& c, const double* H, int64_t ldh, int64_t n, double* Z, int64_t ldz, TwoStage& ts,
+                   Arena& ar) {
+  const int b = kSbNb, g = kSbGroup;
+  const int64_t ldp = ts.ldzp, span = ts.span;
+  // padded copy: the group row ranges reach past row n - 1 with zero V rows
+  ZOLO_CHECK(ke_fill(ts.Zp, ldp, ldp, n, 0.0, c.s));
+  ZOLO_HIP(hipMemcpy2DAsync(ts.Zp, sizeof(double) * ldp, Z, sizeof(double) * ldz, sizeof(double) * n, n,
+                            hipMemcpyDeviceToDevice, c.s));
+  if (ts.ngroups > 0 && n >= 3) {
+    GemmDesc d;  // G = V^T V per group
+    d.batch = ts.ngroups;
+    d.transA = 1; d.M = g; d.N = g; d.K = span;
+    d.A = ts.vstore; d.lda = span; d.strideA = (int64_t)g * span;
+    d.B = ts.vstore; d.ldb = span; d.strideB = (int64_t)g * span;
+    d.C = ts.Gs; d.ldc = g; d.strideC = (int64_t)g * g;
+    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    ZOLO_CHECK(ke_q2_form_t(ts.Gs, ts.tstore, ts.ngroups, ts.Ts, c.s));
+    const int J = ts.J, K = ts.kmax;
+    for (int t = 0; t <= J + K - 2; ++t) {
+      // groups (j, k = t - (J - 1 - j)) with 0 <= k < K and sweep j g of task k present
+      const int64_t base_k = (int64_t)t - (J - 1);
+      int64_t jlo = std::max<int64_t>(0, -base_k);
+      int64_t jhi = std::min<int64_t>(J - 1, K - 1 - base_k);
+      const int64_t lim = n - 2 - base_k * b;  // j (g + b) <= n - 2 - base_k b
+      if (lim < 0) continue;
+      jhi = std::min<int64_t>(jhi, lim / (g + b));
+      if (jhi < jlo) continue;
+      const int cnt = (int)(jhi - jlo + 1);
+      const int64_t k0 = base_k + jlo;
+      const int64_t grp0 = jlo * K + k0;
+      const int64_t row0 = jlo * g + 1 + k0 * b;
+      const int64_t sgrp = K + 1;  // group index stride along the wavefront
+      GemmDesc w;  // W = V^T Zp(rows)   (V(r, q) = 0 unless q <= r <= q + b - 1)
+      w.batch = cnt;
+      w.transA = 1; w.M = g; w.N = n; w.K = span;
+      w.A = ts.vstore + grp0 * g * span; w.lda = span; w.strideA = sgrp * g * span;
+      w.B = ts.Zp + row0; w.ldb = ldp; w.strideB = g + b;
+      w.C = ts.W; w.ldc = g; w.strideC = (int64_t)g * n;
+      w.aLo = 0; w.aHi = b - 1;
+      ZOLO_CHECK(gemm(Prec::F64, w, c.s));
+      GemmDesc t2;  // W2 = T W   (T upper)
+      t2.batch = cnt;
+      t2.M = g; t2.N = n; t2.K = g;
+      t2.A = ts.Ts + grp0 * g * g; t2.lda = g; t2.strideA = sgrp * g * g;
+      t2.B = ts.W; t2.ldb = g; t2.strideB = (int64_t)g * n;
+      t2.C = ts.W2; t2.ldc = g; t2.strideC = (int64_t)g * n;
+      t2.aLo = 0;
+      ZOLO_CHECK(gemm(Prec::F64, t2, c.s));
+      GemmDesc u;  // Zp(rows) -= V W2
+      u.batch = cnt;
+      u.M = span; u.N = n; u.K = g;
+      u.A = ts.vstore + grp0 * g * span; u.lda = span; u.strideA = sgrp * g * span;
+      u.B = ts.W2; u.ldb = g; u.strideB = (int64_t)g * n;
+      u.C = ts.Zp + row0; u.ldc = ldp; u.strideC = g + b;
+      u.alpha = -1.0; u.beta = 1.0;
+      u.aLo = b - 1; u.aHi = 0;
+      ZOLO_CHECK(gemm(Prec::F64, u, c.s));
+    }
+  }
+  // stage 1
+  const size_t mark = ar.used();
+  for (int64_t P = ((ts.ncols1 - 1) / 256) * 256; ts.ncols1 > 0 && P >= 0; P -= 256) {
+    const int jb = (int)std::min<int64_t>(256, ts.ncols1 - P);
+    ar.reset();
+    ar.take<char>(mark);
+    ZOLO_CHECK(apply_trd_block(c, H, ldh, ts.tau1, P, jb, n, ts.Zp, ldp, n, ar, b));
+  }
+  ar.reset();
+  ar.take<char>(mark);
+  ZOLO_HIP(hipMemcpy2DAsync(Z, sizeof(double) * ldz, ts.Zp, sizeof(double) * ldp, sizeof(double) * n, n,
+                            hipMemcpyDeviceToDevice, c.s));
+  return kOk;
+}
+
+int take_two_stage(Arena& ar, int64_t n, TwoStage& ts) {
+  const int64_t b = kSbNb, g = kSbGroup;
+  ts.n = n;
+  ts.ldb = 2 * b;
+  ts.span = b + g - 1;
+  ts.J = (int)((n - 2 + g - 1) / g);
+  ts.kmax = (int)((n - 2) / b + 1);
+  ts.ngroups = ts.J * ts.kmax;
+  ts.ldzp = n + ts.span + b + g;
+  ts.tau1 = ar.take<double>(n);
+  ts.LB = ar.take<double>((size_t)n * ts.ldb);
+  ts.vstore = ar.take<double>((size_t)ts.ngroups * g * ts.span);
+  ts.tstore = ar.take<double>((size_t)ts.ngroups * g);
+  ts.Gs = ar.take<double>((size_t)ts.ngroups * g * g);
+  ts.Ts = ar.take<double>((size_t)ts.ngroups * g * g);
+  ts.progress = ar.take<int>((size_t)n + 64);
+  ts.status = ts.progress + n;
+  ts.Zp = ar.take<double>((size_t)ts.ldzp * n);
+  ts.W = ar.take<double>((size_t)ts.J * g * n);
+  ts.W2 = ar.take<double>((size_t)ts.J * g * n);
+  ZOLO_ARENA_OK(ar);
+  return kOk;
+}
+
 }  // namespace
 
 size_t syevd_bytes(int64_t n) {
@@ -68,6 +294,7 @@ size_t syevd_bytes(int64_t n) {
   b += (sizeof(MergeH) + 3 * sizeof(int) + sizeof(double)) * (size_t)n;  // merges, counts, rho
   const size_t nbo = 256;
   b += sizeof(double) * ((size_t)n * nbo + 3 * nbo * nbo + 2 * nbo * (size_t)n);  // back-transformation
+  if (n >= kTwoStageMin) b += two_stage_bytes(n);
   return b + 64 * 256;
 }
 
@@ -97,7 +324,14 @@ int syevd_run(int64_t n, double* H, int64_t ldh, double* lambda, double* Z, int6
   ZOLO_ARENA_OK(ar);
   ZOLO_CHECK(ke_fill(W, n, n, kTrdNb, 0.0, s));
   ZOLO_CHECK(ke_fill(e, n, n, 1, 0.0, s));
-  for (int64_t k0 = 0; k0 < n; k0 += kTrdNb) {
+  const bool two = n >= kTwoStageMin;
+  TwoStage ts;
+  if (two) {
+    ZOLO_CHECK(take_two_stage(ar, n, ts));
+    ZOLO_CHECK(sy2sb(c, H, ldh, n, ts, ar));
+    ZOLO_CHECK(sb2st(c, H, ldh, n, d, e, ts));
+  }
+  for (int64_t k0 = 0; !two && k0 < n; k0 += kTrdNb) {
     const int jb = (int)std::min<int64_t>(kTrdNb, n - k0);
     for (int64_t i = k0; i < k0 + jb; ++i)
       ZOLO_CHECK(ke_trd_column(H, ldh, n, k0, i, W, v, tau, e, tw, i + 1 == k0 + jb, s));
@@ -210,7 +444,8 @@ int syevd_run(int64_t n, double* H, int64_t ldh, double* lambda, double* Z, int6
   const auto t2 = clk::now();
   // ---- 3. back-transformation: Z <- Q_trd Z ------------------------------------
   const size_t mark = ar.used();
-  const int64_t nref = n - 1;  // reflectors k = 0 .. n-2
+  if (two) ZOLO_CHECK(back_two_stage(c, H, ldh, n, Z, ldz, ts, ar));
+  const int64_t nref = two ? 0 : n - 1;  // reflectors k = 0 .. n-2
   for (int64_t P = ((nref - 1) / 256) * 256; nref > 0 && P >= 0; P -= 256) {
     const int jb = (int)std::min<int64_t>(256, nref - P);
     ar.reset();
