@@ -14,11 +14,12 @@
 // task k-1 filled in; H is applied from the left to the rest of E_{k-1}, on
 // both sides of the diagonal block D_k = A(R_k, R_k), and from the right to
 // E_k = A(R_{k+1}, R_k), which fills it in for task k+1.  Task (i+1, k)
-// touches entries that task (i, k+2) also touches and none of those of tasks
-// (i, k+3) and later, so sweep i+1 may run task k as soon as sweep i has
-// finished task k+2: the sweeps form a pipeline.  One workgroup runs one
-// sweep at a time (sweeps i, i + W, i + 2W, ... for workgroup i of W), keeps
-// E_k in LDS from task k to task k+1, and publishes its progress per sweep.
+// overlaps task (i, k+1) and shares one entry with task (i, k+2) (the head
+// of the column task k+2 clears) and nothing with later tasks of sweep i, so
+// the sweeps form a pipeline (see the kernel for the lag).  One workgroup
+// runs one sweep at a time (sweeps i, i + W, i + 2W, ... for workgroup i of
+// W), keeps E_k in LDS from task k to task k+1, and publishes its progress
+// per sweep.
 #include <algorithm>
 
 #include "eig.hpp"
@@ -74,14 +75,22 @@ struct SbArgs {
 // loads them (all loads of a task in flight at once), updates them in
 // registers and stores them back.  The three products a task needs (D v,
 // E_k v and v^T E_{k-1}) are partial sums over a quarter of the block per
-// wave, combined through LDS: five barriers and three wave reductions per task.
+// wave, combined through LDS.
+//
+// The reflector of task k+1 depends only on the first column of E_k, which is
+// final once task k has applied H from the right (no other sweep touches it
+// in between), so task k computes it at its end, stores the cleared column
+// (beta e1) and only then publishes its progress.  The one entry that task
+// (i+1, k) shares with task (i, k+2) is that column's head, so sweep i+1 may
+// run task k once sweep i has published k + 2 (two tasks of lag instead of
+// three).
 template <int B>
 __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
   static_assert(B == 64, "one lane per block row");
   constexpr int R = B / 4;            // block columns per thread
   __shared__ double blk[3][B * kLd];  // two E blocks (carried and new) and D (both triangles)
-  __shared__ double vs[B], ws[B], qs[B], es[B], part[3][4][B];
-  __shared__ double misc[2];
+  __shared__ double vs[2][B], ws[B], qs[B], es[B], part[3][4][B];
+  __shared__ double misc[2][2];       // tau, beta of the current and the next reflector
   __shared__ int flag;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t n = a.n, ldb = a.ldb;
@@ -94,15 +103,17 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
     double* Dm = blk[2];
     const int j = i / a.g, q = i % a.g;
     for (int k = 0; k < K; ++k) {
+      const int cur = k & 1;
       const int64_t s = i + 1 + (int64_t)k * B;
       const int L = (int)std::min<int64_t>(B, n - s);
       const int64_t s2 = s + B;
       const int L2 = s2 < n ? (int)std::min<int64_t>(B, n - s2) : 0;
-      // ---- wait until sweep i-1 has finished task k+2 (or all of its tasks)
+      // ---- wait until sweep i-1 has published task k+1 (finished, with the
+      // cleared column of its task k+2)
       if (threadIdx.x == 0) {
         int ok = 1;
         if (i > 0) {
-          const int need = std::min(k + 3, Kprev);
+          const int need = std::min(k + 2, Kprev);
           long long spins = 0;
           while (__hip_atomic_load(a.progress + (i - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
             if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
@@ -119,8 +130,8 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
       }
       __syncthreads();  // (1)
       if (!flag) return;
-      // ---- loads: D_k (lower triangle) and E_k in registers, the column of
-      // sweep i (k = 0) or the first column of the carried E_{k-1}
+      // ---- loads: D_k (lower triangle) and E_k in registers; for k = 0 the
+      // column of sweep i and its reflector
       double dreg[R], ereg[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -131,7 +142,7 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
         ereg[r] = ine ? LB[(s + u) * ldb + (B + lane - u)] : 0.0;
       }
       double x = 0.0;
-      if (wv == 0 && lane < L) x = (k == 0) ? LB[(int64_t)i * ldb + 1 + lane] : Ep[lane];
+      if (k == 0 && wv == 0 && lane < L) x = LB[(int64_t)i * ldb + 1 + lane];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int u = wv + 4 * r;
@@ -141,8 +152,8 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
         }
         En[u * kLd + lane] = ereg[r];
       }
-      // ---- reflector (wave 0): H x = beta e1, LAPACK dlarfg without rescaling
-      if (wv == 0) {
+      if (k == 0 && wv == 0) {
+        // H x = beta e1, LAPACK dlarfg without rescaling
         const double alpha = __shfl(x, 0, 64);
         const double sig = wsum(lane > 0 ? x * x : 0.0);
         double tau = 0.0, beta = alpha, v = lane == 0 ? 1.0 : 0.0;
@@ -151,36 +162,34 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
           tau = (beta - alpha) / beta;
           v = lane == 0 ? 1.0 : x * (1.0 / (alpha - beta));
         }
-        vs[lane] = lane < L ? v : 0.0;
+        vs[0][lane] = lane < L ? v : 0.0;
+        if (lane < L) LB[(int64_t)i * ldb + 1 + lane] = lane == 0 ? beta : 0.0;
         if (lane == 0) {
-          misc[0] = tau;
-          misc[1] = beta;
+          misc[0][0] = tau;
+          misc[0][1] = beta;
+          a.tstore[(int64_t)j * a.kmax * a.g + q] = tau;
         }
+        if (lane < L) a.vstore[((int64_t)j * a.kmax * a.g + q) * a.span + q + lane] = vs[0][lane];
       }
       __syncthreads();  // (2)
-      const double tau = misc[0], beta = misc[1];
-      const double vl = vs[lane];
-      // ---- the cleared column (k = 0), the reflector store, and the partial
-      // sums of D v, E_k v (rows: lane; this wave's columns) and v^T E_{k-1}
-      // (columns: lane; this wave's quarter of the rows)
-      if (k == 0 && wv == 0 && lane < L) LB[(int64_t)i * ldb + 1 + lane] = lane == 0 ? beta : 0.0;
-      if (wv == 1 && lane < L) {
-        const int64_t grp = (int64_t)j * a.kmax + k;
-        a.vstore[(grp * a.g + q) * a.span + q + lane] = vl;
-        if (lane == 0) a.tstore[grp * a.g + q] = tau;
-      }
+      const double tau = misc[cur][0], beta = misc[cur][1];
+      const double vl = vs[cur][lane];
+      double vu[R];  // v at this thread's block columns
+#pragma unroll
+      for (int r = 0; r < R; ++r) vu[r] = vs[cur][wv + 4 * r];
+      // ---- partial sums of D v, E_k v (rows: lane; this wave's columns) and
+      // v^T E_{k-1} (columns: lane; this wave's quarter of the rows)
       {
         double p = 0.0, rr = 0.0, ee = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int u = wv + 4 * r;
-          const double vu = vs[u];
           if (u < L) {
-            if (lane < L) p += Dm[u * kLd + lane] * vu;
-            rr += ereg[r] * vu;
+            if (lane < L) p += Dm[u * kLd + lane] * vu[r];
+            rr += ereg[r] * vu[r];
           }
           const int t = wv * R + r;
-          if (k > 0) ee += vs[t] * Ep[lane * kLd + t];
+          if (k > 0) ee += vs[cur][t] * Ep[lane * kLd + t];
         }
         part[0][wv][lane] = p;
         part[1][wv][lane] = rr;
@@ -199,19 +208,44 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
         es[lane] = tau * (part[2][0][lane] + part[2][1][lane] + part[2][2][lane] + part[2][3][lane]);
       }
       __syncthreads();  // (4)
-      // ---- D <- H D H (own lower entries, stored), E_k <- E_k H (kept in LDS
-      // for task k+1), E_{k-1} <- H E_{k-1} (column 0 is beta e1), stored
+      // ---- D <- H D H (own lower entries, stored), E_{k-1} <- H E_{k-1}
+      // (column 0 is beta e1) stored, E_k <- E_k H kept in LDS for task k+1;
+      // wave 0 holds column 0 of E_k and takes the next reflector from it
       const double wl = ws[lane], ql = qs[lane];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (wv + 4 * r < L && lane < L2) ereg[r] -= ql * vu[r];
+      if (wv == 0 && L2 > 0) {
+        const int nx = cur ^ 1;
+        const double xx = lane < L2 ? ereg[0] : 0.0;
+        const double alpha = __shfl(xx, 0, 64);
+        const double sig = wsum(lane > 0 ? xx * xx : 0.0);
+        double tn = 0.0, bn = alpha, v = lane == 0 ? 1.0 : 0.0;
+        if (sig > 0.0) {
+          bn = -copysign(sqrt(alpha * alpha + sig), alpha);
+          tn = (bn - alpha) / bn;
+          v = lane == 0 ? 1.0 : xx * (1.0 / (alpha - bn));
+        }
+        vs[nx][lane] = lane < L2 ? v : 0.0;
+        ereg[0] = lane == 0 ? bn : 0.0;
+        if (lane < L2) LB[s * ldb + B + lane] = ereg[0];   // column 0 of E_k, cleared
+        const int64_t grp = (int64_t)j * a.kmax + (k + 1);
+        if (lane < L2) a.vstore[(grp * a.g + q) * a.span + q + lane] = v;
+        if (lane == 0) {
+          misc[nx][0] = tn;
+          misc[nx][1] = bn;
+          a.tstore[grp * a.g + q] = tn;
+        }
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int u = wv + 4 * r;
-        if (u < L && lane >= u && lane < L)
-          LB[(s + u) * ldb + (lane - u)] = dreg[r] - (vl * ws[u] + wl * vs[u]);
-        if (u < L && lane < L2) En[u * kLd + lane] = ereg[r] - ql * vs[u];
+        if (u < L && lane >= u && lane < L) LB[(s + u) * ldb + (lane - u)] = dreg[r] - (vl * ws[u] + wl * vu[r]);
         if (k > 0 && lane < L) {
           const double y = u == 0 ? (lane == 0 ? beta : 0.0) : Ep[u * kLd + lane] - vl * es[u];
           LB[(s - B + u) * ldb + (B + lane - u)] = y;
         }
+        En[u * kLd + lane] = ereg[r];
       }
       __syncthreads();  // (5)
       if (threadIdx.x == 0)
@@ -227,10 +261,10 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
 // GEMM engine): V is span x g with column q the reflector of sweep j g + q at
 // rows q .., and H_0 H_1 ... H_{g-1} = I - V T V^T with T upper triangular
 // (LAPACK dlarft, forward, columnwise): T(q, q) = tau_q and T(0:q, q) =
-// -tau_q T(0:q, 0:q) G(0:q, q).  One wave per group, lane p holds row p.
+// -tau_q T(0:q, 0:q) G(0:q, q).  One workgroup per group, thread p holds row p.
 template <int G>
-__global__ __launch_bounds__(64) void q2_form_t_kernel(const double* Gstore, const double* tstore, int ngroups,
-                                                       double* Tstore) {
+__global__ __launch_bounds__(G) void q2_form_t_kernel(const double* Gstore, const double* tstore, int ngroups,
+                                                      double* Tstore) {
   __shared__ double Tl[G * (G + 1)];
   const int grp = blockIdx.x;
   if (grp >= ngroups) return;
@@ -243,7 +277,7 @@ __global__ __launch_bounds__(64) void q2_form_t_kernel(const double* Gstore, con
     double y = 0.0;
     for (int c = p; c < qq; ++c) y += Tl[c * (G + 1) + p] * Gm[qq * G + c];
     Tl[qq * (G + 1) + p] = p < qq ? -tq * y : (p == qq ? tq : 0.0);
-    __builtin_amdgcn_wave_barrier();
+    __syncthreads();
   }
   for (int c = 0; c < G; ++c) Tm[c * G + p] = Tl[c * (G + 1) + p];
 }
@@ -295,8 +329,8 @@ int ke_sb2st(double* LB, int64_t ldb, int64_t n, int g, int kmax, double* vstore
 
 int ke_q2_form_t(const double* Gstore, const double* tstore, int ngroups, double* Tstore, hipStream_t s) {
   if (ngroups <= 0) return kOk;
-  static_assert(kSbGroup == 64, "one lane per row of T");
-  hipLaunchKernelGGL((q2_form_t_kernel<kSbGroup>), dim3(ngroups), dim3(64), 0, s, Gstore, tstore, ngroups, Tstore);
+  hipLaunchKernelGGL((q2_form_t_kernel<kSbGroup>), dim3(ngroups), dim3(kSbGroup), 0, s, Gstore, tstore, ngroups,
+                     Tstore);
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }
