@@ -56,6 +56,7 @@ struct BatchOptions {
   int tile = -1;                   // GEMM tile: -1 autotune per bucket, 0 automatic, 1..4 fixed
   bool use_graph = true;           // capture each phase into one hipGraph
   int continue_rounds = 2;         // restarts of matrices still off after their planned steps
+  bool one_bucket = false;         // all matrices padded to the largest bucket (few-matrix recomputations)
 };
 
 struct BatchResult {

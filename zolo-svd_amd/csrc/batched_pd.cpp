@@ -467,8 +467,13 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
   c.low = opt.precision == 2 ? Prec::BF16X3 : Prec::F32;
 
   // buckets (chunks of at most kMaxPerBucket matrices)
+  // A recomputation of a few matrices (fallback, restart) pads them all to
+  // the largest order among them: one bucket, so one set of launches instead
+  // of one per size.
   std::map<int64_t, std::vector<int>> by_nb;
-  for (int i = 0; i < count; ++i) by_nb[bucket_of(n[i])].push_back(i);
+  int64_t nb_max = 0;
+  for (int i = 0; i < count; ++i) nb_max = std::max(nb_max, bucket_of(n[i]));
+  for (int i = 0; i < count; ++i) by_nb[opt.one_bucket ? nb_max : bucket_of(n[i])].push_back(i);
   std::vector<Bucket> buckets;
   for (auto& kv : by_nb)
     for (size_t o = 0; o < kv.second.size(); o += kMaxPerBucket) {
@@ -723,6 +728,7 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
       BatchOptions o2 = opt;
       o2.l0_source = 0;
       o2.beta_in = nullptr;
+      o2.one_bucket = (int)redo.size() < kDirectBelow;
       // a few matrices: launching directly costs less than capturing and
       // instantiating a graph for them
       if ((int)redo.size() < kDirectBelow) o2.use_graph = false;
@@ -767,6 +773,7 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
     oc.l0_source = 0;
     oc.beta_in = nullptr;
     oc.continue_rounds = 0;
+    oc.one_bucket = (int)redo.size() < kDirectBelow;
     if ((int)redo.size() < kDirectBelow) oc.use_graph = false;
     BatchResult r3;
     ZOLO_CHECK(pd_batched_run((int)redo.size(), rn.data(), rA.data(), rld.data(), rU.data(), rld.data(), oc, r3, s));

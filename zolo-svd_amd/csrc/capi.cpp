@@ -14,14 +14,6 @@
 #include "driver.hpp"
 #include "eig.hpp"
 
-namespace zolo {
-namespace {
-thread_local std::string g_err;
-}
-void set_error(const std::string& msg) { g_err = msg; }
-const char* last_error() { return g_err.c_str(); }
-}  // namespace zolo
-
 using namespace zolo;
 
 namespace {

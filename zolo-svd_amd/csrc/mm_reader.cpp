@@ -13,9 +13,9 @@
 //
 // Fields 'real', 'integer' and 'pattern' (value 1) are read; 'complex' and the
 // dense 'array' form are refused (the reference's assembly uses only the
-// real values and the row/column indices of coordinate files).  Parsing runs
-// on the host; the device form uploads only the de-duplicated triplets and
-// scatters and symmetrises them on the GPU.
+// real values and the row/column indices of coordinate files).  Parsing and
+// the host assembly are plain host code (no HIP calls); the device assembly
+// is in mm_device.cpp.
 #include <algorithm>
 #include <cctype>
 #include <cstdio>
@@ -24,19 +24,10 @@
 #include <string>
 #include <vector>
 
-#include "common.hpp"
-#include "kernels.hpp"
+#include "mm_reader.hpp"
 
 namespace zolo {
 namespace {
-
-struct MmFile {
-  int64_t rows = 0, cols = 0, nnz = 0;
-  int mtype = 2;   // 0 symmetric, 1 general, 2 other (prepare_matrix.f90:230-241)
-  bool pattern = false;
-  std::vector<int64_t> i, j;  // 1-based as in the file
-  std::vector<double> v;
-};
 
 std::string lower(std::string s) {
   for (auto& ch : s) ch = (char)std::tolower((unsigned char)ch);
@@ -85,6 +76,8 @@ int mm_error(int code, const std::string& msg) {
   set_error("MatrixMarket: " + msg);
   return code;
 }
+
+}  // namespace
 
 // Header and size line; with entries = true also every entry.
 int mm_parse(const char* path, bool entries, MmFile& m) {
@@ -166,7 +159,7 @@ int mm_parse(const char* path, bool entries, MmFile& m) {
 
 // Keep the last entry of every repeated position (file order decides, as the
 // reference's sequential stores); result sorted by column, then row.
-void dedupe_last(MmFile& m) {
+void mm_dedupe_last(MmFile& m) {
   const size_t z = m.v.size();
   std::vector<size_t> idx(z);
   for (size_t k = 0; k < z; ++k) idx[k] = k;
@@ -186,14 +179,12 @@ void dedupe_last(MmFile& m) {
   m.i.swap(i2); m.j.swap(j2); m.v.swap(v2);
 }
 
-int check_dense(const MmFile& m, double* A, int64_t lda) {
+int mm_check_dense(const MmFile& m, double* A, int64_t lda) {
   if (m.rows != m.cols) return mm_error(-6, "matrix is not square");
   if (!A) return mm_error(-7, "no output array");
   if (lda < std::max<int64_t>(m.rows, 1)) return mm_error(-8, "lda smaller than the order");
   return kOk;
 }
-
-}  // namespace
 
 int mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* mtype, int64_t* bandwidth) {
   MmFile m;
@@ -213,7 +204,7 @@ int mm_info(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int* m
 int mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* mtype) {
   MmFile m;
   ZOLO_CHECK(mm_parse(path, true, m));
-  ZOLO_CHECK(check_dense(m, A, lda));
+  ZOLO_CHECK(mm_check_dense(m, A, lda));
   const int64_t n = m.rows;
   for (int64_t c = 0; c < n; ++c) std::fill(A + c * lda, A + c * lda + n, 0.0);
   for (size_t k = 0; k < m.v.size(); ++k) A[(m.i[k] - 1) + (m.j[k] - 1) * lda] = m.v[k];
@@ -227,40 +218,6 @@ int mm_dense_host(const char* path, double sigma, double* A, int64_t lda, int* m
       }
     for (int64_t d = 0; d < n; ++d) A[d + d * lda] = (A[d + d * lda] + A[d + d * lda]) / 2.0 - sigma;
   }
-  if (mtype) *mtype = m.mtype;
-  return kOk;
-}
-
-int mm_dense_device(const char* path, double sigma, double* A, int64_t lda, int* mtype, hipStream_t s) {
-  MmFile m;
-  ZOLO_CHECK(mm_parse(path, true, m));
-  ZOLO_CHECK(check_dense(m, A, lda));
-  dedupe_last(m);
-  const int64_t n = m.rows, z = (int64_t)m.v.size();
-  ZOLO_CHECK(k_fill<double>(A, lda, 0, n, n, 0.0, 1, s));
-  if (z > 0) {
-    // triplets: 0-based row and column (int64) and the value
-    std::vector<int64_t> ij((size_t)2 * z);
-    for (int64_t k = 0; k < z; ++k) {
-      ij[(size_t)k] = m.i[(size_t)k] - 1;
-      ij[(size_t)(z + k)] = m.j[(size_t)k] - 1;
-    }
-    char* d = nullptr;
-    const size_t bytes = sizeof(int64_t) * 2 * (size_t)z + sizeof(double) * (size_t)z;
-    ZOLO_HIP(hipMalloc(&d, bytes));
-    struct Free {
-      char* p;
-      ~Free() { (void)hipFree(p); }
-    } fr{d};
-    int64_t* dij = reinterpret_cast<int64_t*>(d);
-    double* dv = reinterpret_cast<double*>(d + sizeof(int64_t) * 2 * (size_t)z);
-    ZOLO_HIP(hipMemcpyAsync(dij, ij.data(), sizeof(int64_t) * 2 * (size_t)z, hipMemcpyHostToDevice, s));
-    ZOLO_HIP(hipMemcpyAsync(dv, m.v.data(), sizeof(double) * (size_t)z, hipMemcpyHostToDevice, s));
-    ZOLO_CHECK(k_scatter(A, lda, dij, dij + z, dv, z, s));
-    // the host arrays and the device triplets must outlive the copies and the scatter
-    ZOLO_HIP(hipStreamSynchronize(s));
-  }
-  if (m.mtype == 0) ZOLO_CHECK(k_mm_symmetrize(A, lda, n, sigma, s));
   if (mtype) *mtype = m.mtype;
   return kOk;
 }
