@@ -197,7 +197,7 @@ def gemm_classes(zl):
     for prec in (0, 1, 2):
         for ta in (0, 1):
             for tb in (0, 1):
-                for tm, tn in ((128, 128), (64, 128), (128, 64), (64, 64), (256, 128)):
+                for tm, tn in ((128, 128), (64, 128), (128, 64), (64, 64), (256, 256)):
                     cnt, ms, fl = zl.profile_gemm(prec, ta, tb, tm, tn)
                     if cnt:
                         exec_fl = fl * (3 if prec == 2 else 1)
@@ -212,7 +212,7 @@ def roofline(zl):
     for prec in (0, 1, 2):
         for ta in (0, 1):
             for tb in (0, 1):
-                for tm, tn in ((128, 128), (64, 128), (128, 64), (64, 64), (256, 128)):
+                for tm, tn in ((128, 128), (64, 128), (128, 64), (64, 64), (256, 256)):
                     cnt, ms, fl = zl.profile_gemm(prec, ta, tb, tm, tn)
                     if cnt and (best is None or ms > best[0]):
                         best = (ms, cnt, fl, prec, ta, tb, tm, tn)

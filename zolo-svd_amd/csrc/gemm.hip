@@ -10,7 +10,7 @@
 //  * BF16X3 : fp32 operands split once per call into hi + lo bfloat16 halves
 //             (a tiled layout whose K slabs are contiguous), three
 //             v_mfma_f32_16x16x32_bf16 products (lo*hi + hi*lo + hi*hi)
-//             accumulated in fp32; 256x128x32 tile of 8 waves.
+//             accumulated in fp32; 256x256x32 tile of 8 waves of 128x64.
 //
 // Work distribution: tiles are numbered so that the 8 XCDs (round-robin
 // workgroup dispatch) each own a contiguous block of tiles, and inside a block
@@ -438,8 +438,14 @@ __device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((u
 constexpr int kSplitTile = 64;
 constexpr int kB3K = 32;            // K depth of one bf16x3 slab and of a layout tile
 constexpr int kB3Rows = 128;        // rows of a layout tile
-constexpr int kB3Pitch = kB3K + 8;  // LDS row pitch in bf16 elements (80 bytes: conflict-free 16-byte reads)
-constexpr int kB3M = 256, kB3N = 128, kB3Threads = 512;  // workgroup tile: 8 waves of 64 x 64
+// LDS rows of a slab are 64 bytes (4 chunks of 16 bytes) with the chunk index
+// XOR-swizzled by bits 1..2 of the row: the 16-byte fragment reads (16 rows x
+// 4 chunks per wave) and the 16-byte tile writes (16 rows x 4 chunks) then
+// touch 64 distinct banks per 256-byte LDS cycle.  A padded 80-byte pitch
+// conflicts two ways on both (counter probe: 8 instead of 4 LDS cycles per
+// wave-wide 16-byte read, 16 instead of 8 per write).
+__device__ inline int b3_chunk(int row, int c) { return c ^ ((row >> 1) & 3); }
+constexpr int kB3M = 256, kB3N = 256, kB3Threads = 512;  // workgroup tile: 8 waves of 128 x 64
 
 // Offset of element (r, k) of a split operand in its tiled layout: tiles of
 // 128 rows x 32 k (8 KB of bf16, row-major inside), ordered k-fastest within a
@@ -493,19 +499,22 @@ __global__ __launch_bounds__(kThreads) void bf16_split_kernel(const float* __res
   L[1] = make_uint4(pl[4], pl[5], pl[6], pl[7]);
 }
 
-// BF16x3 GEMM on pre-split operands in the tiled layout: op(A) rows (hi/lo,
-// padded to 256-row multiples) and op(B) columns (padded to 128).  256 x 128
-// workgroup tile of 8 waves (each 64 x 64), K slabs of 32 double-buffered in
-// LDS with a register stage (slab k+2 loads in flight while slab k computes),
-// one LDS-only barrier per slab.  Per 16x16 block: three
-// v_mfma_f32_16x16x32_bf16 (lo*hi + hi*lo + hi*hi), fp32 accumulation.
+// BF16x3 GEMM on pre-split operands in the tiled layout: op(A) rows and op(B)
+// columns (hi/lo, both padded to 256-multiples).  256 x 256 workgroup tile of
+// 8 waves, each 128 x 64 (8 x 4 blocks of 16 x 16), so that a slab's 96
+// MFMAs per wave amortise the slab's barrier and LDS latency; K slabs of 32
+// double-buffered in LDS with a register stage (slab k+2 loads in flight while
+// slab k computes), one LDS-only barrier per slab.  Per 16x16 block: three
+// v_mfma_f32_16x16x32_bf16 (lo*hi + hi*lo + hi*hi), fp32 accumulation; the
+// products with hi(A) go first, then those with lo(A), so only one half of the
+// A fragments is live at a time.
 __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, const unsigned short* __restrict__ Ahi,
                                                                  const unsigned short* __restrict__ Alo, int64_t sAs,
                                                                  const unsigned short* __restrict__ Bhi,
                                                                  const unsigned short* __restrict__ Blo, int64_t sBs,
                                                                  int64_t Kp) {
-  __shared__ __attribute__((aligned(16))) unsigned short LA[2][2][kB3M][kB3Pitch];  // [stage][hi lo][row][k]
-  __shared__ __attribute__((aligned(16))) unsigned short LB[2][2][kB3N][kB3Pitch];
+  __shared__ __attribute__((aligned(16))) unsigned short LA[2][2][kB3M][kB3K];  // [stage][hi lo][row][k, swizzled]
+  __shared__ __attribute__((aligned(16))) unsigned short LB[2][2][kB3N][kB3K];
 
   const TileCoord tc = tile_of<kB3M, kB3N>(d.M, d.N, d.lowerC, banded(d));
   if (tc.skip) return;
@@ -524,35 +533,36 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave >> 2, wn = wave & 3;  // wave tile: rows wm*128 .. +128, columns wn*64 .. +64
 
-  f4 acc[4][4];
+  f4 acc[8][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
 
-  // A slab: 2 layout tiles (1024 chunks of 16 B per half), B slab: 1 tile (512
-  // chunks); thread t takes A chunks t and t + 512 and B chunk t.  Chunk e of a
-  // tile is row e / 4, k (e % 4) * 8.
-  // The register stage as named vectors (an array of them was kept on the
-  // scratch stack by the compiler).
+  // A and B slabs: 2 layout tiles each (512 chunks of 16 B per tile and
+  // half); thread t takes chunk t of each.  Chunk e of a tile is row e / 4,
+  // k (e % 4) * 8.  The register stage as named vectors (an array of them was
+  // kept on the scratch stack by the compiler).
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-  u32x4 ah0, ah1, al0, al1, bh0, bl0;
+  u32x4 ah0, ah1, al0, al1, bh0, bh1, bl0, bl1;
   const int64_t tiles_k = Kp / kB3K;
-  const int64_t oa0 = (i0 / kB3Rows) * tiles_k * (kB3Rows * kB3K) + (int64_t)t * 8;
-  const int64_t oa1 = oa0 + tiles_k * (kB3Rows * kB3K);
-  const int64_t ob0 = (j0 / kB3Rows) * tiles_k * (kB3Rows * kB3K) + (int64_t)t * 8;
+  const int64_t tile_elems = tiles_k * (kB3Rows * kB3K);  // one 128-row block over all of K
+  const int64_t oa0 = (i0 / kB3Rows) * tile_elems + (int64_t)t * 8;
+  const int64_t ob0 = (j0 / kB3Rows) * tile_elems + (int64_t)t * 8;
   auto load_tiles = [&](int64_t k0) {
     const int64_t ko = (k0 / kB3K) * (kB3Rows * kB3K);
     ah0 = *reinterpret_cast<const u32x4*>(Ah + oa0 + ko);
     al0 = *reinterpret_cast<const u32x4*>(Al + oa0 + ko);
-    ah1 = *reinterpret_cast<const u32x4*>(Ah + oa1 + ko);
-    al1 = *reinterpret_cast<const u32x4*>(Al + oa1 + ko);
+    ah1 = *reinterpret_cast<const u32x4*>(Ah + oa0 + tile_elems + ko);
+    al1 = *reinterpret_cast<const u32x4*>(Al + oa0 + tile_elems + ko);
     bh0 = *reinterpret_cast<const u32x4*>(Bh + ob0 + ko);
     bl0 = *reinterpret_cast<const u32x4*>(Bl + ob0 + ko);
+    bh1 = *reinterpret_cast<const u32x4*>(Bh + ob0 + tile_elems + ko);
+    bl1 = *reinterpret_cast<const u32x4*>(Bl + ob0 + tile_elems + ko);
   };
-  const int lrow = t >> 2, lk = (t & 3) * 8;
+  const int lrow = t >> 2, lk = b3_chunk(lrow, t & 3) * 8;  // rows lrow and 128 + lrow swizzle alike
   auto store_tiles = [&](int b) {
     *reinterpret_cast<u32x4*>(&LA[b][0][lrow][lk]) = ah0;
     *reinterpret_cast<u32x4*>(&LA[b][1][lrow][lk]) = al0;
@@ -560,32 +570,58 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
     *reinterpret_cast<u32x4*>(&LA[b][1][kB3Rows + lrow][lk]) = al1;
     *reinterpret_cast<u32x4*>(&LB[b][0][lrow][lk]) = bh0;
     *reinterpret_cast<u32x4*>(&LB[b][1][lrow][lk]) = bl0;
+    *reinterpret_cast<u32x4*>(&LB[b][0][kB3Rows + lrow][lk]) = bh1;
+    *reinterpret_cast<u32x4*>(&LB[b][1][kB3Rows + lrow][lk]) = bl1;
   };
-  auto compute = [&](int b) {
-    const int kc = (lane >> 4) * 8;
-    bf8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int r = wm * 64 + m * 16 + (lane & 15);
-      ah[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LA[b][0][r][kc]));
-      al[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LA[b][1][r][kc]));
-    }
+  // One slab from LDS buffer b: the fragments of B and hi(A), the 64 hi(A)
+  // products with the LDS stores of the staged slab k+1 (st) and the global
+  // loads of slab k+2 (ld) spread among them, then lo(A) and its 32
+  // products.  Spreading the stores keeps the other wave of the SIMD issuing
+  // MFMAs while one waits for its staged loads (the stores bunched after the
+  // MFMAs cost about 10 % of the kernel's time).
+  int64_t knext = 0;
+  auto slab = [&](int b, bool st, bool ld) {
+    // row bits 1..2 come from lane & 15 alone (the block offsets are multiples of 16)
+    const int kc = b3_chunk(lane & 15, lane >> 4) * 8;
+    bf8 bh[4], bl[4], a8[8];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       const int r = wn * 64 + n * 16 + (lane & 15);
       bh[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LB[b][0][r][kc]));
       bl[n] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LB[b][1][r][kc]));
     }
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      a8[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LA[b][0][wm * 128 + m * 16 + (lane & 15)][kc]));
     // B fragment as the row operand: the accumulators hold C^T blocks, so the
     // epilogue writes contiguous column segments
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 8; ++m)
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
-        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[n], ah[m], acc[m][n], 0, 0, 0);
-        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[n], al[m], acc[m][n], 0, 0, 0);
-        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[n], ah[m], acc[m][n], 0, 0, 0);
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[n], a8[m], acc[m][n], 0, 0, 0);
+        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[n], a8[m], acc[m][n], 0, 0, 0);
       }
+    if (st) store_tiles(b ^ 1);
+    if (ld) load_tiles(knext);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      a8[m] = __builtin_bit_cast(bf8, *reinterpret_cast<const u16x8*>(&LA[b][1][wm * 128 + m * 16 + (lane & 15)][kc]));
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[n], a8[m], acc[m][n], 0, 0, 0);
+    // instruction order: 16 fragment reads; 8 x (8 MFMAs, 1 LDS store, 1
+    // global load); the 8 lo(A) reads; the last 32 MFMAs
+    __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);  // LDS reads
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);  // MFMA
+      if (st) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+      if (ld) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load
+    }
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
   };
 
   if (kb < ke) {
@@ -594,15 +630,20 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
     if (kb + kB3K < ke) load_tiles(kb + kB3K);
     lds_barrier();
     int buf = 0;
-    for (int64_t k0 = kb; k0 < ke; k0 += kB3K) {
-      compute(buf);
-      if (k0 + kB3K < ke) {
-        store_tiles(buf ^ 1);
-        if (k0 + 2 * kB3K < ke) load_tiles(k0 + 2 * kB3K);
-      }
+    int64_t k0 = kb;
+    // the loop body is straight-line code so the schedule above holds in it
+    for (; k0 + 2 * kB3K < ke; k0 += kB3K) {
+      knext = k0 + 2 * kB3K;
+      slab(buf, true, true);
       lds_barrier();
       buf ^= 1;
     }
+    if (k0 + kB3K < ke) {
+      slab(buf, true, false);
+      lds_barrier();
+      buf ^= 1;
+    }
+    slab(buf, false, false);
   }
 
   // lane layout of the 16x16x32 outputs (C^T blocks): row lane%16, column 4*(lane/16) + v
@@ -612,8 +653,8 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
   const bool rd = d.beta != 0.0;
   if (i0 + kB3M <= d.M && j0 + kB3N <= d.N) {  // interior tile: straight-line loads and stores
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      float* Cr = C + i0 + wm * 64 + m * 16 + (lane & 15);
+    for (int m = 0; m < 8; ++m) {
+      float* Cr = C + i0 + wm * 128 + m * 16 + (lane & 15);
       float cv[4][4];
       if (rd) {
 #pragma unroll
@@ -632,8 +673,8 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
     return;
   }
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int64_t row = i0 + wm * 64 + m * 16 + (lane & 15);
+  for (int m = 0; m < 8; ++m) {
+    const int64_t row = i0 + wm * 128 + m * 16 + (lane & 15);
     if (row >= d.M) continue;
     float cv[4][4];
 #pragma unroll
