@@ -156,7 +156,10 @@ int zolo_mpdgeqrf_host(int64_t n, double* S, int64_t lds, double* tau);
 int zolo_mpdorgqr_host(int64_t n, double* S, int64_t lds, const double* tau);
 
 /* ---- building blocks (device pointers; used by tests and the benchmark) --- */
-/* C = alpha op(A) op(B) + beta C, precision 0 = F64, 1 = F32 (float data), 2 = BF16x3 (float data) */
+/* C = alpha op(A) op(B) + beta C, precision 0 = F64, 1 = F32 (float data), 2 = BF16x3 (float data),
+ * 3 = F64 data with the products formed exactly from int8 residues (16 moduli, int8 MFMA) and
+ * the result rebuilt by the Chinese remainder theorem: FP64-level accuracy (inputs rounded to
+ * 53 or more bits relative to their row / column maximum), K <= 131072 (larger K: FP64 MFMA) */
 int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
                      const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc,
                      void* stream);
@@ -306,6 +309,15 @@ int zolo_l0_grad_device(const float* params, const float* feat, const float* tar
 /* steps of Adam (lr, beta1 0.9, beta2 0.999) on params in place; loss_hist: host, steps values or NULL */
 int zolo_l0_train_device(float* params, const float* feat, const float* target, int N, int steps, float lr, float tau,
                          float* loss_hist, void* stream);
+
+/* ---- FP64 products of the mixed precisions (no counterpart in the reference) ----
+ * The FP64 parts of precisions 1-4 (the sum over shifts sum_q alpha_q Q1_q Q2_q^T of the QR
+ * step, the polish's U^T U and U^T X, H = (U^T A + A^T U)/2) take their large products
+ * from exact int8 products modulo 16 moduli (mode 1, the default; see zolo_gemm_device
+ * precision 3) or from FP64 MFMA (mode 0).  Precision 0 always uses FP64 MFMA.  The mode is
+ * process-wide; it returns -1 for a mode other than 0 or 1. */
+int zolo_set_f64_products(int mode);
+int zolo_get_f64_products(void);
 
 /* ---- measurement: GEMM launch ledger (no counterpart in the reference) ----
  * While enabled, every MFMA GEMM launch is bracketed by hipEvents on its stream

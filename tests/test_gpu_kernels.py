@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 def _gemm_case(torch, zl, prec, ta, tb, m, n, k, beta, seed):
     rng = np.random.default_rng(seed)
-    dt = np.float64 if prec == "f64" else np.float32
+    dt = np.float64 if prec in ("f64", "f64x") else np.float32
     Ah = rng.standard_normal((k, m) if ta else (m, k)).astype(dt)
     Bh = rng.standard_normal((n, k) if tb else (k, n)).astype(dt)
     Ch = rng.standard_normal((m, n)).astype(dt)
@@ -28,7 +28,7 @@ def _gemm_case(torch, zl, prec, ta, tb, m, n, k, beta, seed):
     return float(np.max(np.abs(got - ref) / scale))
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6), ("bf16x3", 2e-5)])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f64x", 1e-14), ("f32", 2e-6), ("bf16x3", 2e-5)])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("m,n,k", [(200, 130, 70), (64, 257, 300), (300, 40, 129), (1, 1, 1), (513, 129, 17),
                                    (256, 256, 33), (1000, 1100, 1037)])
@@ -44,7 +44,7 @@ def test_chol_inv(torch_gpu, zl, prec, n):
     rng = np.random.default_rng(n)
     G = rng.standard_normal((n, n))
     W = G.T @ G / n + 0.5 * np.eye(n)
-    dt = np.float64 if prec == "f64" else np.float32
+    dt = np.float64 if prec in ("f64", "f64x") else np.float32
     dA = zl.to_device(W.astype(dt))
     dLi = zl.to_device(np.zeros((n, n), dtype=dt))
     zl.chol_inv_device(dA, dLi, precision=prec)
@@ -114,3 +114,27 @@ def test_zoloqr(zl, n, con, rk):
         # two backward-stable QRs agree to about eps * kappa(X)
         tol = max(1e-11, 1e-15 * con)
         assert np.linalg.norm(Zg - Zo) / max(np.linalg.norm(Zo), 1e-300) < tol, ip
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
+def test_gemm_f64x_graded(torch_gpu, zl, ta, tb):
+    """int8-modular FP64 products: rows of op(A) and columns of op(B) scaled over 2^+-100,
+    zero rows/columns, K not a multiple of the slab; error relative to |A||B| at FP64 level."""
+    rng = np.random.default_rng(5 + 2 * ta + tb)
+    m, n, k = 390, 271, 1111
+    opA = rng.standard_normal((m, k)) * np.exp2(rng.integers(-100, 100, size=(m, 1)))
+    opB = rng.standard_normal((k, n)) * np.exp2(rng.integers(-100, 100, size=(1, n)))
+    opA[7] = 0.0
+    opB[:, 11] = 0.0
+    Ah = np.ascontiguousarray(opA.T) if ta else opA
+    Bh = np.ascontiguousarray(opB.T) if tb else opB
+    Ch = rng.standard_normal((m, n))
+    dA, dB, dC = zl.to_device(Ah), zl.to_device(Bh), zl.to_device(Ch)
+    zl.gemm_device(ta, tb, m, n, k, -0.75, dA, Ah.shape[0], dB, Bh.shape[0], 0.0, dC, m, precision="f64x")
+    torch_gpu.cuda.synchronize()
+    got = zl.from_device(dC)
+    ref = -0.75 * opA @ opB
+    scale = 0.75 * np.abs(opA) @ np.abs(opB)
+    assert np.all(got[7] == 0.0) and np.all(got[:, 11] == 0.0)
+    mask = scale > 0
+    assert float(np.max(np.abs(got - ref)[mask] / scale[mask])) < 1e-14

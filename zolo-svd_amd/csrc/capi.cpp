@@ -438,7 +438,7 @@ int zolo_normest_device(int64_t m, int64_t n, const double* A, int64_t lda, doub
 int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
                      const void* A, int64_t lda, const void* B, int64_t ldb, double beta, void* C, int64_t ldc,
                      void* stream) try {
-  if (precision < 0 || precision > 2) return -1;
+  if (precision < 0 || precision > 3) return -1;
   if (m < 0) return -4;
   if (n < 0) return -5;
   if (k < 0) return -6;
@@ -452,6 +452,16 @@ int zolo_gemm_device(int precision, int transa, int transb, int64_t m, int64_t n
 } catch (...) {
   return exception_status();
 }
+
+int zolo_set_f64_products(int mode) try {
+  if (mode < 0 || mode > 1) return -1;
+  set_f64_products(mode);
+  return kOk;
+} catch (...) {
+  return exception_status();
+}
+
+int zolo_get_f64_products(void) { return f64_products(); }
 
 int zolo_profile_enable(int on) try {
   ledger_enable(on != 0);

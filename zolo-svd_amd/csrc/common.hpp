@@ -76,9 +76,15 @@ struct GemmDesc {
   int64_t splitStride = 0;
 };
 
-enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2 };
+// F64X: FP64 operands and result, the products formed exactly from int8
+// residues modulo 16 primes-like moduli (ozaki.hip); FP64-level accuracy.
+enum class Prec : int { F64 = 0, F32 = 1, BF16X3 = 2, F64X = 3 };
 
 int gemm(Prec p, const GemmDesc& d, hipStream_t s);
+// The int8-modular FP64 product (ozaki.hip); descriptors it does not cover
+// (operand maps, K > 131072) take the FP64 MFMA engine in gemm().
+bool gemm_ozaki_supported(const GemmDesc& d);
+int gemm_ozaki(const GemmDesc& d, hipStream_t s, bool capture);
 // Graph-capture mode: no split-K (its workspace may need an allocation) and
 // no launch ledger events while on.
 void gemm_set_capture(bool on);

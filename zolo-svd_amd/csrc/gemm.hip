@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gemm_tiles.hpp"
 
 namespace zolo {
 
@@ -133,60 +134,6 @@ __device__ inline d4 mfma16(double a, double b, d4 c) {
 }
 __device__ inline f4 mfma16(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-
-// Workgroup barrier that first waits for this wave's LDS operations only:
-// global loads in flight stay in flight across it.  The memory clobber keeps
-// the compiler from moving LDS accesses across the barrier.
-__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
-struct TileCoord { int64_t i0, j0; bool skip; };
-
-// Map the linear workgroup id to a (row, col) tile, XCD-aware and grouped.
-// XCD x (= pid % 8, the round-robin dispatch) owns a contiguous eighth of the
-// logical tiles, which walk groups of 8 tile-rows column by column so that
-// neighbouring workgroups share A and B panels in the XCD's L2.  When the work
-// per tile varies along the rows (triangular outputs, banded operands),
-// "balance" orders the full row groups 0, last, 1, last-1, ... (with groups
-// small enough that every XCD gets at least two) so that every XCD's eighth
-// pairs heavy and light groups.
-template <int BM, int BN>
-__device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC, bool balance) {
-  const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
-  const int64_t nt = tm * tn;
-  int64_t pid = blockIdx.x;
-  const int64_t full = (nt / 8) * 8;
-  if (pid < full) {  // XCD x (= pid % 8) owns logical tiles [x*nt/8, (x+1)*nt/8)
-    pid = (pid % 8) * (full / 8) + pid / 8;
-  }
-  // tile-rows per group; balancing wants at least two full groups per XCD
-  const int64_t G = balance ? max((int64_t)1, min((int64_t)8, tm / 16)) : 8;
-  int64_t group = pid / (G * tn);
-  const int64_t r = pid - group * G * tn;
-  const int64_t nfull = tm / G;  // groups of exactly G tile-rows (a partial one stays last)
-  if (balance && group < nfull) group = (group & 1) ? nfull - 1 - (group >> 1) : (group >> 1);
-  const int64_t first = group * G;
-  const int64_t gsz = (tm - first) < G ? (tm - first) : G;
-  TileCoord t;
-  t.i0 = (first + r % gsz) * BM;
-  t.j0 = (r / gsz) * BN;
-  t.skip = lowerC && (t.j0 > t.i0 + BM - 1);
-  return t;
-}
-
-// Work per tile depends on its position: triangular output or trimmed K range.
-__device__ inline bool banded(const GemmDesc& d) {
-  return d.lowerC || d.aLo >= 0 || d.aHi >= 0 || d.bLo >= 0 || d.bHi >= 0;
-}
-
-template <int BM, int BN>
-__device__ inline void k_range(const GemmDesc& d, int64_t i0, int64_t j0, int64_t& kb, int64_t& ke) {
-  kb = 0;
-  ke = d.K;
-  if (d.aLo >= 0) kb = max(kb, i0 - d.aLo);
-  if (d.bLo >= 0) kb = max(kb, j0 - d.bLo);
-  if (d.aHi >= 0) ke = min(ke, i0 + BM + d.aHi);
-  if (d.bHi >= 0) ke = min(ke, j0 + BN + d.bHi);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,13 +385,6 @@ __device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((u
 constexpr int kSplitTile = 64;
 constexpr int kB3K = 32;            // K depth of one bf16x3 slab and of a layout tile
 constexpr int kB3Rows = 128;        // rows of a layout tile
-// LDS rows of a slab are 64 bytes (4 chunks of 16 bytes) with the chunk index
-// XOR-swizzled by bits 1..2 of the row: the 16-byte fragment reads (16 rows x
-// 4 chunks per wave) and the 16-byte tile writes (16 rows x 4 chunks) then
-// touch 64 distinct banks per 256-byte LDS cycle.  A padded 80-byte pitch
-// conflicts two ways on both (counter probe: 8 instead of 4 LDS cycles per
-// wave-wide 16-byte read, 16 instead of 8 per write).
-__device__ inline int b3_chunk(int row, int c) { return c ^ ((row >> 1) & 3); }
 constexpr int kB3M = 256, kB3N = 256, kB3Threads = 512;  // workgroup tile: 8 waves of 128 x 64
 
 // Offset of element (r, k) of a split operand in its tiled layout: tiles of
@@ -562,7 +502,7 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
     bh1 = *reinterpret_cast<const u32x4*>(Bh + ob0 + tile_elems + ko);
     bl1 = *reinterpret_cast<const u32x4*>(Bl + ob0 + tile_elems + ko);
   };
-  const int lrow = t >> 2, lk = b3_chunk(lrow, t & 3) * 8;  // rows lrow and 128 + lrow swizzle alike
+  const int lrow = t >> 2, lk = lds_chunk(lrow, t & 3) * 8;  // rows lrow and 128 + lrow swizzle alike
   auto store_tiles = [&](int b) {
     *reinterpret_cast<u32x4*>(&LA[b][0][lrow][lk]) = ah0;
     *reinterpret_cast<u32x4*>(&LA[b][1][lrow][lk]) = al0;
@@ -582,7 +522,7 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
   int64_t knext = 0;
   auto slab = [&](int b, bool st, bool ld) {
     // row bits 1..2 come from lane & 15 alone (the block offsets are multiples of 16)
-    const int kc = b3_chunk(lane & 15, lane >> 4) * 8;
+    const int kc = lds_chunk(lane & 15, lane >> 4) * 8;
     bf8 bh[4], bl[4], a8[8];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
@@ -888,6 +828,11 @@ int gemm(Prec p, const GemmDesc& d, hipStream_t s) {
     case Prec::F64: return launch_typed<double>(d, s);
     case Prec::F32: return launch_typed<float>(d, s);
     case Prec::BF16X3: return launch_bf16x3(d, s);
+    case Prec::F64X: {
+      if (!gemm_ozaki_supported(d) || d.K <= 0) return launch_typed<double>(d, s);
+      LedgerScope ls(d, 3, 256, 256, s);
+      return gemm_ozaki(d, s, g_capture);
+    }
   }
   return kErrArg;
 }

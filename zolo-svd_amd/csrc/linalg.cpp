@@ -220,7 +220,7 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
     d.B = Q + n; d.ldb = ls; d.strideB = ss;
     d.C = Z; d.ldc = ldz; d.strideC = sZ;
     d.alpha = 1.0; d.alphaVec = d_alpha; d.beta = 0.0; d.bLo = 256;
-    ZOLO_CHECK(gemm_t<T>(c, d));
+    ZOLO_CHECK(gemm(big_prec<T>(c), with_tile(c, d), c.s));
     *nz = r;
     return kOk;
   }
@@ -256,7 +256,7 @@ int qr_step(const Ctx& c, const T* X, int64_t ldx, int64_t n, int r, const doubl
     d.B = Q + q * ss + n; d.ldb = ls;
     d.C = Z; d.ldc = ldz;
     d.alpha = 1.0; d.alphaVec = d_alpha + q; d.beta = q ? 1.0 : 0.0; d.aLo = 256; d.bLo = 256;
-    ZOLO_CHECK(gemm_t<T>(c, d));
+    ZOLO_CHECK(gemm(big_prec<T>(c), with_tile(c, d), c.s));
   }
   // Z = Q_X Z~
   ZOLO_CHECK(dense_apply_q<T>(c, Xw, n, 0, n, 1, Tx, Z, ldz, 0, ar));

@@ -38,11 +38,20 @@ class Arena {
 
 // Grow-only device workspace shared by the entry points (not re-entrant).
 int workspace_acquire(size_t bytes, char** base);
+
+// Products of the mixed precisions' FP64 parts (zolo_set_f64_products): 1 =
+// exact int8 products (default), 0 = FP64 MFMA.
+int f64_products();
+void set_f64_products(int mode);
 void workspace_release_all();
 
 struct Ctx {
   hipStream_t s = nullptr;
   Prec low = Prec::F32;  // GEMM precision used for float operands (F32 or BF16X3)
+  // GEMM precision of the large square FP64 products (the Zolotarev sum
+  // sum_q alpha_q Q1_q Q2_q^T, the polish's U^T U and U^T X, H = U^T A):
+  // F64, or F64X (exact int8 products, FP64-level accuracy) in the mixed precisions
+  Prec big = Prec::F64;
   int tile = 0;          // GEMM tile hint for products with M, N > 64 (0: automatic)
 };
 
@@ -55,6 +64,9 @@ inline GemmDesc with_tile(const Ctx& c, GemmDesc d) {
 template <typename T> inline Prec prec_for(const Ctx& c) { return Prec::F32; }
 template <> inline Prec prec_for<double>(const Ctx&) { return Prec::F64; }
 template <> inline Prec prec_for<float>(const Ctx& c) { return c.low; }
+// precision of the large square products: the context's choice for FP64
+template <typename T> inline Prec big_prec(const Ctx& c) { return prec_for<T>(c); }
+template <> inline Prec big_prec<double>(const Ctx& c) { return c.big; }
 
 // Batched Cholesky with explicit inverse: A_q (lower) -> L_q, Li_q = L_q^{-1}.
 // tmp must hold batch * ceil(n/2)^2 elements (stride sT).

@@ -196,15 +196,15 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
   const int max_passes = 3;
   int pass = 0;
   for (; pass < max_passes; ++pass) {
-    // G = U^T U (lower), C = U^T Xref  -- FP64
+    // G = U^T U (lower), C = U^T Xref  -- FP64 (FP64 MFMA or exact int8 products)
     GemmDesc d;
     d.transA = 1; d.M = n; d.N = n; d.K = n;
     d.A = U; d.lda = ldu; d.B = U; d.ldb = ldu; d.C = G; d.ldc = n; d.lowerC = 1;
-    ZOLO_CHECK(gemm(Prec::F64, d, s));
+    ZOLO_CHECK(gemm(c.big, d, s));
     d = GemmDesc();
     d.transA = 1; d.M = n; d.N = n; d.K = n;
     d.A = U; d.lda = ldu; d.B = Xref; d.ldb = ldx; d.C = C; d.ldc = n;
-    ZOLO_CHECK(gemm(Prec::F64, d, s));
+    ZOLO_CHECK(gemm(c.big, d, s));
     float* none = nullptr;
     hipLaunchKernelGGL(pair_tile_kernel<0>, tiles, dim3(kT), 0, s, (const double*)G, n, F, none);
     hipLaunchKernelGGL(pair_tile_kernel<1>, tiles, dim3(kT), 0, s, (const double*)C, n, Hs, R);

@@ -7,6 +7,7 @@
 // and sums the shift contributions inside the fused FormX2 kernel, which
 // replaces the DGSUM2D reduction across shift groups.  The only host round
 // trip per iteration is the ERR scalar that the loop condition needs.
+#include <atomic>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -18,6 +19,13 @@
 #include "linalg.hpp"
 
 namespace zolo {
+
+namespace {
+std::atomic<int> g_f64_products{1};
+}  // namespace
+int f64_products() { return g_f64_products.load(); }
+void set_f64_products(int mode) { g_f64_products.store(mode); }
+
 
 namespace {
 
@@ -89,7 +97,7 @@ int form_h(const Ctx& c, const double* U, int64_t ldu, const double* A, int64_t 
   GemmDesc d;
   d.transA = 1; d.transB = 0; d.M = n; d.N = n; d.K = n;
   d.A = U; d.lda = ldu; d.B = A; d.ldb = lda; d.C = H; d.ldc = ldh;
-  ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+  ZOLO_CHECK(gemm(c.big, d, c.s));
   return k_symmetrize<double>(H, ldh, n, c.s);
 }
 
@@ -163,6 +171,7 @@ int zolo_pd_run(int precision, char sym, int rk, int64_t n, double alpha, double
   Ctx c;
   c.s = s;
   c.low = (precision == 2) ? Prec::BF16X3 : Prec::F32;
+  c.big = (mixed && f64_products() == 1) ? Prec::F64X : Prec::F64;
   Timer tm(s);
   tm.mark(kPhOther);
 

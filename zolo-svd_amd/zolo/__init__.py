@@ -46,6 +46,7 @@ except ImportError:  # pragma: no cover - host-only use of the scalar helpers
 _lib = C.CDLL(_LIB_PATH)
 
 PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "f32_refsched": 3, "bf16x3_refsched": 4}
+GEMM_PREC = {"f64": 0, "f32": 1, "bf16x3": 2, "f64x": 3}
 ERR_NOT_SPD = 1002
 ERR_NO_CONVERGENCE = 1003
 
@@ -164,7 +165,18 @@ _sig("zolo_l0_grad_device", C.c_int, [_P, _P, _P, C.c_int, C.c_float, _P, C.POIN
 _sig("zolo_l0_train_device", C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_float, C.c_float, _P, _P])
 _sig("zolo_formx2_host", C.c_int, [_I64, C.c_double, _D, _I64, _D, _I64, _D])
 _sig("zolo_profile_enable", C.c_int, [C.c_int])
+_sig("zolo_set_f64_products", C.c_int, [C.c_int])
+_sig("zolo_get_f64_products", C.c_int, [])
 _sig("zolo_profile_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), _D, _D])
+
+
+def set_f64_products(mode: int):
+    """FP64 parts of the mixed precisions: 1 = exact int8 products (default), 0 = FP64 MFMA."""
+    _check("zolo_set_f64_products", _lib.zolo_set_f64_products(int(mode)))
+
+
+def get_f64_products() -> int:
+    return int(_lib.zolo_get_f64_products())
 
 
 def profile_enable(on: bool = True):
@@ -402,8 +414,9 @@ def pdnormest_device(dA, tol: float = 1e-6):
 
 def gemm_device(transa: int, transb: int, m: int, n: int, k: int, alpha: float, dA, lda: int, dB, ldb: int,
                 beta: float, dC, ldc: int, precision: str = "f64"):
-    """C = alpha op(A) op(B) + beta C on device buffers (column-major)."""
-    _check("zolo_gemm_device", _lib.zolo_gemm_device(PREC[precision], transa, transb, m, n, k, alpha,
+    """C = alpha op(A) op(B) + beta C on device buffers (column-major).  precision: "f64", "f32",
+    "bf16x3" or "f64x" (FP64 data, exact int8 products modulo 16 moduli)."""
+    _check("zolo_gemm_device", _lib.zolo_gemm_device(GEMM_PREC[precision], transa, transb, m, n, k, alpha,
                                                      C.c_void_p(dA.data_ptr()), lda, C.c_void_p(dB.data_ptr()), ldb,
                                                      beta, C.c_void_p(dC.data_ptr()), ldc, _stream()))
 
