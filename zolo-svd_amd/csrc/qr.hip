@@ -12,8 +12,8 @@
 //
 // Panel factorisation, three forms: the whole panel in one workgroup for
 // stacks of at most kQrPanelWgRows rows (qr_panel_wg_kernel); the whole panel
-// in one launch with one barrier per column among the workgroups of a shift
-// (qr_panel_coop_kernel); and, where neither applies (graph capture of a
+// in one launch whose workgroups exchange one column sum per column through
+// per-workgroup flags (qr_panel_coop_kernel); and, where neither applies (graph capture of a
 // multi-workgroup panel, grids too large to be resident), one launch per
 // column (qr_panel_kernel): launch j applies the reflector of column j-1 to
 // the rest of the panel and, in the same pass over the rows each thread owns,
@@ -23,8 +23,8 @@
 // scheduling.  The trailing update and the explicit Q use the compact WY form
 // (I - V T V^T) through the MFMA GEMM.
 #include <algorithm>
+#include <type_traits>
 
-#include "grid_sync.hpp"
 #include "kernels.hpp"
 #include "wave_sums.hpp"
 
@@ -141,27 +141,55 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
 // Whole panel (jb <= 64 columns) in one launch: one thread per panel row (rows
 // [p, r1_end) then [n, r2_end)) keeps the row's panel values in registers,
 // shifted so that the current column is always v[0] (no register is selected
-// by the loop counter), and stores each column when it is finished.  Per
-// column: the Gram row g_c = sum_{i>j} S[i][j] S[i][j+c] and the pivot row go
-// to a per-workgroup slot, one barrier among the workgroups of the shift, then
-// every workgroup sums the slots in the same fixed order, forms the reflector
-// (dlarfg) and applies it to its rows.  The slots alternate between two
-// buffers, so the next column's partials never overwrite ones still being
-// read.  Shifts q0 .. q0 + gridDim.y - 1; bar holds one zeroed counter per
-// shift (kCoopBarStride words apart), or is null when nwg == 1 (no barrier
-// between workgroups).
+// by the loop counter), and stores each column when it is finished.  The
+// columns run in three phases by the number of columns still in the panel
+// (more than 32, 17 to 32, at most 16): the column sums, the update and the
+// shift then touch 64, 32 or 16 registers.
+//
+// Per column: the wave column sums give the Gram row g_c = sum_{i>j} S[i][j]
+// S[i][j+c] (lane 0: the squared norm below the diagonal); after a workgroup
+// barrier wave 0 adds the 8 wave sums.  With one workgroup per shift it forms
+// the reflector (dlarfg) and w_c = tau (r_c + scal g_c) at once; after a
+// second barrier every thread applies the reflector to its row.
+//
+// With several workgroups per shift, wave 0 exchanges the sums: it stores the
+// workgroup's 64 sums (workgroup 0 also the pivot row j, its row jj) to this
+// column's slot, waits until those stores are complete, raises the
+// workgroup's flag to jj + 1, and polls the nwg flags of the shift together
+// (lane k reads flag k) until all have reached jj + 1.  After a barrier every
+// wave loads its share of the slots (wave w: workgroups w, w + 8, ...), and
+// after another wave 0 adds the 8 shares and forms the reflector.  All sums
+// run in a fixed order, so the results do not depend on scheduling, and no
+// counter is shared by the workgroups.  Slots and flags are only accessed by
+// device-scope atomic loads and stores, which gfx950 performs at the device's
+// coherence point (stores written through, loads not served from an
+// incoherent cache); the vmcnt(0) wait before the flag store makes the slot
+// stores complete first, and the slot loads are issued only after the flag
+// loads have returned.  That orders the exchange without a release/acquire
+// pair, whose L2 writeback and invalidate cost the per-column exchange about
+// 2 us at 32 workgroups.  The slots alternate between two buffers: a
+// workgroup writes column jj's slot only after every workgroup has raised its
+// flag for column jj - 1, which each does after it has read the slots of
+// column jj - 2.  Shifts q0 .. q0 + gridDim.y - 1; flags holds
+// kCoopFlagStride zeroed words per shift, or is null when nwg == 1.  A poll
+// that outlasts kCoopSpinCap rounds gives up (the results are then wrong, but
+// the kernel ends), so a launch that is not co-resident cannot hang the device.
 constexpr int kCoopT = 512;
-constexpr int kCoopMaxSlots = 8;  // partial slots per thread when summing: nwg <= 8 * (kCoopT / 64)
-constexpr int kCoopBarStride = 64;  // unsigned words between the barrier counters of two shifts
+constexpr int kCoopWaves = kCoopT / 64;
+constexpr int kCoopMaxWg = 64;        // workgroups per shift: one flag per lane of a wave
+constexpr int kCoopFlagStride = 64;   // unsigned words between the flags of two shifts (one 256-byte line)
+constexpr unsigned kCoopSpinCap = 1u << 24;
+// s_waitcnt operand (gfx9 encoding): vmcnt(0), expcnt and lgkmcnt not waited for
+constexpr int kWaitVm0 = 0x0F70;
 
 template <typename T>
 __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
                                                                int64_t p, int jb, int64_t r1_end, int64_t r2_end,
-                                                               double* part, unsigned* bar, int nwg, T* tau,
+                                                               double* part, unsigned* flags, int nwg, T* tau,
                                                                int64_t strideTau, int q0) {
-  __shared__ double red[kCoopT / 64][NBP];
+  __shared__ double red[kCoopWaves][NBP];
   __shared__ double wv[NBP], prj[NBP];
-  __shared__ double sc[3];
+  __shared__ double sc[2];
   const int qi = blockIdx.y, q = q0 + qi;
   T* Sq = S + (int64_t)q * strideS;
   const int t = threadIdx.x, w = blockIdx.x, lane = t & 63, wave = t >> 6;
@@ -170,61 +198,93 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
   const bool valid = l < len1 + (r2_end - n);
   const int64_t rec = (int64_t)nwg * NBP + NBP;
   double* base = part + (int64_t)qi * 2 * rec;
+  const unsigned* flr = flags ? flags + (int64_t)qi * kCoopFlagStride : nullptr;
+  unsigned* flw = flags ? flags + (int64_t)qi * kCoopFlagStride + w : nullptr;
   // v[c]: the row's value in panel column jj + c; zero for jj + c >= jb
   double v[NBP];
 #pragma unroll
   for (int c = 0; c < NBP; ++c) v[c] = (valid && c < jb) ? (double)Sq[i + (p + c) * lds] : 0.0;
   T* out = Sq + i + p * lds;
-  for (int jj = 0; jj < jb; ++jj) {
+
+  auto column = [&](auto cols, int jj) {
+    constexpr int NV = decltype(cols)::value;
     const int64_t j = p + jj;
-    const int rem = jb - jj;  // columns jj .. jb - 1 are v[0 .. rem)
-    double* G = base + (int64_t)(jj & 1) * rec;
+    const int rem = jb - jj;  // columns jj .. jb - 1 are v[0 .. rem), rem <= NV
     // bottom rows beyond n + j take no part in the reflector of column j
     const bool live = valid && i >= j && (i < r1_end || i <= n + j);
+    // the pivot row j is row jj of the top range: thread jj of workgroup 0 (wave 0)
     if (live && i == j) {
 #pragma unroll
-      for (int c = 0; c < NBP; ++c) prj[c] = v[c];
+      for (int c = 0; c < NV; ++c) prj[c] = v[c];
     }
     const double x = (live && i > j) ? v[0] : 0.0;
-    // lane 0: the squared norm below the diagonal; lanes c >= 1: the Gram row
-    red[wave][lane] = wave_colsum64_swap(v, x, lane);
-    __syncthreads();
-    // wave 0 publishes the slots (the barrier's release is issued by its thread 0);
-    // row j (the pivot row) lies in the top range, in workgroup (j - p) / kCoopT
-    if (t < rem) {
-      double sum = 0.0;
+    double cs;
+    if constexpr (NV == 64) {
+      cs = wave_colsum64_swap(v, x, lane);
+    } else {
+      double a[NV];
 #pragma unroll
-      for (int k = 0; k < kCoopT / 64; ++k) sum += red[k][t];
-      G[(int64_t)w * NBP + t] = sum;
-      if ((j - p) / kCoopT == w) G[(int64_t)nwg * NBP + t] = prj[t];
+      for (int c = 0; c < NV; ++c) a[c] = x * v[c];
+      if constexpr (NV == 32) cs = wave_colsum32_swap(a, lane);
+      else cs = wave_colsum16_swap(a, lane);
     }
-    if (bar) grid_barrier(bar + (int64_t)qi * kCoopBarStride, (unsigned)(jj + 1) * (unsigned)nwg);
-    else __syncthreads();  // one workgroup per shift
-    // the pivot row rides along with the first group's partial loads (wave 0:
-    // lane c holds r_c)
-    double pr = 0.0;
-    {  // sum the nwg partials of each column: 8 groups of 64 threads, all loads independent
-      const int c = t & 63, g = t >> 6;
-      double pv[kCoopMaxSlots];
-#pragma unroll
-      for (int m = 0; m < kCoopMaxSlots; ++m) {
-        const int k = g + m * (kCoopT / 64);
-        pv[m] = (c < rem && k < nwg) ? G[(int64_t)k * NBP + c] : 0.0;
-      }
-      if (g == 0 && c < rem) pr = G[(int64_t)nwg * NBP + c];
-      double sum = 0.0;
-#pragma unroll
-      for (int m = 0; m < kCoopMaxSlots; ++m) sum += pv[m];
-      red[g][c] = sum;
-    }
+    red[wave][lane] = cs;  // lanes >= NV hold copies of lane & (NV - 1), ignored below (rem <= NV)
     __syncthreads();
-    if (wave == 0) {  // lane c: g_c, then the reflector (dlarfg) from lane 0's r_0 and g_0
-      double gsum = 0.0;
-      if (t < rem) {
+    double g = 0.0, pr = 0.0;
+    if (wave == 0) {
 #pragma unroll
-        for (int g = 0; g < kCoopT / 64; ++g) gsum += red[g][t];
+      for (int k = 0; k < kCoopWaves; ++k) g += red[k][lane];
+      pr = prj[lane];
+    }
+    if (flr) {
+      double* G = base + (int64_t)(jj & 1) * rec;
+      const unsigned target = (unsigned)jj + 1u;
+      if (wave == 0) {
+        if (lane < rem) {
+          __hip_atomic_store(G + (int64_t)w * NBP + lane, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (w == 0) __hip_atomic_store(G + (int64_t)nwg * NBP + lane, pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // the slot stores are complete (at the device coherence point) before the flag goes out
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (lane == 0) __hip_atomic_store(flw, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (unsigned spin = 0; spin < kCoopSpinCap; ++spin) {
+          const unsigned f =
+              lane < nwg ? __hip_atomic_load(flr + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
+          if (__all(f >= target)) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
       }
-      const double alpha = __shfl(pr, 0, 64), xn2 = __shfl(gsum, 0, 64);
+      __syncthreads();
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      // this wave's share of the slots (workgroups wave, wave + 8, ...), all
+      // loads in flight together: out-of-range entries read slot 0 and are masked
+      double sv[kCoopMaxWg / kCoopWaves];
+      const int cl = lane < rem ? lane : 0;
+#pragma unroll
+      for (int m = 0; m < kCoopMaxWg / kCoopWaves; ++m) {
+        const int k = wave + m * kCoopWaves;
+        sv[m] = __hip_atomic_load(G + (int64_t)(k < nwg ? k : 0) * NBP + cl, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (wave == 0) pr = __hip_atomic_load(G + (int64_t)nwg * NBP + cl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < kCoopMaxWg / kCoopWaves; ++m)
+        if (wave + m * kCoopWaves < nwg) s += sv[m];
+      red[wave][lane] = s;  // wave 0 read red before the barrier above
+      __syncthreads();
+      if (wave == 0) {
+        g = 0.0;
+#pragma unroll
+        for (int k = 0; k < kCoopWaves; ++k) g += red[k][lane];
+        if (lane >= rem) pr = 0.0;
+      }
+    }
+    if (wave == 0) {
+      // the reflector (dlarfg) from lane 0's r_0 and g_0, in every lane
+      const double alpha = __shfl(pr, 0, 64), xn2 = __shfl(g, 0, 64);
       double beta = alpha, tv = 0.0, scal = 0.0;
       if (xn2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
@@ -232,29 +292,34 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
         scal = 1.0 / (alpha - beta);
       }
       // w_c = tau (r_c + scal g_c): the update of column c is v_i w_c; zero past the panel
-      wv[t] = (t >= 1 && t < rem) ? tv * (pr + scal * gsum) : 0.0;
-      if (t == 0) {
-        sc[0] = beta; sc[1] = tv; sc[2] = scal;
+      if (lane < NV) wv[lane] = (lane >= 1 && lane < rem) ? tv * (pr + scal * g) : 0.0;
+      if (lane == 0) {
+        sc[0] = beta;
+        sc[1] = scal;
         if (w == 0) tau[(int64_t)q * strideTau + j] = (T)tv;
       }
     }
     __syncthreads();
     if (live) {
-      const double vi = (i == j) ? 1.0 : v[0] * sc[2];
+      const double vi = (i == j) ? 1.0 : v[0] * sc[1];
       v[0] = (i == j) ? sc[0] : vi;
 #pragma unroll
-      for (int c = 1; c < NBP; ++c) v[c] -= vi * wv[c];
+      for (int c = 1; c < NV; ++c) v[c] -= vi * wv[c];
     }
     // column jj is final in every row (rows above j were finished by earlier reflectors)
     if (valid) *out = (T)v[0];
     out += lds;
 #pragma unroll
-    for (int c = 0; c + 1 < NBP; ++c) v[c] = v[c + 1];
-    v[NBP - 1] = 0.0;
-    // No barrier here: the next column writes prj and red only after this
-    // column's last barrier, which every reader of them has passed, and wv/sc
-    // only after the next grid barrier.
-  }
+    for (int c = 0; c + 1 < NV; ++c) v[c] = v[c + 1];
+    v[NV - 1] = 0.0;
+    // No barrier here: red and prj are written next only after every thread
+    // has passed this column's last barrier (wave 0 read them before it), wv
+    // and sc only after the next column's first barrier.
+  };
+  int jj = 0;
+  for (; jj < jb && jb - jj > 32; ++jj) column(std::integral_constant<int, 64>{}, jj);
+  for (; jj < jb && jb - jj > 16; ++jj) column(std::integral_constant<int, 32>{}, jj);
+  for (; jj < jb; ++jj) column(std::integral_constant<int, 16>{}, jj);
 }
 
 // Whole-panel factorisation in one workgroup for stacks of at most
@@ -455,10 +520,10 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
       return kErrArg;
     }
     int one = 1;
-    unsigned* nobar = nullptr;
+    unsigned* noflags = nullptr;
     int q0 = 0;
     hipLaunchKernelGGL((qr_panel_coop_kernel<T>), dim3(1, batch), dim3(kCoopT), 0, s, S, lds, strideS, n, p, jb,
-                       r1_end, r2_end, part, nobar, one, tau, strideTau, q0);
+                       r1_end, r2_end, part, noflags, one, tau, strideTau, q0);
     ZOLO_HIP(hipGetLastError());
     *done = true;
     return kOk;
@@ -484,22 +549,21 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   const int64_t rows = (r1_end - p) + (r2_end - n);
   const int nwg = (int)((rows + kCoopT - 1) / kCoopT);
   // too large for one resident grid or for the partial sums: the caller takes the per-column kernels
-  if (cap[dev] < nwg || nwg > kCoopMaxSlots * (kCoopT / 64)) return kOk;
+  if (cap[dev] < nwg || nwg > kCoopMaxWg) return kOk;
   const int per = std::min(batch, cap[dev] / nwg);
   const int64_t rec = (int64_t)nwg * NBP + NBP;
-  // partial slots of the group, then one barrier counter per shift, each on a
-  // 256-byte line of its own (counters sharing a line serialise the arrivals
-  // of all shifts: 925 us instead of 615 us for 8 stacks of 16384 x 64)
-  if ((int64_t)per * 2 * rec + (int64_t)per * kCoopBarStride / 2 + 32 > part_doubles) {
+  // partial slots of the group, then the flags of each shift on a 256-byte
+  // line of their own
+  if ((int64_t)per * 2 * rec + (int64_t)per * kCoopFlagStride / 2 + 32 > part_doubles) {
     set_error("qr panel (cooperative): partials buffer too small");
     return kErrArg;
   }
-  unsigned* bar = reinterpret_cast<unsigned*>(part + ((int64_t)per * 2 * rec + 31) / 32 * 32);
+  unsigned* flags = reinterpret_cast<unsigned*>(part + ((int64_t)per * 2 * rec + 31) / 32 * 32);
   for (int q0 = 0; q0 < batch; q0 += per) {
     const int nq = std::min(per, batch - q0);
     int nw = nwg;
-    ZOLO_HIP(hipMemsetAsync(bar, 0, sizeof(unsigned) * kCoopBarStride * nq, s));
-    void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &bar, &nw, &tau, &strideTau,
+    ZOLO_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * kCoopFlagStride * nq, s));
+    void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &flags, &nw, &tau, &strideTau,
                     (void*)&q0};
     ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T>, dim3(nwg, nq), dim3(kCoopT), args, 0,
                                         s));

@@ -77,6 +77,20 @@ __device__ inline V wave_colsum32_swap(V* a, int lane) {
   return swap_add<32>(a[0], a[0]);
 }
 
+// Column sums over the 64 lanes for 16 columns, a[c] (c < 16) holding each
+// lane's term of column c (a is overwritten): four shuffle stages inside each
+// 16-lane row, then the four rows added with the 16- and 32-lane swaps; lane
+// l returns the sum of column l & 15.
+template <typename V>
+__device__ inline V wave_colsum16_swap(V* a, int lane) {
+  butterfly_step<8>(a, lane);
+  butterfly_step<4>(a, lane);
+  butterfly_step<2>(a, lane);
+  butterfly_step<1>(a, lane);
+  const V r = swap_add<16>(a[0], a[0]);
+  return swap_add<32>(r, r);
+}
+
 // Column sums of x * v[c] over the 64 lanes for all 64 columns: lane l
 // returns the sum of column l.  The first two stages use the lane swaps, the
 // last four shuffles; the products are formed in the first stage.
