@@ -57,8 +57,10 @@ def test_chol_inv(torch_gpu, zl, prec, n):
     assert np.linalg.norm(Li - np.linalg.inv(L)) / np.linalg.norm(np.linalg.inv(L)) < tol
 
 
-# n > 544: the panels run as one cooperative launch per 64 columns
-@pytest.mark.parametrize("n,s", [(5, 1.0), (64, 1e-3), (100, 0.5), (130, 1e4), (257, 2.0), (600, 1e-3), (777, 3.0)])
+# n > 544: the panels run as one cooperative launch per 64 columns; n = 1100:
+# five 256-wide outer blocks, the last one partial (76 columns)
+@pytest.mark.parametrize("n,s", [(5, 1.0), (64, 1e-3), (100, 0.5), (130, 1e4), (257, 2.0), (600, 1e-3), (777, 3.0),
+                                 (1100, 0.7)])
 def test_stacked_qr(torch_gpu, zl, n, s):
     """MPDGEQRF+MPDORGQR restatement vs the oracle's structured Householder QR."""
     rng = np.random.default_rng(n)
