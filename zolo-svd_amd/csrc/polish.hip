@@ -82,6 +82,75 @@ __global__ __launch_bounds__(kT) void skew_tile_kernel(float* D, const float* M,
   }
 }
 
+// 64 x 64 tiles of D = M - M^T (the Lyapunov operator applied to the CG
+// direction P, with M = P Hs) and, in the same pass, each tile's sum of
+// P .* D in double: part[tile] (tiles numbered x-fastest; the sums run in a
+// fixed order, so the result does not depend on scheduling).
+__global__ __launch_bounds__(kT) void skew_dot_kernel(float* D, const float* M, const float* P, int64_t n,
+                                                      double* part) {
+  __shared__ float tt[64][65];
+  __shared__ double sh[kT];
+  const int64_t bi = (int64_t)blockIdx.x * 64, bj = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  // tt[b][a] = M[bj + a][bi + b]
+  for (int b = ty; b < 64; b += kT / 64)
+    tt[b][tx] = (bj + tx < n && bi + b < n) ? M[(bj + tx) + (bi + b) * n] : 0.f;
+  __syncthreads();
+  double acc = 0.0;
+  if (bi + tx < n) {
+    for (int r = ty; r < 64; r += kT / 64) {
+      if (bj + r >= n) break;
+      const int64_t e = (bi + tx) + (bj + r) * n;
+      const float v = M[e] - tt[tx][r];
+      D[e] = v;
+      acc += (double)P[e] * (double)v;
+    }
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = kT / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// One CG step on the iterate and the residual, four floats per thread and
+// iteration: S += a P, R -= a AP, and part[block] = the block's sum of R^2
+// (double) of the updated residual.
+__global__ __launch_bounds__(kT) void cg_update_kernel(float* __restrict__ S, float* __restrict__ R,
+                                                       const float* __restrict__ P, const float* __restrict__ AP,
+                                                       int64_t len, float a, double* part) {
+  __shared__ double sh[kT];
+  double acc = 0.0;
+  const int64_t n4 = len / 4, stride = (int64_t)gridDim.x * kT;
+  float4* S4 = reinterpret_cast<float4*>(S);
+  float4* R4 = reinterpret_cast<float4*>(R);
+  const float4* P4 = reinterpret_cast<const float4*>(P);
+  const float4* A4 = reinterpret_cast<const float4*>(AP);
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < n4; e += stride) {
+    const float4 p = P4[e], ap = A4[e];
+    float4 sv = S4[e], r = R4[e];
+    sv.x += a * p.x; sv.y += a * p.y; sv.z += a * p.z; sv.w += a * p.w;
+    r.x -= a * ap.x; r.y -= a * ap.y; r.z -= a * ap.z; r.w -= a * ap.w;
+    S4[e] = sv;
+    R4[e] = r;
+    acc += ((double)r.x * r.x + (double)r.y * r.y) + ((double)r.z * r.z + (double)r.w * r.w);
+  }
+  for (int64_t e = n4 * 4 + (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += stride) {
+    S[e] += a * P[e];
+    R[e] -= a * AP[e];
+    acc += (double)R[e] * (double)R[e];
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = kT / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 // partial[blk] = sum a*b (double accumulation)
 __global__ __launch_bounds__(kT) void dot_kernel(const float* a, const float* b, int64_t len, double* partial) {
   __shared__ double sh[kT];
@@ -124,13 +193,19 @@ __global__ __launch_bounds__(kT) void finish_reduce_kernel(const double* p, int 
   if (threadIdx.x == 0) *out = sh[0];
 }
 
-// x += a*y ; then optionally (cg direction) y2 = r + b*y2 handled by separate calls
-__global__ __launch_bounds__(kT) void axpy_f_kernel(float* x, const float* y, int64_t len, float a) {
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += (int64_t)gridDim.x * kT) x[e] += a * y[e];
-}
-// p = r + b*p
-__global__ __launch_bounds__(kT) void xpby_f_kernel(float* p, const float* r, int64_t len, float b) {
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += (int64_t)gridDim.x * kT) p[e] = r[e] + b * p[e];
+// p = r + b*p, four floats per thread and iteration
+__global__ __launch_bounds__(kT) void xpby_f_kernel(float* __restrict__ p, const float* __restrict__ r, int64_t len,
+                                                    float b) {
+  const int64_t n4 = len / 4, stride = (int64_t)gridDim.x * kT;
+  float4* P4 = reinterpret_cast<float4*>(p);
+  const float4* R4 = reinterpret_cast<const float4*>(r);
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < n4; e += stride) {
+    const float4 rv = R4[e];
+    float4 pv = P4[e];
+    pv.x = rv.x + b * pv.x; pv.y = rv.y + b * pv.y; pv.z = rv.z + b * pv.z; pv.w = rv.w + b * pv.w;
+    P4[e] = pv;
+  }
+  for (int64_t e = n4 * 4 + (int64_t)blockIdx.x * kT + threadIdx.x; e < len; e += stride) p[e] = r[e] + b * p[e];
 }
 // E = S - F/2
 __global__ __launch_bounds__(kT) void form_e_kernel(float* E, const float* S, const float* F, int64_t len) {
@@ -147,9 +222,15 @@ __global__ __launch_bounds__(kT) void add_f_to_d_kernel(double* U, int64_t ldu, 
 
 }  // namespace
 
+// partial sums: one per 64 x 64 tile or per reduction block
+size_t polish_parts(int64_t n) {
+  const size_t t = (size_t)((n + 63) / 64);
+  return std::max<size_t>(kNp, t * t);
+}
+
 size_t polish_bytes(int64_t n) {
   const size_t nn = (size_t)n * n;
-  return 2 * nn * sizeof(double) + 8 * nn * sizeof(float) + (kNp + 16) * sizeof(double) + 32 * 256;
+  return 2 * nn * sizeof(double) + 8 * nn * sizeof(float) + (polish_parts(n) + 16) * sizeof(double) + 32 * 256;
 }
 
 int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx, int64_t n, char* ws, int* passes,
@@ -167,20 +248,21 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
   float* AP = ar.take<float>(nn);
   float* M = ar.take<float>(nn);
   float* U32 = ar.take<float>(nn);
-  double* part = ar.take<double>(kNp);
+  const size_t nparts = polish_parts(n);
+  double* part = ar.take<double>(nparts);
   double* out = ar.take<double>(16);
   ZOLO_ARENA_OK(ar);
   const unsigned g = grid_for(nn);
   const dim3 tiles((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
-  auto reduce = [&](bool is_max, double* h) -> int {
-    hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(kT), 0, s, part, kNp, is_max ? 1 : 0, out);
+  auto reduce_n = [&](int np, bool is_max, double* h) -> int {
+    hipLaunchKernelGGL(finish_reduce_kernel, dim3(1), dim3(kT), 0, s, part, np, is_max ? 1 : 0, out);
     ZOLO_HIP(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, s));
     ZOLO_HIP(hipStreamSynchronize(s));
     return kOk;
   };
   auto dot = [&](const float* a, const float* b, double* h) -> int {
     hipLaunchKernelGGL(dot_kernel, dim3(kNp), dim3(kT), 0, s, a, b, nn, part);
-    return reduce(false, h);
+    return reduce_n(kNp, false, h);
   };
   // M = A_ * B_ (n x n) in the low precision of the run (FP32, or BF16x3 with
   // its ~2^-16 relative accuracy): the CG only needs S to a relative accuracy
@@ -226,17 +308,19 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
     const double rho = std::max(rho_floor, std::min(1e-2, 1e-15 / std::max(delta, 1e-300)));
     const double tol2 = rho * rho * rr0;
     int it = 0;
+    // per iteration: the operator product, then two passes over the n x n
+    // arrays: AP = skew(P Hs) with P . AP, and the S, R update with R . R
     for (; it < 40 && rr > tol2 && rr > 0.0; ++it) {
       ZOLO_CHECK(lgemm(P, Hs, M));
-      hipLaunchKernelGGL(skew_tile_kernel<false>, tiles, dim3(kT), 0, s, AP, M, n, 1.0f);
+      hipLaunchKernelGGL(skew_dot_kernel, tiles, dim3(kT), 0, s, AP, (const float*)M, (const float*)P, n, part);
       double pap;
-      ZOLO_CHECK(dot(P, AP, &pap));
+      ZOLO_CHECK(reduce_n((int)(tiles.x * tiles.y), false, &pap));
       if (!(pap > 0.0)) break;
       const float a = (float)(rr / pap);
-      hipLaunchKernelGGL(axpy_f_kernel, dim3(g), dim3(kT), 0, s, S, P, nn, a);
-      hipLaunchKernelGGL(axpy_f_kernel, dim3(g), dim3(kT), 0, s, R, AP, nn, -a);
+      hipLaunchKernelGGL(cg_update_kernel, dim3(kNp), dim3(kT), 0, s, S, R, (const float*)P, (const float*)AP, nn, a,
+                         part);
       double rn;
-      ZOLO_CHECK(dot(R, R, &rn));
+      ZOLO_CHECK(reduce_n(kNp, false, &rn));
       hipLaunchKernelGGL(xpby_f_kernel, dim3(g), dim3(kT), 0, s, P, R, nn, (float)(rn / rr));
       rr = rn;
     }
