@@ -118,6 +118,24 @@ def test_zoloqr(zl, n, con, rk):
         assert np.linalg.norm(Zg - Zo) / max(np.linalg.norm(Zo), 1e-300) < tol, ip
 
 
+@pytest.mark.parametrize("k", [1, 3, 13])
+def test_gemm_f64x_small_k(torch_gpu, zl, k):
+    """int8-modular FP64 products at tiny K, where the inputs are rounded to 59-60 bits
+    (scaled integers up to 2^60, the widest the residue split handles): FP64-level error."""
+    rng = np.random.default_rng(40 + k)
+    m, n = 300, 257
+    opA = rng.standard_normal((m, k)) * np.exp2(rng.integers(-30, 30, size=(m, 1)))
+    opB = rng.standard_normal((k, n))
+    Ch = rng.standard_normal((m, n))
+    dA, dB, dC = zl.to_device(opA), zl.to_device(opB), zl.to_device(Ch)
+    zl.gemm_device(0, 0, m, n, k, 1.0, dA, m, dB, k, 0.5, dC, m, precision="f64x")
+    torch_gpu.cuda.synchronize()
+    got = zl.from_device(dC)
+    ref = opA @ opB + 0.5 * Ch
+    scale = np.abs(opA) @ np.abs(opB) + 0.5 * np.abs(Ch)
+    assert float(np.max(np.abs(got - ref) / scale)) < 1e-15
+
+
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
 def test_gemm_f64x_graded(torch_gpu, zl, ta, tb):
     """int8-modular FP64 products: rows of op(A) and columns of op(B) scaled over 2^+-100,

@@ -43,8 +43,8 @@ constexpr int kOzM = 256, kOzN = 256, kOzGemmThreads = 512;
 // Pairwise coprime odd moduli (greedy from 255 down): product 2^124.95.
 struct OzTables {
   int m[kOzMods];
+  double inv_m[kOzMods];      // 1 / m_l rounded
   int inv[kOzMods][kOzMods];  // inv[l][j] = m_j^{-1} mod m_l for j < l
-  int two32[kOzMods];         // 2^32 mod m_l
   int pre[kOzMods][kOzMods];  // pre[l][j] = (m_0 ... m_{j-1}) mod m_l for j < l
   int qinv[kOzMods];          // (m_0 ... m_{l-1})^{-1} mod m_l
 };
@@ -60,10 +60,8 @@ constexpr int inv_mod(int a, int m) {
 constexpr OzTables make_tables() {
   OzTables T{{255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191}, {}, {}, {}, {}};
   for (int l = 0; l < kOzMods; ++l) {
+    T.inv_m[l] = 1.0 / T.m[l];
     for (int j = 0; j < kOzMods; ++j) T.inv[l][j] = j < l ? inv_mod(T.m[j] % T.m[l], T.m[l]) : 0;
-    long long p = 1;
-    for (int b = 0; b < 32; ++b) p = (p * 2) % T.m[l];
-    T.two32[l] = (int)p;
     long long q = 1;
     for (int j = 0; j < kOzMods; ++j) {
       T.pre[l][j] = j < l ? (int)q : 0;
@@ -133,14 +131,18 @@ __global__ __launch_bounds__(kOzThreads) void oz_exp_kernel(const double* __rest
   if ((threadIdx.x & 3) == 0 && r < R && e != INT_MIN) atomicMax(emax + z * estride + r, e);
 }
 
-// Symmetric residue of a (|a| < 2^62) modulo m: a = hi 2^32 + lo.
-__device__ inline int oz_residue(int hi, unsigned lo, int m, int two32) {
-  int r = (hi % m) * two32 + (int)(lo % (unsigned)m);
-  r %= m;
-  const int h = m >> 1;
-  if (r > h) r -= m;
-  if (r < -h) r += m;
-  return r;
+// Symmetric residue of an integer a held exactly in a double (|a| < 2^62)
+// modulo m, in FP64: q = floor(a / m) from the rounded reciprocal is off by at
+// most a few units, and each fma is exact (its exact result is an integer of
+// a few times m); the second reduction brings the remainder into [0, m] (m
+// itself only when it is an exact multiple and the product rounded down), and
+// one comparison into [-h, h] (h = (m - 1) / 2; m maps to 0).  Full-rate FP64
+// operations instead of 32-bit integer divisions by constants.
+__device__ inline int oz_residue_f64(double a, double m, double inv_m, int h) {
+  double r = fma(-floor(a * inv_m), m, a);
+  r = fma(-floor(r * inv_m), m, r);
+  const int v = (int)r;
+  return v > h ? v - (int)m : v;
 }
 
 // Residue planes of an operand: row r scaled by 2^(bits - 1 - emax[r]) and
@@ -162,14 +164,11 @@ __global__ __launch_bounds__(kOzThreads) void oz_split_kernel(const double* __re
   const int em = r < R ? emax[z * estride + r] : INT_MIN;
   const bool live = em != INT_MIN && em != INT_MAX;
   const int sc = live ? bits - 1 - em : 0;
-  int hi[16];
-  unsigned lo[16];
+  // the scaled integers, exactly (x 2^sc is exact and rint rounds it to an
+  // integer, which a double holds exactly below 2^62)
+  double av[16];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    const long long a = live ? (long long)rint(ldexp(tile[rr][kk + c], sc)) : 0;
-    hi[c] = (int)(a >> 32);
-    lo[c] = (unsigned)(a & 0xffffffffll);
-  }
+  for (int c = 0; c < 16; ++c) av[c] = live ? rint(ldexp(tile[rr][kk + c], sc)) : 0.0;
   const int64_t o = oz_offset(r, k0 + kk, Kp);
   signed char* dst = out + z * kOzMods * plane + o;
 #pragma unroll
@@ -177,7 +176,7 @@ __global__ __launch_bounds__(kOzThreads) void oz_split_kernel(const double* __re
     unsigned w[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
-      const int v = oz_residue(hi[c], lo[c], kOz.m[l], kOz.two32[l]);
+      const int v = oz_residue_f64(av[c], (double)kOz.m[l], kOz.inv_m[l], kOz.m[l] >> 1);
       w[c >> 2] |= ((unsigned)v & 0xffu) << (8 * (c & 3));
     }
     *reinterpret_cast<uint4*>(dst + l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
