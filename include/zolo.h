@@ -235,6 +235,15 @@ int zolo_syevd_device(int64_t n, double* H, int64_t ldh, double* w, double* Z, i
                       void* stream);
 int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U, int64_t ldu,
                     double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream);
+/* zolo_svd_ex_device: zolo_svd_device with the choice of bounds (argument 2):
+ * 0 = from the QR the first step reuses (what zolo_svd_device does), 1 = LU +
+ * 1-norm estimate, alpha = ||A||_F and beta = rcond/sqrt(n), as the
+ * reference's driver computes them before PZOLOPD1 (test_zolodrv1.f90:179-189;
+ * ZoloCHK1 receives them).  The other arguments follow zolo_svd_device, one
+ * position later. */
+int zolo_svd_ex_device(int precision, int bounds, int rk, int64_t n, const double* A, int64_t lda, double* S,
+                       double* U, int64_t ldu, double* V, int64_t ldv, zolo_info_t* info, double* phase_ms,
+                       void* stream);
 /* ---- spectral division (QDWH-eig / Zolo-eig; device pointers, FP64) ----------
  * zolo_subit_device replaces PQDWHSUBIT (pqdwhsubit.f90:2): on entry Q is the
  * polar factor of A - sigma I (A symmetric n x n, normA = ||A||_F > 0); C =
@@ -257,6 +266,15 @@ int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, 
 int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
                           double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
                           void* stream);
+/* zolo_eig_below_ex_device: zolo_eig_below_device with the choice of bounds
+ * (argument 2): 0 = from the QR the first step reuses, 1 = LU + 1-norm
+ * estimate of A - sigma I with PQDWHFACS's convention, beta =
+ * ||(A - sigma I)/alpha||_1 rcond/sqrt(n) (pqdwhfacs.f90:318-344, the bound of
+ * the reference's pqdwheig flow).  The other arguments follow
+ * zolo_eig_below_device, one position later. */
+int zolo_eig_below_ex_device(int precision, int bounds, int rk, int64_t n, const double* A, int64_t lda, double sigma,
+                             double* w, double* V, int64_t ldv, int* count, zolo_info_t* info, double* err,
+                             int* iters, void* stream);
 /* Host-array forms (the Fortran PQDWHSUBIT and ZoloEig drop-ins call them):
  * zolo_subit_host overwrites A and Q as zolo_subit_device; zolo_eig_host leaves
  * Q and As unchanged and returns w(1:rk) and V(:, 1:rk). */

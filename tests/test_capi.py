@@ -93,6 +93,15 @@ def test_spectral_division_argument_codes_without_gpu(zl):
     assert L.zolo_eig_below_device(0, 0, 4, p, 4, float("nan"), p, p, 4, C.byref(cnt), None, None, None,
                                    None) == -6
     assert L.zolo_eig_below_device(0, 0, 4, p, 4, 0.0, p, p, 4, None, None, None, None, None) == -10
+    # zolo_eig_below_ex_device(precision, bounds, rk, n, ...): one position later, bounds is -2
+    assert L.zolo_eig_below_ex_device(0, 2, 0, 4, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -2
+    assert L.zolo_eig_below_ex_device(0, 1, 9, 4, p, 4, 0.0, p, p, 4, C.byref(cnt), None, None, None, None) == -3
+    assert L.zolo_eig_below_ex_device(0, 1, 0, 4, p, 4, 0.0, p, p, 4, None, None, None, None, None) == -11
+    # zolo_svd_device(precision, rk, n, A, lda, S, U, ldu, V, ldv, info, phase_ms, stream) and the _ex form
+    assert L.zolo_svd_device(0, 0, 4, p, 4, p, p, 4, p, 3, None, None, None) == -10
+    assert L.zolo_svd_ex_device(0, 5, 0, 4, p, 4, p, p, 4, p, 4, None, None, None) == -2
+    assert L.zolo_svd_ex_device(0, 1, 0, 4, p, 4, p, p, 4, p, 3, None, None, None) == -11
+    assert L.zolo_svd_ex_device(0, 0, 0, 4, None, 4, p, p, 4, p, 4, None, None, None) == -5
     # host forms
     d = C.cast(buf, C.POINTER(C.c_double))
     assert L.zolo_subit_host(4, d, 4, -1.0, d, 4, C.byref(rk), None, None) == -4

@@ -125,3 +125,21 @@ def test_eig_below_order_one(zl, torch_gpu):
     w, V, count, _, _, _ = zl.eig_below_device(zl.to_device(A), 1.0)
     assert count == 1 and abs(float(w[0]) - 0.75) < 1e-15
     assert abs(abs(float(zl.from_device(V)[0, 0])) - 1.0) < 1e-15
+
+
+@pytest.mark.parametrize("bounds", ["qr", "lu"])
+def test_eig_below_bounds_modes(zl, torch_gpu, bounds):
+    """alpha, beta and r of the polar step of A - sigma I: from the QR its first step
+    reuses (oracle bounds_from_qr) or from LU + 1-norm estimate with PQDWHFACS's
+    convention (pqdwhfacs.f90:318-344, oracle bounds_estimate(..., 'qdwh'))."""
+    n = 160
+    A, sigma = _case(n, 91, 0.4)
+    B = A - sigma * np.eye(n)
+    w, V, count, info, err, iters = zl.eig_below_device(zl.to_device(A), sigma, bounds=bounds)
+    alpha, beta = zo.bounds_from_qr(B) if bounds == "qr" else zo.bounds_estimate(B, "qdwh")
+    assert abs(info.alpha - alpha) <= 1e-13 * alpha
+    assert abs(info.beta - beta) <= 1e-8 * beta, (info.beta, beta)
+    assert info.r == zo.choosem(1.0 / beta)
+    w = w.cpu().numpy()
+    V = zl.from_device(V) if count else np.zeros((n, 0))
+    _check_pairs(A, sigma, w, V, count, tol=1e-12)

@@ -78,3 +78,24 @@ def test_zolo_svd_c4_closed_form(zl, torch_gpu, n):
     assert np.linalg.norm(U.T @ U - np.eye(n)) / math.sqrt(n) <= 1e-13
     assert np.linalg.norm(V.T @ V - np.eye(n)) / math.sqrt(n) <= 1e-13
     assert np.max(np.abs(S - Sref)) <= 1e-13
+
+
+@pytest.mark.parametrize("bounds", ["qr", "lu"])
+def test_zolo_svd_bounds_modes(zl, torch_gpu, bounds):
+    """alpha, beta and r of the SVD's polar step: from the QR its first step reuses
+    (oracle bounds_from_qr) or from LU + 1-norm estimate as the reference's driver
+    computes them (test_zolodrv1.f90:179-189, oracle bounds_estimate); r = choosem(1/beta)."""
+    n = 200
+    rng = np.random.default_rng(77)
+    U0, V0 = zo.haar(n, rng), zo.haar(n, rng)
+    s = np.geomspace(1.0, 1e-6, n)
+    A = (U0 * s) @ V0.T
+    S, dU, dV, info, _ = zl.svd_device(zl.to_device(A), bounds=bounds)
+    alpha, beta = zo.bounds_from_qr(A) if bounds == "qr" else zo.bounds_estimate(A)
+    assert abs(info.alpha - alpha) <= 1e-13 * alpha
+    assert abs(info.beta - beta) <= 1e-8 * beta, (info.beta, beta)
+    assert info.r == zo.choosem(1.0 / beta)
+    S = S.cpu().numpy()
+    U, V = zl.from_device(dU), zl.from_device(dV)
+    assert np.max(np.abs(S - np.sort(s))) <= 1e-13
+    assert np.linalg.norm(A - (U * S) @ V.T) / np.linalg.norm(A) <= 1e-13

@@ -648,19 +648,10 @@ int zolo_eig_device(int64_t n, double* Q, int64_t ldq, double* As, int64_t lda, 
   return exception_status();
 }
 
-int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
-                          double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
-                          void* stream) try {
-  if (precision < 0 || precision > 4) return -1;
-  if (rk < 0 || rk > 8) return -2;
-  if (n < 1 || n > 32768) return -3;
-  if (!A) return -4;
-  if (lda < n) return -5;
-  if (!std::isfinite(sigma)) return -6;
-  if (!w) return -7;
-  if (!V) return -8;
-  if (ldv < n) return -9;
-  if (!count) return -10;
+namespace {
+// pqdwheig.f90:113-144 with the Zolo polar factor, on validated arguments
+int eig_below_run(int precision, int bounds, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
+                  double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters, void* stream) {
   ZOLO_CHECK(ensure_device());
   hipStream_t s = (hipStream_t)stream;
   DevMat B, Q, As;
@@ -674,12 +665,55 @@ int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int
                             hipMemcpyDeviceToDevice, s));
   if (sigma != 0.0) ZOLO_CHECK(k_add_diag<double>(B.p, n, n, -sigma, s));
   zolo_info_t local{};
-  // polar factor of the symmetric A - sigma I into Q, its bounds from the QR
-  // the first step reuses (rk = 0: choosem)
-  const int st = zolo_pd_run(precision, 'S', rk, n, 0.0, 0.0, B.p, n, Q.p, n, nullptr, n, info ? info : &local, s);
+  // polar factor of the symmetric A - sigma I into Q (rk = 0: choosem), its
+  // bounds from the QR the first step reuses (bounds = 0), or by LU + 1-norm
+  // estimate as PQDWHFACS computes them in the reference's pqdwheig flow
+  // (bounds = 1, pqdwhfacs.f90:318-344: beta = ||B/alpha||_1 rcond / sqrt(n))
+  double alpha = 0.0, beta = 0.0;
+  if (bounds == 1) {
+    ZOLO_CHECK(zolo_bounds_device(1, n, B.p, n, &alpha, &beta, stream));
+    if (rk == 0) rk = choosem(1.0 / beta);
+  }
+  const int st = zolo_pd_run(precision, 'S', rk, n, alpha, beta, B.p, n, Q.p, n, nullptr, n, info ? info : &local, s);
   if (st) return st;
   // PQDWHSUBIT, PDSYEVD on the Rk block, V1 VV1 (pqdwheig.f90:119-144 = ZoloEig.f90:80-103)
   return zoloeig_run(n, Q.p, n, As.p, n, w, V, ldv, count, err, iters, s);
+}
+}  // namespace
+
+int zolo_eig_below_ex_device(int precision, int bounds, int rk, int64_t n, const double* A, int64_t lda, double sigma,
+                             double* w, double* V, int64_t ldv, int* count, zolo_info_t* info, double* err,
+                             int* iters, void* stream) try {
+  if (precision < 0 || precision > 4) return -1;
+  if (bounds != 0 && bounds != 1) return -2;
+  if (rk < 0 || rk > 8) return -3;
+  if (n < 1 || n > 32768) return -4;
+  if (!A) return -5;
+  if (lda < n) return -6;
+  if (!std::isfinite(sigma)) return -7;
+  if (!w) return -8;
+  if (!V) return -9;
+  if (ldv < n) return -10;
+  if (!count) return -11;
+  return eig_below_run(precision, bounds, rk, n, A, lda, sigma, w, V, ldv, count, info, err, iters, stream);
+} catch (...) {
+  return exception_status();
+}
+
+int zolo_eig_below_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double sigma, double* w,
+                          double* V, int64_t ldv, int* count, zolo_info_t* info, double* err, int* iters,
+                          void* stream) try {
+  if (precision < 0 || precision > 4) return -1;
+  if (rk < 0 || rk > 8) return -2;
+  if (n < 1 || n > 32768) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!std::isfinite(sigma)) return -6;
+  if (!w) return -7;
+  if (!V) return -8;
+  if (ldv < n) return -9;
+  if (!count) return -10;
+  return eig_below_run(precision, 0, rk, n, A, lda, sigma, w, V, ldv, count, info, err, iters, stream);
 } catch (...) {
   return exception_status();
 }
@@ -824,18 +858,10 @@ int zolo_syevd_host(int64_t n, double* H, int64_t ldh, double* w, double* Z, int
   return exception_status();
 }
 
-int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U, int64_t ldu,
-                    double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream) try {
-  if (precision < 0 || precision > 4) return -1;
-  if (rk < 0 || rk > 8) return -2;
-  if (n < 1 || n > 32768) return -3;
-  if (!A) return -4;
-  if (lda < n) return -5;
-  if (!S) return -6;
-  if (!U) return -7;
-  if (ldu < n) return -8;
-  if (!V) return -9;
-  if (ldv < n) return -10;
+namespace {
+// ZoloCHK1.f90:80-91 on validated arguments (bounds: 0 = from the first QR, 1 = LU)
+int svd_run(int precision, int bounds, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U,
+            int64_t ldu, double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream) {
   ZOLO_CHECK(ensure_device());
   hipStream_t s = (hipStream_t)stream;
   using clk = std::chrono::steady_clock;
@@ -844,9 +870,17 @@ int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t l
   ZOLO_CHECK(Ht.alloc(n));
   zolo_info_t local{};
   zolo_info_t* inf = info ? info : &local;
-  // polar factor U_p into U, H = (U_p^T A + A^T U_p)/2 into Ht (ZoloCHK1.f90:80-83);
-  // alpha, beta from the QR its first step reuses (rk = 0: choosem)
-  const int st = zolo_pd_run(precision, 'N', rk, n, 0.0, 0.0, A, lda, U, ldu, Ht.p, n, inf, s);
+  // bounds = 1: alpha, beta by LU + 1-norm estimate as the reference's driver
+  // computes them (test_zolodrv1.f90:179-189); 0: from the QR the first step
+  // reuses (alpha = beta = 0 below)
+  double alpha = 0.0, beta = 0.0;
+  if (bounds == 1) {
+    ZOLO_CHECK(zolo_bounds_device(0, n, A, lda, &alpha, &beta, stream));
+    if (rk == 0) rk = choosem(1.0 / beta);  // the driver's rule (pzolopd1.f90 is handed r)
+  }
+  // polar factor U_p into U, H = (U_p^T A + A^T U_p)/2 into Ht (ZoloCHK1.f90:80-83)
+  // (rk = 0: choosem)
+  const int st = zolo_pd_run(precision, 'N', rk, n, alpha, beta, A, lda, U, ldu, Ht.p, n, inf, s);
   if (st) return st;
   ZOLO_HIP(hipStreamSynchronize(s));
   const auto t1 = clk::now();
@@ -870,6 +904,41 @@ int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t l
     phase_ms[4] = std::chrono::duration<double, std::milli>(t2 - t1).count();   // whole eigensolve + U
   }
   return kOk;
+}
+}  // namespace
+
+int zolo_svd_ex_device(int precision, int bounds, int rk, int64_t n, const double* A, int64_t lda, double* S,
+                       double* U, int64_t ldu, double* V, int64_t ldv, zolo_info_t* info, double* phase_ms,
+                       void* stream) try {
+  if (precision < 0 || precision > 4) return -1;
+  if (bounds != 0 && bounds != 1) return -2;
+  if (rk < 0 || rk > 8) return -3;
+  if (n < 1 || n > 32768) return -4;
+  if (!A) return -5;
+  if (lda < n) return -6;
+  if (!S) return -7;
+  if (!U) return -8;
+  if (ldu < n) return -9;
+  if (!V) return -10;
+  if (ldv < n) return -11;
+  return svd_run(precision, bounds, rk, n, A, lda, S, U, ldu, V, ldv, info, phase_ms, stream);
+} catch (...) {
+  return exception_status();
+}
+
+int zolo_svd_device(int precision, int rk, int64_t n, const double* A, int64_t lda, double* S, double* U, int64_t ldu,
+                    double* V, int64_t ldv, zolo_info_t* info, double* phase_ms, void* stream) try {
+  if (precision < 0 || precision > 4) return -1;
+  if (rk < 0 || rk > 8) return -2;
+  if (n < 1 || n > 32768) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!S) return -6;
+  if (!U) return -7;
+  if (ldu < n) return -8;
+  if (!V) return -9;
+  if (ldv < n) return -10;
+  return svd_run(precision, 0, rk, n, A, lda, S, U, ldu, V, ldv, info, phase_ms, stream);
 } catch (...) {
   return exception_status();
 }

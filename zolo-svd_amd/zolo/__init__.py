@@ -143,6 +143,8 @@ _sig("zolo_batched_tile", C.c_int, [_I64, C.c_int])
 _sig("zolo_syevd_device", C.c_int, [_I64, _P, _I64, _P, _P, _I64, _P, _P])
 _sig("zolo_svd_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _I64, _P, _I64, C.POINTER(_Info), _P,
                                   _P])
+_sig("zolo_svd_ex_device", C.c_int, [C.c_int, C.c_int, C.c_int, _I64, _P, _I64, _P, _P, _I64, _P, _I64,
+                                     C.POINTER(_Info), _P, _P])
 _sig("zolo_pd_auto_device", C.c_int, [C.c_int, C.c_char, C.c_int, _I64, _P, _I64, _P, _I64, _P, _I64,
                                       C.POINTER(_Info), _P])
 _sig("zolo_subit_device", C.c_int, [_I64, _P, _I64, C.c_double, _P, _I64, C.POINTER(C.c_int), _D,
@@ -151,6 +153,8 @@ _sig("zolo_eig_device", C.c_int, [_I64, _P, _I64, _P, _I64, _P, _P, _I64, C.POIN
                                   _P])
 _sig("zolo_eig_below_device", C.c_int, [C.c_int, C.c_int, _I64, _P, _I64, C.c_double, _P, _P, _I64,
                                         C.POINTER(C.c_int), C.POINTER(_Info), _D, C.POINTER(C.c_int), _P])
+_sig("zolo_eig_below_ex_device", C.c_int, [C.c_int, C.c_int, C.c_int, _I64, _P, _I64, C.c_double, _P, _P, _I64,
+                                           C.POINTER(C.c_int), C.POINTER(_Info), _D, C.POINTER(C.c_int), _P])
 _sig("zolo_subit_host", C.c_int, [_I64, _D, _I64, C.c_double, _D, _I64, C.POINTER(C.c_int), _D, C.POINTER(C.c_int)])
 _sig("zolo_eig_host", C.c_int, [_I64, _D, _I64, _D, _I64, _D, _D, _I64, C.POINTER(C.c_int)])
 _sig("zolo_mm_info", C.c_int, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
@@ -533,8 +537,12 @@ def syevd_device(dH):
     return w, Z, dict(zip(("tridiagonal", "divide_conquer", "back_transform"), ph.tolist()))
 
 
-def svd_device(dA, precision: str = "f64", rk: int = 0):
-    """Zolo-SVD (ZoloCHK1.f90:80-91): A = U diag(S) V^T with S ascending; returns (S, U, V, RunInfo, phase_ms)."""
+BOUNDS = {"qr": 0, "lu": 1}
+
+
+def svd_device(dA, precision: str = "f64", rk: int = 0, bounds: str = "qr"):
+    """Zolo-SVD (ZoloCHK1.f90:80-91): A = U diag(S) V^T with S ascending; returns (S, U, V, RunInfo, phase_ms).
+    bounds: "qr" (from the QR the first step reuses) or "lu" (LU + 1-norm estimate, test_zolodrv1.f90:179-189)."""
     torch = _torch()
     n = dA.shape[0]
     S = torch.empty((n,), dtype=torch.float64, device="cuda")
@@ -542,10 +550,10 @@ def svd_device(dA, precision: str = "f64", rk: int = 0):
     V = torch.empty_like(dA)
     ci = _Info()
     ph = np.zeros(5)
-    _check("zolo_svd_device", _lib.zolo_svd_device(PREC[precision], rk, n, C.c_void_p(dA.data_ptr()), n,
-                                                   C.c_void_p(S.data_ptr()), C.c_void_p(U.data_ptr()), n,
-                                                   C.c_void_p(V.data_ptr()), n, C.byref(ci),
-                                                   ph.ctypes.data_as(C.c_void_p), _stream()))
+    _check("zolo_svd_ex_device", _lib.zolo_svd_ex_device(PREC[precision], BOUNDS[bounds], rk, n,
+                                                         C.c_void_p(dA.data_ptr()), n, C.c_void_p(S.data_ptr()),
+                                                         C.c_void_p(U.data_ptr()), n, C.c_void_p(V.data_ptr()), n,
+                                                         C.byref(ci), ph.ctypes.data_as(C.c_void_p), _stream()))
     names = ("polar_h", "tridiagonal", "divide_conquer", "back_transform", "eig_total")
     return S, U, V, RunInfo.from_c(ci), dict(zip(names, ph.tolist()))
 
@@ -628,10 +636,12 @@ def eig_device(dQ, dAs):
     return w[:rk.value], V[:rk.value], rk.value, err.value, it.value
 
 
-def eig_below_device(dA, sigma: float, precision: str = "f64", rk: int = 0):
+def eig_below_device(dA, sigma: float, precision: str = "f64", rk: int = 0, bounds: str = "qr"):
     """Eigenpairs of the symmetric A below sigma (pqdwheig.f90:113-144 with the Zolo
     polar factor of A - sigma I).  dA is not modified.  Returns
-    (w, V, count, RunInfo, err, iterations) as ``eig_device``."""
+    (w, V, count, RunInfo, err, iterations) as ``eig_device``.  bounds: "qr" (from the
+    QR the first step reuses) or "lu" (LU + 1-norm estimate of A - sigma I with
+    PQDWHFACS's convention, pqdwhfacs.f90:318-344)."""
     torch = _torch()
     n = dA.shape[0]
     w = torch.zeros((n,), dtype=torch.float64, device="cuda")
@@ -639,10 +649,10 @@ def eig_below_device(dA, sigma: float, precision: str = "f64", rk: int = 0):
     cnt, it = C.c_int(0), C.c_int(0)
     err = C.c_double(0.0)
     ci = _Info()
-    _check("zolo_eig_below_device", _lib.zolo_eig_below_device(PREC[precision], rk, n, C.c_void_p(dA.data_ptr()), n,
-                                                               float(sigma), C.c_void_p(w.data_ptr()),
-                                                               C.c_void_p(V.data_ptr()), n, C.byref(cnt),
-                                                               C.byref(ci), C.byref(err), C.byref(it), _stream()))
+    _check("zolo_eig_below_ex_device",
+           _lib.zolo_eig_below_ex_device(PREC[precision], BOUNDS[bounds], rk, n, C.c_void_p(dA.data_ptr()), n,
+                                         float(sigma), C.c_void_p(w.data_ptr()), C.c_void_p(V.data_ptr()), n,
+                                         C.byref(cnt), C.byref(ci), C.byref(err), C.byref(it), _stream()))
     k = cnt.value
     return w[:k], V[:k], k, RunInfo.from_c(ci), err.value, it.value
 
