@@ -69,6 +69,13 @@ struct GemmDesc {
   const int* mapC = nullptr;
   // tile hint: 0 auto, 1 128x128, 2 64x128, 3 128x64, 4 64x64 (F64 / F32)
   int tile = 0;
+  // Two row ranges (F64 / F32 engine): rows k >= bGapAt of B (transB = 0)
+  // are read from row k + bGap, and rows i >= cGapAt of C are read and
+  // written at row i + cGap, so a product over [rows r0, r1) of one block and
+  // [n, r2) of another is one launch.  bGapAt: a multiple of 16 with no lower
+  // K trimming (aLo, bLo < 0); cGapAt: a multiple of 128; -1: no gap.
+  int64_t bGapAt = -1, bGap = 0;
+  int64_t cGapAt = -1, cGap = 0;
   // split-K (set by the launcher): blockIdx.y selects a K chunk whose raw
   // partial product goes to C + blockIdx.y * splitStride
   int splitk = 1;

@@ -158,7 +158,8 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmDesc d) {
   const int64_t z = blockIdx.z;
   const T* __restrict__ A = reinterpret_cast<const T*>(d.A) + (d.mapA ? (int64_t)d.mapA[z] : z) * d.strideA;
   const T* __restrict__ B = reinterpret_cast<const T*>(d.B) + (d.mapB ? (int64_t)d.mapB[z] : z) * d.strideB;
-  T* __restrict__ C = reinterpret_cast<T*>(d.C) + (d.mapC ? (int64_t)d.mapC[z] : z) * d.strideC;
+  T* __restrict__ C = reinterpret_cast<T*>(d.C) + (d.mapC ? (int64_t)d.mapC[z] : z) * d.strideC +
+                      ((d.cGapAt >= 0 && i0 >= d.cGapAt) ? d.cGap : 0);
 
   int64_t kb, ke;
   k_range<TBM, TBN>(d, i0, j0, kb, ke);
@@ -191,6 +192,8 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmDesc d) {
   // Interior tiles take unguarded loads (no per-element exec branches); edge
   // tiles and the K remainder take the guarded path.
   const bool interior = (i0 + TBM <= d.M) && (j0 + TBN <= d.N);
+  // the second row range of B (transB = 0): a slab lies on one side of bGapAt
+  auto boff = [&](int64_t k0) -> int64_t { return (d.bGapAt >= 0 && k0 >= d.bGapAt) ? d.bGap : 0; };
   auto load_fast = [&](int64_t k0) {
     if (TA == 0) {
       const T* src = A + (i0 + a_c) + (k0 + a_r) * d.lda;
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmDesc d) {
       for (int e = 0; e < EA; ++e) ra[e] = src[e];
     }
     if (TB == 0) {
-      const T* src = B + (k0 + b_c) + (j0 + b_r) * d.ldb;
+      const T* src = B + (k0 + b_c + boff(k0)) + (j0 + b_r) * d.ldb;
 #pragma unroll
       for (int e = 0; e < EB; ++e) rb[e] = src[e];
     } else {
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmDesc d) {
 #pragma unroll
       for (int e = 0; e < EB; ++e) {
         const int64_t k = k0 + b_c + e;
-        rb[e] = (j < d.N && k < ke) ? B[k + j * d.ldb] : T(0);
+        rb[e] = (j < d.N && k < ke) ? B[k + boff(k0) + j * d.ldb] : T(0);
       }
     } else {
       const int64_t k = k0 + b_r;
@@ -703,7 +706,7 @@ int launch_tile(const GemmDesc& d0, hipStream_t s) {
   GemmDesc d = d0;
   int split = 1;
   const int64_t wgs = tiles * d0.batch;
-  if (!g_capture && !d0.lowerC && wgs < 512 && d0.K >= 1024) {
+  if (!g_capture && !d0.lowerC && d0.cGapAt < 0 && wgs < 512 && d0.K >= 1024) {
     split = (int)std::min<int64_t>(std::min<int64_t>(32, cdiv(1024, wgs)), d0.K / 512);
     if (split < 2) split = 1;
   }
@@ -823,6 +826,15 @@ int gemm(Prec p, const GemmDesc& d, hipStream_t s) {
   if (cdiv(d.M, BM) * cdiv(d.N, BN) > 0x7fffffffLL || d.batch > 65535) {
     set_error("gemm: grid too large");
     return kErrArg;
+  }
+  if (d.bGapAt >= 0 || d.cGapAt >= 0) {
+    const bool ok = (p == Prec::F64 || p == Prec::F32) &&
+                    (d.bGapAt < 0 || (d.transB == 0 && d.bGapAt % 16 == 0 && d.aLo < 0 && d.bLo < 0)) &&
+                    (d.cGapAt < 0 || d.cGapAt % 128 == 0) && !d.mapA && !d.mapB && !d.mapC;
+    if (!ok) {
+      set_error("gemm: unsupported row-range gap");
+      return kErrArg;
+    }
   }
   switch (p) {
     case Prec::F64: return launch_typed<double>(d, s);

@@ -102,13 +102,24 @@ int apply_block_reflector(const Ctx& c, const T* V, int64_t ldv, int64_t sV, con
                           int64_t ncols, bool transpose, T* W, T* W2, int64_t sW, int batch) {
   const bool one = (off2 == len1) || len2 == 0;  // a single contiguous row range
   const int64_t m1 = (off2 == len1) ? len1 + len2 : len1;
+  const Prec pr = prec_for<T>(c);
+  // two row ranges in one launch where the engine takes them (FP64 / FP32):
+  // rows [len1, len1 + len2) of V are rows off2 .. of C
+  const bool gapB = !one && (pr == Prec::F64 || pr == Prec::F32) && len1 % 16 == 0;
+  const bool gapC = gapB && len1 % 128 == 0;
   GemmDesc d;  // W = V^T C
   d.batch = batch;
-  d.transA = 1; d.M = jb; d.N = ncols; d.K = m1;
+  d.transA = 1; d.M = jb; d.N = ncols; d.K = gapB ? len1 + len2 : m1;
   d.A = V; d.lda = ldv; d.strideA = sV; d.B = Cm; d.ldb = ldc; d.strideB = sC;
   d.C = W; d.ldc = jb; d.strideC = sW; d.aLo = 0;
+  if (gapB) {  // V2[k][i] = 0 for k > p + i: in the joined depth, k > len1 + p + i
+    d.aLo = -1;
+    d.aHi = (int)(len1 + p);
+    d.bGapAt = len1;
+    d.bGap = off2 - len1;
+  }
   ZOLO_CHECK(gemm_t<T>(c, d));
-  if (!one) {  // W += V2^T C2   (V2[k][i] = 0 for k > p + i)
+  if (!one && !gapB) {  // W += V2^T C2   (V2[k][i] = 0 for k > p + i)
     d = GemmDesc();
     d.batch = batch;
     d.transA = 1; d.M = jb; d.N = ncols; d.K = len2;
@@ -126,11 +137,16 @@ int apply_block_reflector(const Ctx& c, const T* V, int64_t ldv, int64_t sV, con
   ZOLO_CHECK(gemm_t<T>(c, d));
   d = GemmDesc();  // C -= V W2
   d.batch = batch;
-  d.M = m1; d.N = ncols; d.K = jb;
+  d.M = gapC ? len1 + len2 : m1; d.N = ncols; d.K = jb;
   d.A = V; d.lda = ldv; d.strideA = sV; d.B = W2; d.ldb = jb; d.strideB = sW;
   d.C = Cm; d.ldc = ldc; d.strideC = sC; d.alpha = -1.0; d.beta = 1.0;
+  if (gapC) {  // V2[i][k] = 0 for i > p + k: in the joined rows, k < i - len1 - p
+    d.aLo = (int)(len1 + p);
+    d.cGapAt = len1;
+    d.cGap = off2 - len1;
+  }
   ZOLO_CHECK(gemm_t<T>(c, d));
-  if (!one) {  // C2 -= V2 W2
+  if (!one && !gapC) {  // C2 -= V2 W2
     d = GemmDesc();
     d.batch = batch;
     d.M = len2; d.N = ncols; d.K = jb;
