@@ -250,10 +250,13 @@ __global__ __launch_bounds__(kT) void chol_leaf_kernel(T* A, int64_t lda, int64_
   __shared__ T a[NB][NB + 1];
   __shared__ T li[NB][NB + 1];
   __shared__ int bad;
+  static_assert(NB == 64 && kT == 256, "one row per lane, four column groups");
   const int q = blockIdx.x;
   T* Aq = A + q * strideA;
   T* Lq = Li + q * strideL;
   const int t = threadIdx.x;
+  // thread (r, g): row r of the block, columns k = g (mod 4) of it
+  const int r = t & 63, g = t >> 6;
   if (t == 0) bad = 0;
   for (int e = t; e < nb * nb; e += kT) {
     const int i = e % nb, j = e / nb;
@@ -261,6 +264,8 @@ __global__ __launch_bounds__(kT) void chol_leaf_kernel(T* A, int64_t lda, int64_
     li[i][j] = (T)0;
   }
   __syncthreads();
+  // right-looking Cholesky: per column the pivot, the scaled column, the
+  // rank-1 update of the lower trailing block (row r, columns k = g mod 4)
   for (int j = 0; j < nb; ++j) {
     if (t == 0) {
       const T d = a[j][j];
@@ -272,13 +277,11 @@ __global__ __launch_bounds__(kT) void chol_leaf_kernel(T* A, int64_t lda, int64_
       }
     }
     __syncthreads();
-    const T dj = a[j][j];
-    for (int i = j + 1 + t; i < nb; i += kT) a[i][j] /= dj;
+    if (g == 0 && r > j && r < nb) a[r][j] /= a[j][j];
     __syncthreads();
-    const int m = nb - j - 1;
-    for (int e = t; e < m * m; e += kT) {
-      const int ii = j + 1 + e % m, kk = j + 1 + e / m;
-      if (ii >= kk) a[ii][kk] -= a[ii][j] * a[kk][j];
+    if (r > j && r < nb) {
+      const T arj = a[r][j];
+      for (int k = j + 1 + ((g - j - 1) & 3); k <= r; k += 4) a[r][k] -= arj * a[k][j];
     }
     __syncthreads();
   }
@@ -291,10 +294,9 @@ __global__ __launch_bounds__(kT) void chol_leaf_kernel(T* A, int64_t lda, int64_
   for (int k = 0; k < nb; ++k) {
     if (t <= k) li[k][t] /= a[k][k];
     __syncthreads();
-    const int m = nb - k - 1;
-    for (int e = t; e < m * (k + 1); e += kT) {
-      const int i = k + 1 + e % m, c = e / m;
-      li[i][c] -= a[i][k] * li[k][c];
+    if (r > k && r < nb) {
+      const T ark = a[r][k];
+      for (int c = g; c <= k; c += 4) li[r][c] -= ark * li[k][c];
     }
     __syncthreads();
   }
