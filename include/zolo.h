@@ -182,8 +182,10 @@ int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* L
  * CON - 1 is below the FP32 unit roundoff.
  * l0_source: 0 = the LU estimate, 1 = the learned estimate (zolo_l0_set_weights
  * first), 2 = beta_in[i] from the caller.  With 1 and 2, a matrix whose
- * ||U^T U - I||_F / sqrt(n_i) exceeds orth_tol is recomputed with the LU
- * estimate.  rk > 0 overrides choosem.  tile: -1 autotune per bucket (cached
+ * ||U^T U - I||_F / sqrt(n_i) exceeds orth_tol falls back to the LU estimate:
+ * it restarts from its iterate with the LU estimate of that iterate (below),
+ * or, when the iterate is not finite, is recomputed from A with the LU
+ * estimate of A.  rk > 0 overrides choosem.  tile: -1 autotune per bucket (cached
  * per process), 0 automatic, 1..4 fixed (128x128, 64x128, 128x64, 64x64).
  * use_graph = 0 launches eagerly.  Per-matrix outputs (host arrays, each may be
  * NULL): beta_out, orth_out, iters_out, flags_out (bit 0: fallback used,
@@ -196,7 +198,7 @@ int zolo_chol_inv_device(int precision, int64_t n, void* A, int64_t lda, void* L
 typedef struct zolo_batch_info {
   int count;             /* matrices                                              */
   int buckets;           /* size buckets (n rounded up to a multiple of 32)       */
-  int fallbacks;         /* matrices recomputed with the LU estimate              */
+  int fallbacks;         /* matrices whose given bound failed (LU estimate used)  */
   int failures;          /* matrices failing the orthogonality check at the end   */
   int max_iters;         /* largest iteration count                               */
   int graph_nodes;       /* nodes of the captured graphs                          */

@@ -104,13 +104,21 @@ def test_batched_fallback_on_overestimated_bound(zl, torch_gpu):
     beta_given = np.where(bad, np.minimum(beta * 1e4, 1.0), beta)
     Us, info, _, _ = _run(zl, mats, l0="given", beta=beta_given)
     # an overestimated sigma_min leaves singular values outside the planned
-    # interval: the orthogonality check fails and the LU estimate takes over
+    # interval: the orthogonality check fails and the LU estimate takes over,
+    # from the iterate (same polar factor) when the given plan ran at least one
+    # step, else (beta_given = 1: nothing to iterate) from A
     assert np.array_equal(info.fallback, bad)
+    assert not info.continued[~bad].any()
+    assert info.continued[bad].sum() >= 2
     assert not info.failed.any()
     for k, (A32, U32) in enumerate(zip(mats, Us)):
         _check_polar(A32, U32)
-        if bad[k]:
+        if bad[k] and not info.continued[k]:
+            assert beta_given[k] == 1.0
             assert info.beta[k] == pytest.approx(beta[k], rel=1e-12)
+        else:
+            # the bound the plan started from is reported
+            assert info.beta[k] == beta_given[k]
 
 
 def test_batched_restart_near_singular(zl, torch_gpu):

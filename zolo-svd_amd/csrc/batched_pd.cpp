@@ -16,8 +16,8 @@
 // launched once.  A matrix stops after the step whose CON - 1 is below the
 // FP32 unit roundoff (the same rule as the mixed-precision single-matrix
 // path); its orthogonality ||U^T U - I||_F / sqrt(n) is then measured in
-// FP64 and a matrix whose learned or supplied bound fails that check is
-// recomputed with the LU estimate.
+// FP64 and a matrix whose learned or supplied bound fails that check falls
+// back to the LU estimate (from its iterate when that is finite, else from A).
 //
 // Layout in HBM: matrices are grouped by nb = n rounded up to a multiple of
 // 32 and stored per bucket as nb x nb column-major slabs, padded as
@@ -713,11 +713,19 @@ int pd_batched_run(int count, const int64_t* n, const float* const* A, const int
 
   // ---- phase 4: fallback to the LU estimate ------------------------------------
   const auto t5 = clk::now();
+  // A matrix whose given (learned) bound fails the orthogonality check falls
+  // back to the LU estimate.  With a finite iterate it restarts from that
+  // iterate (same polar factor, far better conditioned than A) with the LU
+  // estimate of the iterate: the continuation below, pzolopd2.f90:397-405.
+  // Only a non-finite iterate is recomputed from A with the LU estimate of A.
   // (a fallback call runs its own continuation; the merge keeps its counts)
   if (opt.l0_source != 0) {
     std::vector<int> redo;
-    for (int i = 0; i < count; ++i)
-      if (!(res.orth[i] <= opt.orth_tol)) redo.push_back(i);
+    for (int i = 0; i < count; ++i) {
+      if (res.orth[i] <= opt.orth_tol) continue;
+      res.fallback[i] = 1;
+      if (!(std::isfinite(res.orth[i]) && res.iters[i] > 0 && opt.continue_rounds > 0)) redo.push_back(i);
+    }
     if (!redo.empty()) {
       std::vector<int64_t> rn, rla, rlu;
       std::vector<const float*> rA;

@@ -148,56 +148,76 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
 //
 // Per column: the wave column sums give the Gram row g_c = sum_{i>j} S[i][j]
 // S[i][j+c] (lane 0: the squared norm below the diagonal); after a workgroup
-// barrier wave 0 adds the 8 wave sums.  With one workgroup per shift it forms
+// barrier wave 0 adds the wave sums.  With one workgroup per shift it forms
 // the reflector (dlarfg) and w_c = tau (r_c + scal g_c) at once; after a
 // second barrier every thread applies the reflector to its row.
 //
-// With several workgroups per shift, wave 0 exchanges the sums: it stores the
-// workgroup's 64 sums (workgroup 0 also the pivot row j, its row jj) to this
-// column's slot, waits until those stores are complete, raises the
-// workgroup's flag to jj + 1, and polls the nwg flags of the shift together
-// (lane k reads flag k) until all have reached jj + 1.  After a barrier every
-// wave loads its share of the slots (wave w: workgroups w, w + 8, ...), and
-// after another wave 0 adds the 8 shares and forms the reflector.  All sums
-// run in a fixed order, so the results do not depend on scheduling, and no
-// counter is shared by the workgroups.  Slots and flags are only accessed by
-// device-scope atomic loads and stores, which gfx950 performs at the device's
-// coherence point (stores written through, loads not served from an
-// incoherent cache); the vmcnt(0) wait before the flag store makes the slot
-// stores complete first, and the slot loads are issued only after the flag
-// loads have returned.  That orders the exchange without a release/acquire
-// pair, whose L2 writeback and invalidate cost the per-column exchange about
-// 2 us at 32 workgroups.  The slots alternate between two buffers: a
-// workgroup writes column jj's slot only after every workgroup has raised its
-// flag for column jj - 1, which each does after it has read the slots of
-// column jj - 2.  Shifts q0 .. q0 + gridDim.y - 1; flags holds
-// kCoopFlagStride zeroed words per shift, or is null when nwg == 1.  A poll
-// that outlasts kCoopSpinCap rounds gives up (the results are then wrong, but
-// the kernel ends), so a launch that is not co-resident cannot hang the device.
-constexpr int kCoopT = 512;
-constexpr int kCoopWaves = kCoopT / 64;
-constexpr int kCoopMaxWg = 64;        // workgroups per shift: one flag per lane of a wave
+// Workgroups of TT = 256 threads (one wave per SIMD, for launches that leave
+// CUs free) or 512 threads (two waves per SIMD, for grids that fill the
+// device).  With several workgroups per shift the sums are exchanged through
+// device memory, in one of two ways.  All sums run in a fixed order, so the
+// results do not depend on scheduling, and no counter is shared by the
+// workgroups.  Slots and flags are only accessed by device-scope atomic loads
+// and stores, which gfx950 performs at the device's coherence point (stores
+// written through, loads not served from an incoherent cache), so neither way
+// needs the L2 writeback and invalidate of a release/acquire pair.
+//
+// Slot polling (TT = 256): every column has a record of its own, filled with
+// kSlotEmpty before the launch (a value the stores never write: a sum with
+// that bit pattern is stored as the default NaN).  Wave 0 stores the
+// workgroup's sums (workgroup 0 also the pivot row j, its row jj) to the
+// column's record; every wave loads its share of the slots (wave w:
+// workgroups w, w + 4, ...) and reloads the empty ones until none is left,
+// then wave 0 adds the shares.  One store-to-load round trip per column.
+//
+// Flags (TT = 512): wave 0 stores the sums to the column's slot, waits until
+// those stores are complete (vmcnt(0)), raises the workgroup's flag to jj + 1
+// and polls the nwg flags of the shift together (lane k reads flag k) until
+// all have reached jj + 1; after a barrier every wave loads its share of the
+// slots, and after another wave 0 adds the shares.  Two round trips, but the
+// pollers read one line per shift instead of every slot, which matters when
+// every CU holds a workgroup that polls.  The slots alternate between two
+// records: a workgroup writes column jj's slot only after every workgroup has
+// raised its flag for column jj - 1, which each does after it has read the
+// slots of column jj - 2.
+//
+// Shifts q0 .. q0 + gridDim.y - 1; flags holds kCoopFlagStride zeroed words
+// per shift (flag exchange) or is null.  A poll that outlasts its spin cap
+// gives up (the results are then wrong, but the kernel ends), so a launch
+// that is not co-resident cannot hang the device.
+constexpr int kCoopMaxWg = 64;        // workgroups per shift: one flag per lane of a wave, 16 slots per wave
 constexpr int kCoopFlagStride = 64;   // unsigned words between the flags of two shifts (one 256-byte line)
 constexpr unsigned kCoopSpinCap = 1u << 24;
 // s_waitcnt operand (gfx9 encoding): vmcnt(0), expcnt and lgkmcnt not waited for
 constexpr int kWaitVm0 = 0x0F70;
 
-template <typename T>
-__global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
+// Slot value that no partial sum is stored as: the launcher fills the slot
+// records with it, and a reader polls a slot until it holds something else.
+constexpr long long kSlotEmpty = -1LL;  // all bits set: a NaN
+constexpr unsigned kSlotSpinCap = 1u << 20;
+__device__ inline bool slot_empty(double x) { return __double_as_longlong(x) == kSlotEmpty; }
+__device__ inline double slot_value(double x) {
+  return slot_empty(x) ? __longlong_as_double(0x7FF8000000000000LL) : x;
+}
+
+template <typename T, int TT>
+__global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
                                                                int64_t p, int jb, int64_t r1_end, int64_t r2_end,
                                                                double* part, unsigned* flags, int nwg, T* tau,
                                                                int64_t strideTau, int q0) {
-  __shared__ double red[kCoopWaves][NBP];
+  constexpr int kCoopWaves = TT / 64;
+  constexpr bool XM = TT <= 256;  // exchange by polling the slots (else by flags)
+  __shared__ double red[kCoopWaves][NBP], red2[kCoopWaves][NBP];
   __shared__ double wv[NBP], prj[NBP];
   __shared__ double sc[2];
   const int qi = blockIdx.y, q = q0 + qi;
   T* Sq = S + (int64_t)q * strideS;
   const int t = threadIdx.x, w = blockIdx.x, lane = t & 63, wave = t >> 6;
-  const int64_t l = (int64_t)w * kCoopT + t, len1 = r1_end - p;
+  const int64_t l = (int64_t)w * TT + t, len1 = r1_end - p;
   const int64_t i = l < len1 ? p + l : n + (l - len1);
   const bool valid = l < len1 + (r2_end - n);
   const int64_t rec = (int64_t)nwg * NBP + NBP;
-  double* base = part + (int64_t)qi * 2 * rec;
+  double* base = part + (int64_t)qi * (XM ? NBP : 2) * rec;
   const unsigned* flr = flags ? flags + (int64_t)qi * kCoopFlagStride : nullptr;
   unsigned* flw = flags ? flags + (int64_t)qi * kCoopFlagStride + w : nullptr;
   // v[c]: the row's value in panel column jj + c; zero for jj + c >= jb
@@ -236,7 +256,57 @@ __global__ __launch_bounds__(kCoopT) void qr_panel_coop_kernel(T* S, int64_t lds
       for (int k = 0; k < kCoopWaves; ++k) g += red[k][lane];
       pr = prj[lane];
     }
-    if (flr) {
+    if constexpr (XM) {
+      if (nwg > 1) {
+      // one record per column, filled with kSlotEmpty before the launch:
+      // wave 0 stores the workgroup's sums, every wave polls its share of the
+      // slots until none is empty
+      double* G = base + (int64_t)jj * rec;
+      if (wave == 0 && lane < rem) {
+        __hip_atomic_store(G + (int64_t)w * NBP + lane, slot_value(g), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (w == 0)
+          __hip_atomic_store(G + (int64_t)nwg * NBP + lane, slot_value(pr), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      }
+      constexpr int M = kCoopMaxWg / kCoopWaves;
+      double sv[M];
+      const double* Gl = G + lane;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int k = wave + m * kCoopWaves;
+        sv[m] = (k < nwg && lane < rem) ? __hip_atomic_load(Gl + (int64_t)k * NBP, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT)
+                                        : 0.0;
+      }
+      double pv = (wave == 0 && lane < rem)
+                      ? __hip_atomic_load(Gl + (int64_t)nwg * NBP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                      : 0.0;
+      for (unsigned spin = 0; spin < kSlotSpinCap; ++spin) {
+        bool miss = slot_empty(pv);
+#pragma unroll
+        for (int m = 0; m < M; ++m) miss = miss || slot_empty(sv[m]);
+        if (!__any(miss)) break;
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+          if (slot_empty(sv[m]))
+            sv[m] = __hip_atomic_load(Gl + (int64_t)(wave + m * kCoopWaves) * NBP, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        if (slot_empty(pv)) pv = __hip_atomic_load(Gl + (int64_t)nwg * NBP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < M; ++m) s += sv[m];
+      red2[wave][lane] = s;
+      __syncthreads();
+      if (wave == 0) {
+        g = 0.0;
+#pragma unroll
+        for (int k = 0; k < kCoopWaves; ++k) g += red2[k][lane];
+        pr = pv;
+      }
+      }
+    } else if (flr) {
       double* G = base + (int64_t)(jj & 1) * rec;
       const unsigned target = (unsigned)jj + 1u;
       if (wave == 0) {
@@ -502,6 +572,70 @@ int k_qr_panel_step(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   return kOk;
 }
 
+// Workgroups of 256 threads (one wave per SIMD) where the launch leaves CUs
+// idle, so that each wave issues its column sums and updates alone; 512
+// threads (two waves per SIMD, half as many workgroups to exchange with)
+// where the grid would fill the device anyway.
+template <typename T, int TT>
+int coop_cap(int dev) {
+  static int cap[64] = {0};
+  if (dev < 0 || dev >= 64) return -1;
+  if (cap[dev] == 0) {
+    // -1 (no cooperative launch) when the device cannot tell
+    int per_cu = 0, cus = 0, coop = 0;
+    const bool ok =
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)qr_panel_coop_kernel<T, TT>, TT, 0) ==
+            hipSuccess;
+    cap[dev] = ok && coop && per_cu * cus > 0 ? per_cu * cus : -1;
+  }
+  return cap[dev];
+}
+
+template <typename T, int TT>
+int coop_launch(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
+                double* part, int64_t part_doubles, T* tau, int64_t strideTau, int batch, int nwg, int cap,
+                hipStream_t s) {
+  const int per = std::min(batch, cap / nwg);
+  const int64_t rec = (int64_t)nwg * NBP + NBP;
+  if constexpr (TT <= 256) {
+    // one slot record per panel column and shift, all empty at the start
+    if ((int64_t)per * NBP * rec > part_doubles) {
+      set_error("qr panel (cooperative): partials buffer too small");
+      return kErrArg;
+    }
+    for (int q0 = 0; q0 < batch; q0 += per) {
+      const int nq = std::min(per, batch - q0);
+      int nw = nwg;
+      unsigned* noflags = nullptr;
+      ZOLO_HIP(hipMemsetAsync(part, 0xFF, sizeof(double) * (size_t)(nq * NBP * rec), s));
+      void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &noflags, &nw, &tau, &strideTau,
+                      (void*)&q0};
+      ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T, TT>, dim3(nwg, nq), dim3(TT), args,
+                                          0, s));
+    }
+  } else {
+    // two alternating slot records per shift, then the flags of each shift on
+    // a 256-byte line of their own
+    if ((int64_t)per * 2 * rec + (int64_t)per * kCoopFlagStride / 2 + 32 > part_doubles) {
+      set_error("qr panel (cooperative): partials buffer too small");
+      return kErrArg;
+    }
+    unsigned* flags = reinterpret_cast<unsigned*>(part + ((int64_t)per * 2 * rec + 31) / 32 * 32);
+    for (int q0 = 0; q0 < batch; q0 += per) {
+      const int nq = std::min(per, batch - q0);
+      int nw = nwg;
+      ZOLO_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * kCoopFlagStride * nq, s));
+      void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &flags, &nw, &tau, &strideTau,
+                      (void*)&q0};
+      ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T, TT>, dim3(nwg, nq), dim3(TT), args,
+                                          0, s));
+    }
+  }
+  return kOk;
+}
+
 template <typename T>
 int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
                     double* part, int64_t part_doubles, T* tau, int64_t strideTau, int batch, hipStream_t s,
@@ -511,8 +645,8 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
     set_error("qr panel (cooperative): width or row ranges out of range");
     return kErrArg;
   }
-  const int64_t rows0 = (r1_end - p) + (r2_end - n);
-  if (rows0 <= kCoopT) {
+  const int64_t rows = (r1_end - p) + (r2_end - n);
+  if (rows <= 512) {
     // one workgroup per shift: no barrier between workgroups, an ordinary
     // launch (also inside a graph capture)
     if ((int64_t)batch * 2 * (2 * NBP) > part_doubles) {
@@ -522,8 +656,12 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
     int one = 1;
     unsigned* noflags = nullptr;
     int q0 = 0;
-    hipLaunchKernelGGL((qr_panel_coop_kernel<T>), dim3(1, batch), dim3(kCoopT), 0, s, S, lds, strideS, n, p, jb,
-                       r1_end, r2_end, part, noflags, one, tau, strideTau, q0);
+    if (rows <= 256)
+      hipLaunchKernelGGL((qr_panel_coop_kernel<T, 256>), dim3(1, batch), dim3(256), 0, s, S, lds, strideS, n, p, jb,
+                         r1_end, r2_end, part, noflags, one, tau, strideTau, q0);
+    else
+      hipLaunchKernelGGL((qr_panel_coop_kernel<T, 512>), dim3(1, batch), dim3(512), 0, s, S, lds, strideS, n, p, jb,
+                         r1_end, r2_end, part, noflags, one, tau, strideTau, q0);
     ZOLO_HIP(hipGetLastError());
     *done = true;
     return kOk;
@@ -533,40 +671,20 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
     ZOLO_HIP(hipStreamIsCapturing(s, &cs));
     if (cs != hipStreamCaptureStatusNone) return kOk;
   }
-  // workgroups one launch may keep resident, per device
-  static int cap[64] = {0};
-  int dev = 0;
+  int dev = 0, cus = 0;
   ZOLO_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64) return kOk;
-  if (cap[dev] == 0) {
-    int per_cu = 0, cus = 0, coop = 0;
-    ZOLO_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-    ZOLO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    ZOLO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)qr_panel_coop_kernel<T>, kCoopT, 0));
-    cap[dev] = coop ? std::max(per_cu * cus, -1) : -1;
-    if (cap[dev] == 0) cap[dev] = -1;
-  }
-  const int64_t rows = (r1_end - p) + (r2_end - n);
-  const int nwg = (int)((rows + kCoopT - 1) / kCoopT);
-  // too large for one resident grid or for the partial sums: the caller takes the per-column kernels
-  if (cap[dev] < nwg || nwg > kCoopMaxWg) return kOk;
-  const int per = std::min(batch, cap[dev] / nwg);
-  const int64_t rec = (int64_t)nwg * NBP + NBP;
-  // partial slots of the group, then the flags of each shift on a 256-byte
-  // line of their own
-  if ((int64_t)per * 2 * rec + (int64_t)per * kCoopFlagStride / 2 + 32 > part_doubles) {
-    set_error("qr panel (cooperative): partials buffer too small");
-    return kErrArg;
-  }
-  unsigned* flags = reinterpret_cast<unsigned*>(part + ((int64_t)per * 2 * rec + 31) / 32 * 32);
-  for (int q0 = 0; q0 < batch; q0 += per) {
-    const int nq = std::min(per, batch - q0);
-    int nw = nwg;
-    ZOLO_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * kCoopFlagStride * nq, s));
-    void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &flags, &nw, &tau, &strideTau,
-                    (void*)&q0};
-    ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T>, dim3(nwg, nq), dim3(kCoopT), args, 0,
-                                        s));
+  ZOLO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int nwg256 = (int)((rows + 255) / 256), nwg512 = (int)((rows + 511) / 512);
+  const int cap256 = coop_cap<T, 256>(dev), cap512 = coop_cap<T, 512>(dev);
+  if (nwg256 <= kCoopMaxWg && (int64_t)batch * nwg256 <= cus && cap256 >= nwg256) {
+    ZOLO_CHECK(coop_launch<T, 256>(S, lds, strideS, n, p, jb, r1_end, r2_end, part, part_doubles, tau, strideTau,
+                                   batch, nwg256, cap256, s));
+  } else if (nwg512 <= kCoopMaxWg && cap512 >= nwg512) {
+    ZOLO_CHECK(coop_launch<T, 512>(S, lds, strideS, n, p, jb, r1_end, r2_end, part, part_doubles, tau, strideTau,
+                                   batch, nwg512, cap512, s));
+  } else {
+    // too large for one resident grid or for the partial sums: the caller takes the per-column kernels
+    return kOk;
   }
   *done = true;
   return kOk;

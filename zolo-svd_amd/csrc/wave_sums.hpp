@@ -8,6 +8,34 @@
 
 namespace zolo {
 
+// The partner lane of a butterfly stage of width H <= 8, read with a DPP
+// move (no LDS round trip): row_mirror (lane i of a 16-lane row reads lane
+// 15 - i) for H = 8, row_half_mirror (7 - i within 8 lanes) for H = 4, the
+// quad permutations [2,3,0,1] and [1,0,3,2] for H = 2 and 1.  Each pairs a
+// lane with one whose bit H differs and whose higher bits agree, which is all
+// the butterfly needs: after the four stages lane l holds the sum over its
+// 16-lane row of column l & 15, as with lane ^ H partners.
+template <int H>
+constexpr int kDppPair = H == 8 ? 0x140 : H == 4 ? 0x141 : H == 2 ? 0x4E : 0xB1;
+template <int H>
+__device__ inline unsigned dpp_pair(unsigned x) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, kDppPair<H>, 0xF, 0xF, true);
+}
+template <int H>
+__device__ inline double lane_pair(double x) {
+  if constexpr (H <= 8) {
+    return __hiloint2double((int)dpp_pair<H>((unsigned)__double2hiint(x)),
+                            (int)dpp_pair<H>((unsigned)__double2loint(x)));
+  } else {
+    return __shfl_xor(x, H, 64);
+  }
+}
+template <int H>
+__device__ inline float lane_pair(float x) {
+  if constexpr (H <= 8) return __uint_as_float(dpp_pair<H>(__float_as_uint(x)));
+  else return __shfl_xor(x, H, 64);
+}
+
 // One shuffle stage: lanes with bit H set keep columns j + H, the others j.
 template <int H, typename V>
 __device__ inline void butterfly_step(V* a, int lane) {
@@ -16,7 +44,7 @@ __device__ inline void butterfly_step(V* a, int lane) {
   for (int j = 0; j < H; ++j) {
     const V keep = up ? a[j + H] : a[j];
     const V send = up ? a[j] : a[j + H];
-    a[j] = keep + __shfl_xor(send, H, 64);
+    a[j] = keep + lane_pair<H>(send);
   }
 }
 template <typename V>
