@@ -11,12 +11,16 @@
 //             (a tiled layout whose K slabs are contiguous), three
 //             v_mfma_f32_16x16x32_bf16 products (lo*hi + hi*lo + hi*hi)
 //             accumulated in fp32; 256x256x32 tile of 8 waves of 128x64.
+//             Small products (M N K batch <= kB3MinWork, the deep levels of the
+//             recursive Cholesky and inverse) run on the F32 path instead:
+//             the two split launches and the 256x256 tile padding would cost
+//             more than the FP32 MFMA rate, and FP32 products are the more
+//             accurate of the two.
 //
 // Work distribution: tiles are numbered so that the 8 XCDs (round-robin
 // workgroup dispatch) each own a contiguous block of tiles, and inside a block
 // tiles walk down groups of 8 tile-rows so neighbouring workgroups share the
 // same A and B panels in their XCD's L2.
-#include <algorithm>
 #include <algorithm>
 #include <vector>
 
@@ -389,6 +393,7 @@ constexpr int kSplitTile = 64;
 constexpr int kB3K = 32;            // K depth of one bf16x3 slab and of a layout tile
 constexpr int kB3Rows = 128;        // rows of a layout tile
 constexpr int kB3M = 256, kB3N = 256, kB3Threads = 512;  // workgroup tile: 8 waves of 128 x 64
+constexpr double kB3MinWork = 2147483648.0;  // M N K batch at or below which BF16X3 takes the F32 path
 
 // Offset of element (r, k) of a split operand in its tiled layout: tiles of
 // 128 rows x 32 k (8 KB of bf16, row-major inside), ordered k-fastest within a
@@ -839,7 +844,9 @@ int gemm(Prec p, const GemmDesc& d, hipStream_t s) {
   switch (p) {
     case Prec::F64: return launch_typed<double>(d, s);
     case Prec::F32: return launch_typed<float>(d, s);
-    case Prec::BF16X3: return launch_bf16x3(d, s);
+    case Prec::BF16X3:
+      if ((double)d.M * (double)d.N * (double)d.K * (double)d.batch <= kB3MinWork) return launch_typed<float>(d, s);
+      return launch_bf16x3(d, s);
     case Prec::F64X: {
       if (!gemm_ozaki_supported(d) || d.K <= 0) return launch_typed<double>(d, s);
       LedgerScope ls(d, 3, 256, 256, s);
