@@ -31,6 +31,22 @@ namespace {
 constexpr int kT = 256;
 constexpr int kLd = kSbNb + 1;  // LDS leading dimension of the 64 x 64 blocks
 
+// The band store is shared by the workgroups of sb2st_kernel (each task reads
+// what the previous sweep's workgroup wrote): inside that kernel every access
+// to it is a device-scope load or store, which gfx950 performs at the
+// device's coherence point (stores written through, loads not served from an
+// incoherent cache).  A workgroup publishes a task after its stores have
+// completed (vmcnt(0) in every thread, then a barrier), so neither side needs
+// the L2 writeback and invalidate of a release/acquire pair.
+__device__ inline double band_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void band_st(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt operand (gfx9 encoding): vmcnt(0), expcnt and lgkmcnt not waited for
+constexpr int kWaitVm0 = 0x0F70;
+
 __device__ inline double wsum(double x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -124,7 +140,6 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
             }
             __builtin_amdgcn_s_sleep(1);
           }
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         flag = ok;
       }
@@ -138,11 +153,11 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
         const int u = wv + 4 * r;
         const bool ind = u < L && lane >= u && lane < L;
         const bool ine = u < L && lane < L2;
-        dreg[r] = ind ? LB[(s + u) * ldb + (lane - u)] : 0.0;
-        ereg[r] = ine ? LB[(s + u) * ldb + (B + lane - u)] : 0.0;
+        dreg[r] = ind ? band_ld(LB + (s + u) * ldb + (lane - u)) : 0.0;
+        ereg[r] = ine ? band_ld(LB + (s + u) * ldb + (B + lane - u)) : 0.0;
       }
       double x = 0.0;
-      if (k == 0 && wv == 0 && lane < L) x = LB[(int64_t)i * ldb + 1 + lane];
+      if (k == 0 && wv == 0 && lane < L) x = band_ld(LB + (int64_t)i * ldb + 1 + lane);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int u = wv + 4 * r;
@@ -163,7 +178,7 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
           v = lane == 0 ? 1.0 : x * (1.0 / (alpha - beta));
         }
         vs[0][lane] = lane < L ? v : 0.0;
-        if (lane < L) LB[(int64_t)i * ldb + 1 + lane] = lane == 0 ? beta : 0.0;
+        if (lane < L) band_st(LB + (int64_t)i * ldb + 1 + lane, lane == 0 ? beta : 0.0);
         if (lane == 0) {
           misc[0][0] = tau;
           misc[0][1] = beta;
@@ -228,7 +243,7 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
         }
         vs[nx][lane] = lane < L2 ? v : 0.0;
         ereg[0] = lane == 0 ? bn : 0.0;
-        if (lane < L2) LB[s * ldb + B + lane] = ereg[0];   // column 0 of E_k, cleared
+        if (lane < L2) band_st(LB + s * ldb + B + lane, ereg[0]);  // column 0 of E_k, cleared
         const int64_t grp = (int64_t)j * a.kmax + (k + 1);
         if (lane < L2) a.vstore[(grp * a.g + q) * a.span + q + lane] = v;
         if (lane == 0) {
@@ -240,16 +255,21 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int u = wv + 4 * r;
-        if (u < L && lane >= u && lane < L) LB[(s + u) * ldb + (lane - u)] = dreg[r] - (vl * ws[u] + wl * vu[r]);
+        if (u < L && lane >= u && lane < L) band_st(LB + (s + u) * ldb + (lane - u), dreg[r] - (vl * ws[u] + wl * vu[r]));
         if (k > 0 && lane < L) {
           const double y = u == 0 ? (lane == 0 ? beta : 0.0) : Ep[u * kLd + lane] - vl * es[u];
-          LB[(s - B + u) * ldb + (B + lane - u)] = y;
+          band_st(LB + (s - B + u) * ldb + (B + lane - u), y);
         }
         En[u * kLd + lane] = ereg[r];
       }
+      // this thread's band stores are complete before the barrier, so all of
+      // the task's are before the progress store
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(kWaitVm0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __syncthreads();  // (5)
       if (threadIdx.x == 0)
-        __hip_atomic_store(a.progress + i, k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.progress + i, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       double* tmp = Ep;
       Ep = En;
       En = tmp;
