@@ -82,15 +82,15 @@ size_t two_stage_bytes(int64_t n) {
   by += sizeof(double) * (size_t)(n + span + b + g) * n;              // padded eigenvectors
   by += 2 * sizeof(double) * (size_t)J * g * n;                       // W, W2 of one wavefront
   // stage 1: panel QR scratch and the block vectors of the two-sided update
-  by += dense_qr_bytes<double>(n) + sizeof(double) * (size_t)(3 * n * b + 4 * b * b + 256 * 256);
+  by += dense_qr_bytes<double>(n) + sizeof(double) * (size_t)(4 * n * b + 4 * b * b + 256 * 256);
   return by + 64 * 256;
 }
 
 // Full -> band (bandwidth b): for every panel of b columns, Householder QR of
 // the part below the band, then the two-sided update of the trailing matrix
 // A22 <- Q^T A22 Q with Q = I - V T V^T:  Y = A22 V T, M = V^T Y,
-// Y <- Y - V (T^T M)/2, A22 <- A22 - V Y^T - Y V^T.  The reflectors stay
-// below R in H; their tau in ts.tau1.
+// Y <- Y - V (T^T M)/2, A22 <- A22 - V Y^T - Y V^T (one product of K = 2b).
+// The reflectors stay below R in H; their tau in ts.tau1.
 int sy2sb(const Ctx& c, double* H, int64_t ldh, int64_t n, TwoStage& ts, Arena& ar) {
   const int b = kSbNb;
   const size_t mark = ar.used();
@@ -102,9 +102,12 @@ int sy2sb(const Ctx& c, double* H, int64_t ldh, int64_t n, TwoStage& ts, Arena& 
     const int kk = (int)std::min<int64_t>(m, b);
     double* P = H + r0 + k0 * ldh;
     double* Tall = ar.take<double>(256 * 256);
-    double* V = ar.take<double>((size_t)m * b);
+    // [V | Y | V] side by side (ld m): [V Y] and [Y V] are the two operands of
+    // the rank-2b update
+    double* V = ar.take<double>((size_t)m * 3 * b);
+    double* Y = V + (size_t)m * b;
+    double* V2 = Y + (size_t)m * b;
     double* VT = ar.take<double>((size_t)m * b);
-    double* Y = ar.take<double>((size_t)m * b);
     double* M = ar.take<double>((size_t)b * b);
     double* Wm = ar.take<double>((size_t)b * b);
     ZOLO_ARENA_OK(ar);
@@ -147,12 +150,21 @@ int sy2sb(const Ctx& c, double* H, int64_t ldh, int64_t n, TwoStage& ts, Arena& 
     d.M = m; d.N = kk; d.K = kk;
     d.A = V; d.lda = m; d.B = Wm; d.ldb = kk; d.C = Y; d.ldc = m; d.alpha = -0.5; d.beta = 1.0;
     ZOLO_CHECK(gemm(Prec::F64, d, c.s));
-    d = GemmDesc();  // A22 -= V Y^T
-    d.transB = 1; d.M = m; d.N = m; d.K = kk;
-    d.A = V; d.lda = m; d.B = Y; d.ldb = m; d.C = A22; d.ldc = ldh; d.alpha = -1.0; d.beta = 1.0;
-    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
-    d.A = Y; d.B = V;  // A22 -= Y V^T
-    ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    if (kk == b) {
+      // A22 -= [V Y] [Y V]^T = V Y^T + Y V^T: one pass over A22 with K = 2b
+      ZOLO_HIP(hipMemcpyAsync(V2, V, sizeof(double) * (size_t)m * b, hipMemcpyDeviceToDevice, c.s));
+      d = GemmDesc();
+      d.transB = 1; d.M = m; d.N = m; d.K = 2 * b;
+      d.A = V; d.lda = m; d.B = Y; d.ldb = m; d.C = A22; d.ldc = ldh; d.alpha = -1.0; d.beta = 1.0;
+      ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    } else {
+      d = GemmDesc();  // A22 -= V Y^T
+      d.transB = 1; d.M = m; d.N = m; d.K = kk;
+      d.A = V; d.lda = m; d.B = Y; d.ldb = m; d.C = A22; d.ldc = ldh; d.alpha = -1.0; d.beta = 1.0;
+      ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+      d.A = Y; d.B = V;  // A22 -= Y V^T
+      ZOLO_CHECK(gemm(Prec::F64, d, c.s));
+    }
   }
   ar.reset();
   ar.take<char>(mark);
