@@ -235,22 +235,22 @@ int back_two_stage(const Ctx& c, const double* H, int64_t ldh, int64_t n, double
       w.C = ts.W; w.ldc = g; w.strideC = (int64_t)g * n;
       w.aLo = 0; w.aHi = b - 1;
       ZOLO_CHECK(gemm(Prec::F64, w, c.s));
-      GemmDesc t2;  // W2 = T W   (T upper)
+      GemmDesc t2;  // VT = V T (span x g per group, in W2; T upper)
       t2.batch = cnt;
-      t2.M = g; t2.N = n; t2.K = g;
-      t2.A = ts.Ts + grp0 * g * g; t2.lda = g; t2.strideA = sgrp * g * g;
-      t2.B = ts.W; t2.ldb = g; t2.strideB = (int64_t)g * n;
-      t2.C = ts.W2; t2.ldc = g; t2.strideC = (int64_t)g * n;
-      t2.aLo = 0;
+      t2.M = span; t2.N = g; t2.K = g;
+      t2.A = ts.vstore + grp0 * g * span; t2.lda = span; t2.strideA = sgrp * g * span;
+      t2.B = ts.Ts + grp0 * g * g; t2.ldb = g; t2.strideB = sgrp * g * g;
+      t2.C = ts.W2; t2.ldc = span; t2.strideC = span * g;
+      t2.aLo = b - 1; t2.aHi = 0; t2.bHi = 0;
       ZOLO_CHECK(gemm(Prec::F64, t2, c.s));
-      GemmDesc u;  // Zp(rows) -= V W2
+      GemmDesc u;  // Zp(rows) -= (V T) W   (VT(r, q) = 0 for r > q + b - 1)
       u.batch = cnt;
       u.M = span; u.N = n; u.K = g;
-      u.A = ts.vstore + grp0 * g * span; u.lda = span; u.strideA = sgrp * g * span;
-      u.B = ts.W2; u.ldb = g; u.strideB = (int64_t)g * n;
+      u.A = ts.W2; u.lda = span; u.strideA = span * g;
+      u.B = ts.W; u.ldb = g; u.strideB = (int64_t)g * n;
       u.C = ts.Zp + row0; u.ldc = ldp; u.strideC = g + b;
       u.alpha = -1.0; u.beta = 1.0;
-      u.aLo = b - 1; u.aHi = 0;
+      u.aLo = b - 1;
       ZOLO_CHECK(gemm(Prec::F64, u, c.s));
     }
   }
