@@ -496,40 +496,83 @@ __global__ __launch_bounds__(kT) void qr_extract_v_kernel(const T* S, int64_t ld
   }
 }
 
-// T (jb x jb upper) of the block reflector from G = V^T V and tau (dlarft, forward columnwise).
+// T (jb x jb upper) of the block reflector from G = V^T V and tau (dlarft,
+// forward columnwise), in the recursive form: the four 16-column diagonal
+// blocks by the column recurrence T[0:i, i] = -tau_i T[0:i, 0:i] G[0:i, i],
+// all four at once, then pairs and halves joined by
+// T12 = -T1 (V1^T V2) T2 = -T1 G12 T2: 38 barriers instead of 128.  Columns
+// from jb to 63 are padded with tau = 0 and G = 0, which leaves them zero.
 template <typename T>
 __global__ __launch_bounds__(kT) void qr_form_t_kernel(const T* G, int64_t ldg, int64_t strideG, const T* tau,
                                                        int64_t strideTau, int64_t p, int jb, T* Tm, int64_t ldt,
                                                        int64_t strideT) {
+  static_assert(NBP == 64 && kT == 256, "four 16-column blocks of 64 threads");
   __shared__ double g[NBP][NBP + 1];
   __shared__ double tt[NBP][NBP + 1];
+  __shared__ double xx[NBP][NBP + 1];
   __shared__ double ts[NBP];
   const int q = blockIdx.x;
   const T* Gq = G + q * strideG;
   const T* tq = tau + q * strideTau;
   T* Tq = Tm + q * strideT;
   const int t = threadIdx.x;
-  for (int e = t; e < jb * jb; e += kT) {
-    const int r = e % jb, c = e / jb;
-    g[r][c] = (double)Gq[r + c * ldg];
+  for (int e = t; e < NBP * NBP; e += kT) {
+    const int r = e % NBP, c = e / NBP;
+    g[r][c] = (r < jb && c < jb) ? (double)Gq[r + c * ldg] : 0.0;
     tt[r][c] = 0.0;
   }
-  if (t < jb) ts[t] = (double)tq[p + t];  // all of tau up front: no global load inside the column loop
+  if (t < NBP) ts[t] = t < jb ? (double)tq[p + t] : 0.0;
   __syncthreads();
-  // column i: T[0:i, i] = -tau_i T[0:i, 0:i] G[0:i, i]; each row's dot is
-  // split over four lanes (k = row + part, row + part + 4, ...) and combined by
-  // two shuffles inside the group of four
-  const int row = t >> 2, part = t & 3;
-  for (int i = 0; i < jb; ++i) {
-    const double ti = ts[i];
-    double s = 0.0;
-    if (row < i)
-      for (int k = row + part; k < i; k += 4) s += tt[row][k] * g[k][i];
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
+  {  // the diagonal blocks: block bk = t / 64, row rr of it, each dot split over four lanes
+    const int bk = t >> 6, rr = (t >> 2) & 15, part = t & 3, o = bk * 16;
+    for (int i = 0; i < 16; ++i) {
+      const int col = o + i;
+      double s = 0.0;
+      if (rr < i)
+        for (int k = rr + part; k < i; k += 4) s += tt[o + rr][o + k] * g[o + k][col];
+      s += __shfl_xor(s, 1, 64);
+      s += __shfl_xor(s, 2, 64);
+      __syncthreads();
+      if (part == 0 && rr < i) tt[o + rr][col] = -ts[col] * s;
+      if (part == 0 && rr == i) tt[col][col] = ts[col];
+      __syncthreads();
+    }
+  }
+  {  // pairs (0, 1) and (2, 3): X = G12 T2, then T12 = -T1 X (16 x 16 each)
+    const int r = t >> 4, c = t & 15;
+#pragma unroll
+    for (int P = 0; P < 2; ++P) {
+      const int oa = 32 * P, ob = oa + 16;
+      double x = 0.0;
+      for (int k = 0; k <= c; ++k) x += g[oa + r][ob + k] * tt[ob + k][ob + c];
+      xx[oa + r][ob + c] = x;
+    }
     __syncthreads();
-    if (part == 0 && row < i) tt[row][i] = -ti * s;
-    if (t == 0) tt[i][i] = ti;
+#pragma unroll
+    for (int P = 0; P < 2; ++P) {
+      const int oa = 32 * P, ob = oa + 16;
+      double y = 0.0;
+      for (int k = r; k < 16; ++k) y += tt[oa + r][oa + k] * xx[oa + k][ob + c];
+      tt[oa + r][ob + c] = -y;
+    }
+    __syncthreads();
+  }
+  {  // halves: X = G[0:32, 32:64] T2, then T[0:32, 32:64] = -T1 X (32 x 32)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int e = t + kT * m, r = e >> 5, c = e & 31;
+      double x = 0.0;
+      for (int k = 0; k <= c; ++k) x += g[r][32 + k] * tt[32 + k][32 + c];
+      xx[r][32 + c] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int e = t + kT * m, r = e >> 5, c = e & 31;
+      double y = 0.0;
+      for (int k = r; k < 32; ++k) y += tt[r][k] * xx[k][32 + c];
+      tt[r][32 + c] = -y;
+    }
     __syncthreads();
   }
   for (int e = t; e < jb * jb; e += kT) {
