@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmDesc d) {
   __shared__ T As[2][BK][TBM + PAD];
   __shared__ T Bs[2][BK][TBN + PAD];
 
-  const TileCoord tc = tile_of<TBM, TBN>(d.M, d.N, d.lowerC, banded(d));
+  const TileCoord tc = tile_of<TBM, TBN>(d.M, d.N, d.lowerC, banded(d), heavy_right(d));
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(kB3Threads) void gemm_bf16x3_kernel(GemmDesc d, con
   __shared__ __attribute__((aligned(16))) unsigned short LA[2][2][kB3M][kB3K];  // [stage][hi lo][row][k, swizzled]
   __shared__ __attribute__((aligned(16))) unsigned short LB[2][2][kB3N][kB3K];
 
-  const TileCoord tc = tile_of<kB3M, kB3N>(d.M, d.N, d.lowerC, banded(d));
+  const TileCoord tc = tile_of<kB3M, kB3N>(d.M, d.N, d.lowerC, banded(d), heavy_right(d));
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;

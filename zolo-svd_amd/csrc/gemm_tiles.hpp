@@ -19,9 +19,12 @@ struct TileCoord { int64_t i0, j0; bool skip; };
 // per tile varies along the rows (triangular outputs, banded operands),
 // "balance" orders the full row groups 0, last, 1, last-1, ... (with groups
 // small enough that every XCD gets at least two) so that every XCD's eighth
-// pairs heavy and light groups.
+// pairs heavy and light groups.  When the work grows with the tile column
+// (rev_cols: op(B) zero below a diagonal, K = [0, j0 + BN + bHi)), each group
+// walks its columns last to first, so the heaviest tiles are dispatched first
+// and the lightest make up the tail.
 template <int BM, int BN>
-__device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC, bool balance) {
+__device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC, bool balance, bool rev_cols = false) {
   const int64_t tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   const int64_t nt = tm * tn;
   int64_t pid = blockIdx.x;
@@ -39,7 +42,7 @@ __device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC, bool balan
   const int64_t gsz = (tm - first) < G ? (tm - first) : G;
   TileCoord t;
   t.i0 = (first + r % gsz) * BM;
-  t.j0 = (r / gsz) * BN;
+  t.j0 = (rev_cols ? tn - 1 - r / gsz : r / gsz) * BN;
   t.skip = lowerC && (t.j0 > t.i0 + BM - 1);
   return t;
 }
@@ -47,6 +50,10 @@ __device__ inline TileCoord tile_of(int64_t M, int64_t N, int lowerC, bool balan
 // Work per tile depends on its position: triangular output or trimmed K range.
 __device__ inline bool banded(const GemmDesc& d) {
   return d.lowerC || d.aLo >= 0 || d.aHi >= 0 || d.bLo >= 0 || d.bHi >= 0;
+}
+// The K range grows with the tile column and nothing else trims it.
+__device__ inline bool heavy_right(const GemmDesc& d) {
+  return d.bHi >= 0 && d.bLo < 0 && d.aLo < 0 && d.aHi < 0 && !d.lowerC;
 }
 
 template <int BM, int BN>

@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kOzGemmThreads) void oz_gemm_kernel(GemmDesc d, con
   __shared__ __attribute__((aligned(16))) signed char LA[2][kOzM][kOzK];  // [stage][row][k, swizzled chunks]
   __shared__ __attribute__((aligned(16))) signed char LB[2][kOzN][kOzK];
 
-  const TileCoord tc = tile_of<kOzM, kOzN>(d.M, d.N, d.lowerC, banded(d));
+  const TileCoord tc = tile_of<kOzM, kOzN>(d.M, d.N, d.lowerC, banded(d), heavy_right(d));
   if (tc.skip) return;
   const int64_t i0 = tc.i0, j0 = tc.j0;
   const int64_t z = blockIdx.z;  // batch item * kOzMods + modulus
