@@ -170,6 +170,18 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
 // workgroups w, w + 4, ...) and reloads the empty ones until none is left,
 // then wave 0 adds the shares.  One store-to-load round trip per column.
 //
+// Shift per XCD (xmap; at most 8 shifts of at most 32 workgroups): workgroup
+// 8 w + qi is row block w of shift qi, so under the round-robin dispatch of
+// workgroups to the 8 XCDs all of a shift's run on one XCD; each checks that
+// from its XCC id at the start.  Then that XCD's L2 is the shift's coherence
+// point: the slot stores are workgroup-scope (sc0: into the L2, not written
+// through) and the device-scope polls (sc1) see them there, about 300 ns after
+// the store instead of 560-840 ns for a device-scope store (measured with
+// one workgroup storing a flag and another on the same XCD polling it;
+// workgroup-scope loads would be served by the poller's own L1 and never see
+// it).  Slot polling, as for TT = 256; 204-222 us per 64-column panel of up to
+// 8192 rows against 236-254 us with device-scope stores across XCDs.
+//
 // Flags (TT = 512): wave 0 stores the sums to the column's slot, waits until
 // those stores are complete (vmcnt(0)), raises the workgroup's flag to jj + 1
 // and polls the nwg flags of the shift together (lane k reads flag k) until
@@ -204,15 +216,24 @@ template <typename T, int TT>
 __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, int64_t strideS, int64_t n,
                                                                int64_t p, int jb, int64_t r1_end, int64_t r2_end,
                                                                double* part, unsigned* flags, int nwg, T* tau,
-                                                               int64_t strideTau, int q0) {
+                                                               int64_t strideTau, int q0, int xmap, int nq) {
   constexpr int kCoopWaves = TT / 64;
-  constexpr bool XM = TT <= 256;  // exchange by polling the slots (else by flags)
+  // exchange by polling the slots (else by flags): 256-thread launches and
+  // the shift-per-XCD launches
+  const bool XM = TT <= 256 || xmap;
   __shared__ double red[kCoopWaves][NBP], red2[kCoopWaves][NBP];
   __shared__ double wv[NBP], prj[NBP];
   __shared__ double sc[2];
-  const int qi = blockIdx.y, q = q0 + qi;
+  __shared__ int xlocal;
+  // xmap: one shift per XCD, workgroup 8 w + qi being row block w of shift
+  // qi (the round-robin dispatch puts every eighth workgroup on one XCD);
+  // else row block blockIdx.x of shift blockIdx.y
+  const int qi = xmap ? (int)(blockIdx.x & 7u) : (int)blockIdx.y;
+  const int w = xmap ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  if (qi >= nq) return;
+  const int q = q0 + qi;
   T* Sq = S + (int64_t)q * strideS;
-  const int t = threadIdx.x, w = blockIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int64_t l = (int64_t)w * TT + t, len1 = r1_end - p;
   const int64_t i = l < len1 ? p + l : n + (l - len1);
   const bool valid = l < len1 + (r2_end - n);
@@ -220,6 +241,33 @@ __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, in
   double* base = part + (int64_t)qi * (XM ? NBP : 2) * rec;
   const unsigned* flr = flags ? flags + (int64_t)qi * kCoopFlagStride : nullptr;
   unsigned* flw = flags ? flags + (int64_t)qi * kCoopFlagStride + w : nullptr;
+  // xmap with several workgroups: do all of this shift's run on one XCD?  Each
+  // publishes 1 + its XCC id (device scope, the words filled with ~0 before
+  // the launch) and reads the others'; all decide from the same values.
+  // Then that XCD's L2 is the shift's coherence point: slot stores go to it
+  // (workgroup scope, sc0) and the device-scope polls (sc1) see them there.
+  if (t == 0) {
+    int loc = 0;
+    if (xmap && nwg > 1) {
+      unsigned x;
+      asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+      const int mine = (int)(x & 0xFu) + 1;
+      int* xid = reinterpret_cast<int*>(part + (int64_t)nq * NBP * rec) + qi * kCoopMaxWg;
+      __hip_atomic_store(xid + w, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      loc = 1;
+      unsigned spin = 0;
+      for (int k = 0; k < nwg; ++k) {
+        int v;
+        while ((v = __hip_atomic_load(xid + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == -1 &&
+               ++spin < kSlotSpinCap)
+          __builtin_amdgcn_s_sleep(1);
+        if (v != mine) loc = 0;
+      }
+    }
+    xlocal = loc;
+  }
+  __syncthreads();
+  const bool l2local = xlocal != 0;
   // v[c]: the row's value in panel column jj + c; zero for jj + c >= jb
   double v[NBP];
 #pragma unroll
@@ -256,17 +304,25 @@ __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, in
       for (int k = 0; k < kCoopWaves; ++k) g += red[k][lane];
       pr = prj[lane];
     }
-    if constexpr (XM) {
+    if (XM) {
       if (nwg > 1) {
       // one record per column, filled with kSlotEmpty before the launch:
       // wave 0 stores the workgroup's sums, every wave polls its share of the
       // slots until none is empty
       double* G = base + (int64_t)jj * rec;
       if (wave == 0 && lane < rem) {
-        __hip_atomic_store(G + (int64_t)w * NBP + lane, slot_value(g), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (w == 0)
-          __hip_atomic_store(G + (int64_t)nwg * NBP + lane, slot_value(pr), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+        if (l2local) {
+          __hip_atomic_store(G + (int64_t)w * NBP + lane, slot_value(g), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (w == 0)
+            __hip_atomic_store(G + (int64_t)nwg * NBP + lane, slot_value(pr), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          __hip_atomic_store(G + (int64_t)w * NBP + lane, slot_value(g), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (w == 0)
+            __hip_atomic_store(G + (int64_t)nwg * NBP + lane, slot_value(pr), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
       constexpr int M = kCoopMaxWg / kCoopWaves;
       double sv[M];
@@ -653,8 +709,9 @@ int coop_launch(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb
       int nw = nwg;
       unsigned* noflags = nullptr;
       ZOLO_HIP(hipMemsetAsync(part, 0xFF, sizeof(double) * (size_t)(nq * NBP * rec), s));
+      int xmap = 0, nqa = nq;
       void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &noflags, &nw, &tau, &strideTau,
-                      (void*)&q0};
+                      (void*)&q0, &xmap, &nqa};
       ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T, TT>, dim3(nwg, nq), dim3(TT), args,
                                           0, s));
     }
@@ -670,12 +727,36 @@ int coop_launch(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb
       const int nq = std::min(per, batch - q0);
       int nw = nwg;
       ZOLO_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * kCoopFlagStride * nq, s));
+      int xmap = 0, nqa = nq;
       void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &flags, &nw, &tau, &strideTau,
-                      (void*)&q0};
+                      (void*)&q0, &xmap, &nqa};
       ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T, TT>, dim3(nwg, nq), dim3(TT), args,
                                           0, s));
     }
   }
+  return kOk;
+}
+
+// One shift per XCD (at most 8 shifts, at most one XCD's 32 CUs each): 8
+// workgroups launched per row block, every eighth on the same XCD under the
+// round-robin dispatch (the kernel checks), slot polling with L2-local stores.
+template <typename T, int TT>
+int coop_launch_xcd(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
+                    double* part, int64_t part_doubles, T* tau, int64_t strideTau, int batch, int nwg, hipStream_t s) {
+  const int64_t rec = (int64_t)nwg * NBP + NBP;
+  const int64_t slots = (int64_t)batch * NBP * rec;
+  const int64_t xid_doubles = (int64_t)batch * kCoopMaxWg / 2;
+  if (slots + xid_doubles > part_doubles) {
+    set_error("qr panel (cooperative): partials buffer too small");
+    return kErrArg;
+  }
+  // slot records and XCC words, all bits set (empty)
+  ZOLO_HIP(hipMemsetAsync(part, 0xFF, sizeof(double) * (size_t)(slots + xid_doubles), s));
+  unsigned* noflags = nullptr;
+  int nw = nwg, q0 = 0, xmap = 1, nq = batch;
+  void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &noflags, &nw, &tau, &strideTau,
+                  (void*)&q0, &xmap, &nq};
+  ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T, TT>, dim3(8 * nwg), dim3(TT), args, 0, s));
   return kOk;
 }
 
@@ -698,13 +779,13 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
     }
     int one = 1;
     unsigned* noflags = nullptr;
-    int q0 = 0;
+    int q0 = 0, xmap = 0;
     if (rows <= 256)
       hipLaunchKernelGGL((qr_panel_coop_kernel<T, 256>), dim3(1, batch), dim3(256), 0, s, S, lds, strideS, n, p, jb,
-                         r1_end, r2_end, part, noflags, one, tau, strideTau, q0);
+                         r1_end, r2_end, part, noflags, one, tau, strideTau, q0, xmap, batch);
     else
       hipLaunchKernelGGL((qr_panel_coop_kernel<T, 512>), dim3(1, batch), dim3(512), 0, s, S, lds, strideS, n, p, jb,
-                         r1_end, r2_end, part, noflags, one, tau, strideTau, q0);
+                         r1_end, r2_end, part, noflags, one, tau, strideTau, q0, xmap, batch);
     ZOLO_HIP(hipGetLastError());
     *done = true;
     return kOk;
@@ -719,7 +800,16 @@ int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
   ZOLO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int nwg256 = (int)((rows + 255) / 256), nwg512 = (int)((rows + 511) / 512);
   const int cap256 = coop_cap<T, 256>(dev), cap512 = coop_cap<T, 512>(dev);
-  if (nwg256 <= kCoopMaxWg && (int64_t)batch * nwg256 <= cus && cap256 >= nwg256) {
+  constexpr int kXcdCus = 32;  // CUs of one XCD: workgroups per shift in the shift-per-XCD form
+  if (batch <= 8 && nwg256 <= kXcdCus && cap256 >= 8 * nwg256) {
+    ZOLO_CHECK(coop_launch_xcd<T, 256>(S, lds, strideS, n, p, jb, r1_end, r2_end, part, part_doubles, tau, strideTau,
+                                       batch, nwg256, s));
+  } else if (batch >= 2 && batch <= 8 && nwg512 <= kXcdCus && cap512 >= 8 * nwg512) {
+    // (one shift over 8192 rows: 64 CUs with one wave per SIMD beat one XCD's
+    // 32 with two, 248-258 against 275-295 us per panel)
+    ZOLO_CHECK(coop_launch_xcd<T, 512>(S, lds, strideS, n, p, jb, r1_end, r2_end, part, part_doubles, tau, strideTau,
+                                       batch, nwg512, s));
+  } else if (nwg256 <= kCoopMaxWg && (int64_t)batch * nwg256 <= cus && cap256 >= nwg256) {
     ZOLO_CHECK(coop_launch<T, 256>(S, lds, strideS, n, p, jb, r1_end, r2_end, part, part_doubles, tau, strideTau,
                                    batch, nwg256, cap256, s));
   } else if (nwg512 <= kCoopMaxWg && cap512 >= nwg512) {
