@@ -218,9 +218,8 @@ __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, in
                                                                double* part, unsigned* flags, int nwg, T* tau,
                                                                int64_t strideTau, int q0, int xmap, int nq) {
   constexpr int kCoopWaves = TT / 64;
-  // exchange by polling the slots (else by flags): 256-thread launches and
-  // the shift-per-XCD launches
-  const bool XM = TT <= 256 || xmap;
+  // exchange by polling the slots (256 threads) or by flags (512 threads)
+  constexpr bool XM = TT <= 256;
   __shared__ double red[kCoopWaves][NBP], red2[kCoopWaves][NBP];
   __shared__ double wv[NBP], prj[NBP];
   __shared__ double sc[2];
@@ -252,7 +251,7 @@ __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, in
       unsigned x;
       asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
       const int mine = (int)(x & 0xFu) + 1;
-      int* xid = reinterpret_cast<int*>(part + (int64_t)nq * NBP * rec) + qi * kCoopMaxWg;
+      int* xid = reinterpret_cast<int*>(part + (int64_t)nq * (XM ? NBP : 2) * rec) + qi * kCoopMaxWg;
       __hip_atomic_store(xid + w, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       loc = 1;
       unsigned spin = 0;
@@ -304,7 +303,7 @@ __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, in
       for (int k = 0; k < kCoopWaves; ++k) g += red[k][lane];
       pr = prj[lane];
     }
-    if (XM) {
+    if constexpr (XM) {
       if (nwg > 1) {
       // one record per column, filled with kSlotEmpty before the launch:
       // wave 0 stores the workgroup's sums, every wave polls its share of the
@@ -367,14 +366,25 @@ __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, in
       const unsigned target = (unsigned)jj + 1u;
       if (wave == 0) {
         if (lane < rem) {
-          __hip_atomic_store(G + (int64_t)w * NBP + lane, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (w == 0) __hip_atomic_store(G + (int64_t)nwg * NBP + lane, pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (l2local) {
+            __hip_atomic_store(G + (int64_t)w * NBP + lane, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (w == 0)
+              __hip_atomic_store(G + (int64_t)nwg * NBP + lane, pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else {
+            __hip_atomic_store(G + (int64_t)w * NBP + lane, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (w == 0)
+              __hip_atomic_store(G + (int64_t)nwg * NBP + lane, pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
         }
-        // the slot stores are complete (at the device coherence point) before the flag goes out
+        // the slot stores are complete (at the coherence point: the device's,
+        // or this XCD's L2 for the shift-per-XCD form) before the flag goes out
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(kWaitVm0);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 0) __hip_atomic_store(flw, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) {
+          if (l2local) __hip_atomic_store(flw, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          else __hip_atomic_store(flw, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         for (unsigned spin = 0; spin < kCoopSpinCap; ++spin) {
           const unsigned f =
               lane < nwg ? __hip_atomic_load(flr + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : target;
@@ -744,17 +754,29 @@ template <typename T, int TT>
 int coop_launch_xcd(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
                     double* part, int64_t part_doubles, T* tau, int64_t strideTau, int batch, int nwg, hipStream_t s) {
   const int64_t rec = (int64_t)nwg * NBP + NBP;
-  const int64_t slots = (int64_t)batch * NBP * rec;
   const int64_t xid_doubles = (int64_t)batch * kCoopMaxWg / 2;
-  if (slots + xid_doubles > part_doubles) {
-    set_error("qr panel (cooperative): partials buffer too small");
-    return kErrArg;
+  unsigned* flags = nullptr;
+  if constexpr (TT <= 256) {
+    // slot records of every column, then the XCC words: all bits set (empty)
+    const int64_t slots = (int64_t)batch * NBP * rec;
+    if (slots + xid_doubles > part_doubles) {
+      set_error("qr panel (cooperative): partials buffer too small");
+      return kErrArg;
+    }
+    ZOLO_HIP(hipMemsetAsync(part, 0xFF, sizeof(double) * (size_t)(slots + xid_doubles), s));
+  } else {
+    // two alternating slot records, the XCC words (all bits set), the flags (zero)
+    const int64_t slots = (int64_t)batch * 2 * rec, flag_doubles = (int64_t)batch * kCoopFlagStride / 2;
+    if (slots + xid_doubles + flag_doubles > part_doubles) {
+      set_error("qr panel (cooperative): partials buffer too small");
+      return kErrArg;
+    }
+    ZOLO_HIP(hipMemsetAsync(part + slots, 0xFF, sizeof(double) * (size_t)xid_doubles, s));
+    flags = reinterpret_cast<unsigned*>(part + slots + xid_doubles);
+    ZOLO_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * (size_t)batch * kCoopFlagStride, s));
   }
-  // slot records and XCC words, all bits set (empty)
-  ZOLO_HIP(hipMemsetAsync(part, 0xFF, sizeof(double) * (size_t)(slots + xid_doubles), s));
-  unsigned* noflags = nullptr;
   int nw = nwg, q0 = 0, xmap = 1, nq = batch;
-  void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &noflags, &nw, &tau, &strideTau,
+  void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &flags, &nw, &tau, &strideTau,
                   (void*)&q0, &xmap, &nq};
   ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T, TT>, dim3(8 * nwg), dim3(TT), args, 0, s));
   return kOk;

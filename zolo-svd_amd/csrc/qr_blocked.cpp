@@ -168,8 +168,8 @@ int geqrf_impl(const Ctx& c, T* S, int64_t lds, int64_t sS, int64_t n, int batch
   QrScratch q;
   ZOLO_CHECK(take_scratch<T>(ar, n, batch, q));
   const int nwg_max = qr_nwg(n + nb);
-  // slot records of every panel column (per shift), then 64 words per shift
-  const int64_t part_doubles = (int64_t)batch * nb * ((int64_t)nwg_max * nb + nb) + (int64_t)batch * 32;
+  // slot records of every panel column (per shift), then 128 words per shift
+  const int64_t part_doubles = (int64_t)batch * nb * ((int64_t)nwg_max * nb + nb) + (int64_t)batch * 64;
   double* part = ar.take<double>((size_t)part_doubles);
   ZOLO_ARENA_OK(ar);
   T* V = (T*)q.V;
