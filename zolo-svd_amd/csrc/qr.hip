@@ -179,8 +179,10 @@ __global__ __launch_bounds__(kT) void qr_panel_kernel(T* S, int64_t lds, int64_t
 // the store instead of 560-840 ns for a device-scope store (measured with
 // one workgroup storing a flag and another on the same XCD polling it;
 // workgroup-scope loads would be served by the poller's own L1 and never see
-// it).  Slot polling, as for TT = 256; 204-222 us per 64-column panel of up to
-// 8192 rows against 236-254 us with device-scope stores across XCDs.
+// it).  TT = 256 polls the slots, TT = 512 exchanges by flags (both with
+// workgroup-scope stores); 204-222 us per 64-column panel of up to 8192 rows
+// against 236-254 us with device-scope stores across XCDs, and 243-266 us
+// for 8-shift panels over 8192 rows against about 320 us.
 //
 // Flags (TT = 512): wave 0 stores the sums to the column's slot, waits until
 // those stores are complete (vmcnt(0)), raises the workgroup's flag to jj + 1
