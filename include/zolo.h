@@ -339,6 +339,16 @@ int zolo_l0_train_device(float* params, const float* feat, const float* target, 
 int zolo_set_f64_products(int mode);
 int zolo_get_f64_products(void);
 
+/* ---- QR panel exchange (no counterpart in the reference; a test and tuning hook) ----
+ * The panels of up to 8 shifts of at most 32 workgroups each run one shift per
+ * XCD and, when every workgroup of a shift is found on one XCD, exchange their
+ * column sums through that XCD's L2.  Mode 1 makes them use device-scope
+ * stores instead (the form they fall back to when the placement check fails);
+ * mode 0 (the default) is the automatic choice.  The results are the same
+ * bits either way.  Process-wide; -1 for a mode other than 0 or 1. */
+int zolo_set_panel_exchange(int mode);
+int zolo_get_panel_exchange(void);
+
 /* ---- measurement: GEMM launch ledger (no counterpart in the reference) ----
  * While enabled, every MFMA GEMM launch is bracketed by hipEvents on its stream
  * and its algorithmic flop count is recorded; zolo_profile_gemm sums launches

@@ -463,6 +463,16 @@ int zolo_set_f64_products(int mode) try {
 
 int zolo_get_f64_products(void) { return f64_products(); }
 
+int zolo_set_panel_exchange(int mode) try {
+  if (mode < 0 || mode > 1) return -1;
+  qr_set_panel_device_stores(mode);
+  return kOk;
+} catch (...) {
+  return exception_status();
+}
+
+int zolo_get_panel_exchange(void) { return qr_panel_device_stores(); }
+
 int zolo_profile_enable(int on) try {
   ledger_enable(on != 0);
   return kOk;

@@ -65,6 +65,10 @@ int k_qr_panel_wg(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int 
 // barrier per column); *done = false when the grid would not stay resident,
 // and the caller then takes k_qr_panel_step.  part: >= 2 * batch * (nwg * 64 + 64)
 // doubles with nwg = ceil(rows / 512).
+// Shift-per-XCD QR panels with device-scope exchange stores even where the
+// L2-local ones apply (1), or the automatic choice (0).
+void qr_set_panel_device_stores(int on);
+int qr_panel_device_stores();
 template <typename T>
 int k_qr_panel_coop(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb, int64_t r1_end, int64_t r2_end,
                     double* part, int64_t part_doubles, T* tau, int64_t strideTau, int batch, hipStream_t s,

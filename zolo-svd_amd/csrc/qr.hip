@@ -249,7 +249,7 @@ __global__ __launch_bounds__(TT) void qr_panel_coop_kernel(T* S, int64_t lds, in
   // (workgroup scope, sc0) and the device-scope polls (sc1) see them there.
   if (t == 0) {
     int loc = 0;
-    if (xmap && nwg > 1) {
+    if (xmap == 1 && nwg > 1) {
       unsigned x;
       asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
       const int mine = (int)(x & 0xFu) + 1;
@@ -749,6 +749,10 @@ int coop_launch(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, int jb
   return kOk;
 }
 
+// Process-wide: the shift-per-XCD panels take device-scope stores even when
+// their workgroups share one XCD (set by qr_set_panel_device_stores).
+int g_panel_device_stores = 0;
+
 // One shift per XCD (at most 8 shifts, at most one XCD's 32 CUs each): 8
 // workgroups launched per row block, every eighth on the same XCD under the
 // round-robin dispatch (the kernel checks), slot polling with L2-local stores.
@@ -777,7 +781,9 @@ int coop_launch_xcd(T* S, int64_t lds, int64_t strideS, int64_t n, int64_t p, in
     flags = reinterpret_cast<unsigned*>(part + slots + xid_doubles);
     ZOLO_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * (size_t)batch * kCoopFlagStride, s));
   }
-  int nw = nwg, q0 = 0, xmap = 1, nq = batch;
+  // xmap 1: L2-local stores when the placement check passes; 2: device-scope
+  // stores regardless (the form the check falls back to)
+  int nw = nwg, q0 = 0, xmap = g_panel_device_stores ? 2 : 1, nq = batch;
   void* args[] = {&S, &lds, &strideS, &n, &p, &jb, &r1_end, &r2_end, &part, &flags, &nw, &tau, &strideTau,
                   (void*)&q0, &xmap, &nq};
   ZOLO_HIP(hipLaunchCooperativeKernel((const void*)qr_panel_coop_kernel<T, TT>, dim3(8 * nwg), dim3(TT), args, 0, s));
@@ -892,6 +898,9 @@ int k_set_identity_top(T* Q, int64_t ldq, int64_t strideQ, int64_t m, int64_t n,
   ZOLO_HIP(hipGetLastError());
   return kOk;
 }
+
+void qr_set_panel_device_stores(int on) { g_panel_device_stores = on ? 1 : 0; }
+int qr_panel_device_stores() { return g_panel_device_stores; }
 
 #define ZOLO_QR_INST(T)                                                                                            \
   template int k_qr_panel_step<T>(T*, int64_t, int64_t, int64_t, int64_t, int, int64_t, int64_t, int64_t, int, int, \

@@ -171,6 +171,8 @@ _sig("zolo_formx2_host", C.c_int, [_I64, C.c_double, _D, _I64, _D, _I64, _D])
 _sig("zolo_profile_enable", C.c_int, [C.c_int])
 _sig("zolo_set_f64_products", C.c_int, [C.c_int])
 _sig("zolo_get_f64_products", C.c_int, [])
+_sig("zolo_set_panel_exchange", C.c_int, [C.c_int])
+_sig("zolo_get_panel_exchange", C.c_int, [])
 _sig("zolo_profile_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64), _D, _D])
 
 
@@ -181,6 +183,16 @@ def set_f64_products(mode: int):
 
 def get_f64_products() -> int:
     return int(_lib.zolo_get_f64_products())
+
+
+def set_panel_exchange(mode: int):
+    """QR panels of one shift per XCD: 0 = automatic (L2-local stores when the
+    placement check passes), 1 = device-scope stores (the fallback form)."""
+    _check("zolo_set_panel_exchange", _lib.zolo_set_panel_exchange(int(mode)))
+
+
+def get_panel_exchange() -> int:
+    return int(_lib.zolo_get_panel_exchange())
 
 
 def profile_enable(on: bool = True):
