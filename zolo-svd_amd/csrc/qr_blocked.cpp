@@ -387,7 +387,7 @@ size_t stacked_qr_bytes(int64_t n, int r) {
   b += 2 * sizeof(T) * (size_t)r * nbo * nbo;                          // G, combine scratch
   b += sizeof(T) * (size_t)r * nout * nbo * nbo;                       // T of every outer block
   b += 2 * sizeof(T) * (size_t)r * nbo * n;                            // W, W2
-  b += sizeof(double) * (size_t)r * kQrNb * ((size_t)nwg * kQrNb + kQrNb);  // panel partials
+  b += sizeof(double) * (size_t)r * (kQrNb * ((size_t)nwg * kQrNb + kQrNb) + 64);  // panel partials, XCC words, flags
   b += sizeof(T) * (size_t)r * n;                                      // tau
   return b + 16 * 256;
 }
