@@ -86,6 +86,42 @@ __device__ double tridiag_eig(const double* a, const double* b, int k, int e) {
   return 0.5 * (lo + hi);
 }
 
+// Sum over j < n of f(A[i + j ld], j) for one row i, eight loads in flight
+// (the loads of a group are issued before any is used) and eight partial
+// sums, added in a fixed order.
+template <typename F>
+__device__ inline double row_sum8(const float* A, int64_t ld, int i, int n, F&& f) {
+  double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const float* a = A + i;
+  int j = 0;
+  for (; j + 8 <= n; j += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = a[(int64_t)(j + u) * ld];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] += f((double)v[u], j + u);
+  }
+  for (; j < n; ++j) s[j & 7] += f((double)a[(int64_t)j * ld], j);
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+// Sum over i < n (1 <= n <= kMaxBatchN) of f(A[i + j ld], i) for one
+// column j, lanes over rows: the kMaxBatchN / 64 loads of a lane all in
+// flight (rows past n read row n - 1 and are left out of the sum).
+template <typename F>
+__device__ inline double col_sum_wave(const float* A, int64_t ld, int j, int n, int lane, F&& f) {
+  constexpr int R = kMaxBatchN / 64;
+  const float* a = A + (int64_t)j * ld;
+  float v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = a[min(lane + 64 * r, n - 1)];
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (lane + 64 * r < n) s += f((double)v[r], lane + 64 * r);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
 __global__ __launch_bounds__(kT) void features_kernel(const BatchMat* mats, const double* alpha, float* feat,
                                                       float* smax) {
   __shared__ double Q[kLanczos + 1][kMaxBatchN];
@@ -104,21 +140,14 @@ __global__ __launch_bounds__(kT) void features_kernel(const BatchMat* mats, cons
   const int64_t ld = m.lda;
   auto x = [&](int i, int j) { return ia * (double)A[i + (int64_t)j * ld]; };
 
-  // norms: row sums (thread per row), column sums (thread per column)
+  // norms: row sums (thread per row), column sums (one wave per column,
+  // lanes over rows: coalesced)
   double rmax = 0.0, cmax = 0.0;
-  for (int i = t; i < n; i += kT) {
-    double s = 0.0;
-    for (int j = 0; j < n; ++j) s += fabs(x(i, j));
-    rmax = fmax(rmax, s);
-  }
-  // column sums: one wave per column, lanes over rows (coalesced)
+  for (int i = t; i < n; i += kT)
+    rmax = fmax(rmax, row_sum8(A, ld, i, n, [&](double a, int) { return fabs(ia * a); }));
   const int lane = t & 63, wave = t >> 6;
-  for (int j = wave; j < n; j += kT / 64) {
-    double s = 0.0;
-    for (int i = lane; i < n; i += 64) s += fabs(x(i, j));
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    cmax = fmax(cmax, s);
-  }
+  for (int j = wave; j < n; j += kT / 64)
+    cmax = fmax(cmax, col_sum_wave(A, ld, j, n, lane, [&](double a, int) { return fabs(ia * a); }));
   const double ninf = block_max(rmax, sh);
   const double n1 = block_max(cmax, sh);
 
@@ -130,7 +159,20 @@ __global__ __launch_bounds__(kT) void features_kernel(const BatchMat* mats, cons
   __syncthreads();
   for (int i = t; i < n; i += kT) {
     double acc[kSketch] = {0.0, 0.0, 0.0, 0.0};
-    for (int j = 0; j < n; ++j) {
+    const float* a = A + i;
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {  // eight loads in flight
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = a[(int64_t)(j + u) * ld];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double xv = ia * (double)v[u];
+#pragma unroll
+        for (int k = 0; k < kSketch; ++k) acc[k] += xv * Om[k][j + u];
+      }
+    }
+    for (; j < n; ++j) {
       const double xv = x(i, j);
 #pragma unroll
       for (int k = 0; k < kSketch; ++k) acc[k] += xv * Om[k][j];
@@ -162,16 +204,11 @@ __global__ __launch_bounds__(kT) void features_kernel(const BatchMat* mats, cons
   __syncthreads();
   int k = 0;
   for (; k < kLanczos && k < n; ++k) {
-    for (int i = t; i < n; i += kT) {  // u = X q_k
-      double s = 0.0;
-      for (int j = 0; j < n; ++j) s += x(i, j) * Q[k][j];
-      u[i] = s;
-    }
+    for (int i = t; i < n; i += kT)  // u = X q_k
+      u[i] = row_sum8(A, ld, i, n, [&](double a, int j) { return ia * a * Q[k][j]; });
     __syncthreads();
     for (int j = wave; j < n; j += kT / 64) {  // w = X^T u, one wave per column
-      double s = 0.0;
-      for (int i = lane; i < n; i += 64) s += x(i, j) * u[i];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      const double s = col_sum_wave(A, ld, j, n, lane, [&](double a, int i) { return ia * a * u[i]; });
       if (lane == 0) w[j] = s;
     }
     __syncthreads();
