@@ -193,22 +193,29 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
 #pragma unroll
       for (int r = 0; r < R; ++r) vu[r] = vs[cur][wv + 4 * r];
       // ---- partial sums of D v, E_k v (rows: lane; this wave's columns) and
-      // v^T E_{k-1} (columns: lane; this wave's quarter of the rows)
+      // v^T E_{k-1} (columns: lane; this wave's quarter of the rows).  Every
+      // LDS read is issued before the first product, so the reads overlap
+      // instead of each waiting for the one before.  Entries outside the
+      // task (rows or columns >= L, E_{k-1} at k = 0) hold stale values; they
+      // are dropped by selects, never multiplied by zero.
       {
+        double dv[R], ev[R], vt[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          dv[r] = Dm[(wv + 4 * r) * kLd + lane];
+          ev[r] = Ep[lane * kLd + wv * R + r];
+          vt[r] = vs[cur][wv * R + r];
+        }
         double p = 0.0, rr = 0.0, ee = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int u = wv + 4 * r;
-          if (u < L) {
-            if (lane < L) p += Dm[u * kLd + lane] * vu[r];
-            rr += ereg[r] * vu[r];
-          }
-          const int t = wv * R + r;
-          if (k > 0) ee += vs[cur][t] * Ep[lane * kLd + t];
+          p += (wv + 4 * r < L && lane < L) ? dv[r] * vu[r] : 0.0;
+          rr += ereg[r] * vu[r];  // ereg and vu are zero outside the task
+          ee += vt[r] * ev[r];
         }
         part[0][wv][lane] = p;
         part[1][wv][lane] = rr;
-        part[2][wv][lane] = ee;
+        part[2][wv][lane] = k > 0 ? ee : 0.0;
       }
       __syncthreads();  // (3)
       if (wv == 0) {
@@ -225,11 +232,31 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
       __syncthreads();  // (4)
       // ---- D <- H D H (own lower entries, stored), E_{k-1} <- H E_{k-1}
       // (column 0 is beta e1) stored, E_k <- E_k H kept in LDS for task k+1;
-      // wave 0 holds column 0 of E_k and takes the next reflector from it
+      // wave 0 holds column 0 of E_k and takes the next reflector from it.
+      // The LDS operands are read first, and the band stores of D and E_{k-1}
+      // are issued before wave 0 forms the reflector, so they are in flight
+      // while it computes.
       const double wl = ws[lane], ql = qs[lane];
+      double wu[R], eu[R], pu[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int u = wv + 4 * r;
+        wu[r] = ws[u];
+        eu[r] = es[u];
+        pu[r] = Ep[u * kLd + lane];
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (wv + 4 * r < L && lane < L2) ereg[r] -= ql * vu[r];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int u = wv + 4 * r;
+        if (u < L && lane >= u && lane < L) band_st(LB + (s + u) * ldb + (lane - u), dreg[r] - (vl * wu[r] + wl * vu[r]));
+        if (k > 0 && lane < L) {
+          const double y = u == 0 ? (lane == 0 ? beta : 0.0) : pu[r] - vl * eu[r];
+          band_st(LB + (s - B + u) * ldb + (B + lane - u), y);
+        }
+      }
       if (wv == 0 && L2 > 0) {
         const int nx = cur ^ 1;
         const double xx = lane < L2 ? ereg[0] : 0.0;
@@ -253,15 +280,7 @@ __global__ __launch_bounds__(kT) void sb2st_kernel(SbArgs a) {
         }
       }
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int u = wv + 4 * r;
-        if (u < L && lane >= u && lane < L) band_st(LB + (s + u) * ldb + (lane - u), dreg[r] - (vl * ws[u] + wl * vu[r]));
-        if (k > 0 && lane < L) {
-          const double y = u == 0 ? (lane == 0 ? beta : 0.0) : Ep[u * kLd + lane] - vl * es[u];
-          band_st(LB + (s - B + u) * ldb + (B + lane - u), y);
-        }
-        En[u * kLd + lane] = ereg[r];
-      }
+      for (int r = 0; r < R; ++r) En[(wv + 4 * r) * kLd + lane] = ereg[r];
       // this thread's band stores are complete before the barrier, so all of
       // the task's are before the progress store
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
