@@ -30,7 +30,8 @@ def _sym(n, kind, seed):
 
 
 @pytest.mark.parametrize("n,kind", [(1, "gauss"), (2, "gauss"), (3, "c4"), (17, "gauss"), (32, "c4"), (33, "repeated"),
-                                    (64, "graded"), (100, "gauss"), (257, "c4"), (300, "repeated"), (600, "c4")])
+                                    (64, "graded"), (100, "gauss"), (257, "c4"), (300, "repeated"), (600, "c4"),
+                                    (1537, "graded"), (2000, "gauss")])
 def test_syevd_matches_lapack(zl, torch_gpu, n, kind):
     H = _sym(n, kind, 3 + n)
     dH = zl.to_device(H)
