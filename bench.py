@@ -29,6 +29,7 @@ import argparse
 import json
 import math
 import os
+import statistics
 import sys
 import time
 
@@ -383,6 +384,8 @@ def main():
                     help="order of a whole CPU polar decomposition timed end to end to check the CPU step-sum "
                          "model (0: skip; 4096 takes about 80 s on 16 threads)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 / FP32 comparison runs")
+    ap.add_argument("--compare-runs", type=int, default=5,
+                    help="timed runs of the FP64 comparison; its median is the speed-up's denominator")
     ap.add_argument("--small-count", type=int, default=4096, help="matrices of the batched small-PD workload")
     ap.add_argument("--no-small", action="store_true", help="skip the batched small-PD workload")
     ap.add_argument("--svd", type=int, default=8192, help="also time the Zolo-SVD tail at this n (0: skip)")
@@ -458,14 +461,19 @@ def main():
         for prec in ("f64",) + extra:
             if prec == args.precision:
                 continue
-            torch.cuda.synchronize()
-            c0 = time.perf_counter()
-            ci = run_pd(zl, dA, prec, dU, dH, args.bounds)
-            torch.cuda.synchronize()
-            c1 = time.perf_counter()
-            compare[prec] = {"s": c1 - c0, "schedule": ci.schedule, "errors": errors(torch, A, dU, dH, Ug, Vg),
+            runs = []
+            for _ in range(max(1, args.compare_runs) if prec == "f64" else 1):
+                torch.cuda.synchronize()
+                c0 = time.perf_counter()
+                ci = run_pd(zl, dA, prec, dU, dH, args.bounds)
+                torch.cuda.synchronize()
+                c1 = time.perf_counter()
+                runs.append(c1 - c0)
+            med = statistics.median(runs)
+            compare[prec] = {"s": med, "runs_s": [round(x, 4) for x in runs], "schedule": ci.schedule,
+                             "errors": errors(torch, A, dU, dH, Ug, Vg),
                              "phase_ms": {k: round(v, 1) for k, v in ci.phase_ms.items() if v > 0}}
-            log(f"{prec}: {c1 - c0:.3f} s schedule {ci.schedule} errors {compare[prec]['errors']}")
+            log(f"{prec}: {med:.3f} s (median of {len(runs)}) schedule {ci.schedule} errors {compare[prec]['errors']}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -498,6 +506,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": sec * 1e3,
+        "step_s": [round(x.time_ms / 1e3, 4) for x in infos],
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
