@@ -160,3 +160,26 @@ def test_gemm_f64x_graded(torch_gpu, zl, ta, tb):
     assert np.all(got[7] == 0.0) and np.all(got[:, 11] == 0.0)
     mask = scale > 0
     assert float(np.max(np.abs(got - ref)[mask] / scale[mask])) < 1e-14
+
+
+@pytest.mark.parametrize("ta", [1, 0])
+def test_gemm_f64x_one_operand(torch_gpu, zl, ta):
+    """int8-modular A^T A and A A^T with one matrix passed as both operands (the residues of
+    op(A) are reused for op(B)); the vectors they pair are graded over 2^+-60. Error relative
+    to |op(A)||op(B)| at FP64 level, and the result is symmetric."""
+    rng = np.random.default_rng(70 + ta)
+    m, k = 333, 1201
+    X = rng.standard_normal((k, m) if ta else (m, k))
+    if ta:
+        X *= np.exp2(rng.integers(-60, 60, size=(1, m)))
+    else:
+        X *= np.exp2(rng.integers(-60, 60, size=(m, 1)))
+    opA, opB = (X.T, X) if ta else (X, X.T)
+    dX, dC = zl.to_device(X), zl.to_device(np.zeros((m, m)))
+    zl.gemm_device(ta, 1 - ta, m, m, k, 1.25, dX, X.shape[0], dX, X.shape[0], 0.0, dC, m, precision="f64x")
+    torch_gpu.cuda.synchronize()
+    got = zl.from_device(dC)
+    ref = 1.25 * opA @ opB
+    scale = 1.25 * np.abs(opA) @ np.abs(opB)
+    assert float(np.max(np.abs(got - ref) / scale)) < 1e-14
+    assert np.array_equal(got, got.T)

@@ -430,10 +430,14 @@ int gemm_ozaki(const GemmDesc& d, hipStream_t s, bool capture) {
     g_oz_cap = bytes;
   }
   signed char* RA = reinterpret_cast<signed char*>(g_oz_ws);
-  signed char* RB = RA + bytesA;
-  unsigned char* RC = reinterpret_cast<unsigned char*>(RB + bytesB);
+  unsigned char* RC = reinterpret_cast<unsigned char*>(RA + bytesA + bytesB);
   int* eA = reinterpret_cast<int*>(reinterpret_cast<char*>(g_oz_ws) + offE);
-  int* eB = eA + Mp;
+  // op(A) = op(B)^T of one matrix (A^T A or A A^T, e.g. the polish's U^T U):
+  // the rows of op(A) and the columns of op(B) are the same vectors with the
+  // same scaling, so the residues of op(A) serve as those of op(B)
+  const bool same = d.A == d.B && d.lda == d.ldb && d.strideA == d.strideB && d.M == d.N && d.transA != d.transB;
+  signed char* RB = same ? RA : RA + bytesA;
+  int* eB = same ? eA : eA + Mp;
   const dim3 ga((unsigned)(Kp / kOzTile), (unsigned)(Mp / kOzTile), 1);
   const dim3 gb((unsigned)(Kp / kOzTile), (unsigned)(Np / kOzTile), 1);
   const int64_t tiles = cdiv(d.M, kOzM) * cdiv(d.N, kOzN);
@@ -446,7 +450,7 @@ int gemm_ozaki(const GemmDesc& d, hipStream_t s, bool capture) {
     db.C = reinterpret_cast<double*>(d.C) + (int64_t)b * d.strideC;
     db.alphaVec = d.alphaVec ? d.alphaVec + b : nullptr;
     // an operand shared by the batch (stride 0) is converted once
-    const bool convA = b == 0 || d.strideA != 0, convB = b == 0 || d.strideB != 0;
+    const bool convA = b == 0 || d.strideA != 0, convB = !same && (b == 0 || d.strideB != 0);
     if (convA) {
       hipLaunchKernelGGL(oz_fill_kernel, dim3((unsigned)cdiv(Mp, kOzThreads)), dim3(kOzThreads), 0, s, eA, Mp, INT_MIN);
       // op(A) row i, depth k: A[k + i lda] when transposed (K-major), else A[i + k lda]
