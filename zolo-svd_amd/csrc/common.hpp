@@ -69,6 +69,10 @@ struct GemmDesc {
   const int* mapC = nullptr;
   // tile hint: 0 auto, 1 128x128, 2 64x128, 3 128x64, 4 64x64 (F64 / F32)
   int tile = 0;
+  // F64X: the caller guarantees that op(A) holds the same values as in the
+  // previous F64X product (same pointer and layout, not written since), so
+  // its residues may be reused when that product left them in the workspace
+  int sameA = 0;
   // Two row ranges (F64 / F32 engine): rows k >= bGapAt of B (transB = 0)
   // are read from row k + bGap, and rows i >= cGapAt of C are read and
   // written at row i + cGap, so a product over [rows r0, r1) of one block and

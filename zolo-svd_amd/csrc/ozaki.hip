@@ -402,6 +402,18 @@ bool gemm_ozaki_supported(const GemmDesc& d) {
   return !d.mapA && !d.mapB && !d.mapC && d.splitk <= 1 && d.K >= 1 && d.K <= 131072;
 }
 
+// The op(A) whose residues and exponents the workspace holds (set after a
+// conversion of a single-item product, cleared by any other conversion of A
+// or a new workspace).
+struct OzLastA {
+  bool valid = false;
+  const void* A = nullptr;
+  void* ws = nullptr;
+  int64_t lda = 0, M = 0, N = 0, K = 0;
+  int transA = 0, bits = 0;
+};
+OzLastA g_oz_last;
+
 int gemm_ozaki(const GemmDesc& d, hipStream_t s, bool capture) {
   if (!gemm_ozaki_supported(d)) {
     set_error("gemm (int8 moduli): unsupported descriptor");
@@ -426,6 +438,7 @@ int gemm_ozaki(const GemmDesc& d, hipStream_t s, bool capture) {
     }
     g_oz_ws = nullptr;
     g_oz_cap = 0;
+    g_oz_last.valid = false;  // a new allocation may reuse the old address
     ZOLO_HIP(hipMalloc(&g_oz_ws, bytes));
     g_oz_cap = bytes;
   }
@@ -450,7 +463,23 @@ int gemm_ozaki(const GemmDesc& d, hipStream_t s, bool capture) {
     db.C = reinterpret_cast<double*>(d.C) + (int64_t)b * d.strideC;
     db.alphaVec = d.alphaVec ? d.alphaVec + b : nullptr;
     // an operand shared by the batch (stride 0) is converted once
-    const bool convA = b == 0 || d.strideA != 0, convB = !same && (b == 0 || d.strideB != 0);
+    // (the layout of the workspace depends on M, N and K only, so a match of
+    // those, the operand and the bits finds the residues where they were left)
+    const bool keepA = d.sameA && d.batch == 1 && g_oz_last.valid && g_oz_last.A == d.A && g_oz_last.ws == g_oz_ws &&
+                       g_oz_last.lda == d.lda && g_oz_last.M == d.M && g_oz_last.N == d.N && g_oz_last.K == d.K &&
+                       g_oz_last.transA == d.transA && g_oz_last.bits == bits;
+    const bool convA = !keepA && (b == 0 || d.strideA != 0), convB = !same && (b == 0 || d.strideB != 0);
+    if (convA) {
+      g_oz_last.valid = d.batch == 1;
+      g_oz_last.A = d.A;
+      g_oz_last.ws = g_oz_ws;
+      g_oz_last.lda = d.lda;
+      g_oz_last.M = d.M;
+      g_oz_last.N = d.N;
+      g_oz_last.K = d.K;
+      g_oz_last.transA = d.transA;
+      g_oz_last.bits = bits;
+    }
     if (convA) {
       hipLaunchKernelGGL(oz_fill_kernel, dim3((unsigned)cdiv(Mp, kOzThreads)), dim3(kOzThreads), 0, s, eA, Mp, INT_MIN);
       // op(A) row i, depth k: A[k + i lda] when transposed (K-major), else A[i + k lda]

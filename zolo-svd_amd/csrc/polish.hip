@@ -286,6 +286,7 @@ int polish(const Ctx& c, double* U, int64_t ldu, const double* Xref, int64_t ldx
     d = GemmDesc();
     d.transA = 1; d.M = n; d.N = n; d.K = n;
     d.A = U; d.lda = ldu; d.B = Xref; d.ldb = ldx; d.C = C; d.ldc = n;
+    d.sameA = 1;  // U is unchanged since G = U^T U: its int8 residues are reused
     ZOLO_CHECK(gemm(c.big, d, s));
     float* none = nullptr;
     hipLaunchKernelGGL(pair_tile_kernel<0>, tiles, dim3(kT), 0, s, (const double*)G, n, F, none);
