@@ -274,6 +274,30 @@ def small_pds(torch, zl, count, seed):
     return res
 
 
+def normest_rate(torch, zl, n, seed):
+    """PDNORMEST (pdnormest.f90:2-80) on an n x n FP64 matrix: the power
+    iteration's GEMVs stream A twice per iteration (A x, then A^T y) after one
+    column-sum pass, so its rate is HBM bandwidth.  hipEvents on the stream the
+    library runs on; tol = 1e-12 keeps it to its 100-iteration cap."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    zl.pdnormest_device(A, tol=1e-12)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        est, cnt = zl.pdnormest_device(A, tol=1e-12)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = (2 * cnt + 1) * n * n * 8.0
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"normest n={n}: {cnt} iterations in {ms:.2f} ms, {gbs:.0f} GB/s ({gbs / 8000 * 100:.1f}% of 8 TB/s)")
+    return {"n": n, "iterations": cnt, "estimate": est, "ms": round(ms, 3), "GB_s": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / 8000, 4)}
+
+
 def svd_tail(torch, zl, n, seed):
     """Zolo-SVD (polar + eigensolver + U = U_p V) of the clustered kappa~3 workload at order n."""
     g = torch.Generator(device="cuda")
@@ -392,8 +416,9 @@ def main():
     ap.add_argument("--bounds", default="qr", choices=["qr", "lu"],
                     help="alpha/beta from the QR the first step reuses, or by LU + condition estimate first")
     ap.add_argument("--eig", type=int, default=8192, help="also time the eigenpairs below sigma at this n (0: skip)")
-    ap.add_argument("--part", default="all", choices=["all", "small", "svd", "eig"],
-                    help="run only the batched small-PD or the Zolo-SVD section (for profiling)")
+    ap.add_argument("--part", default="all", choices=["all", "small", "svd", "eig", "normest"],
+                    help="run only one section: the batched small PDs, the Zolo-SVD, the eigenpairs "
+                         "below sigma, or PDNORMEST's bandwidth at --n (for profiling)")
     ap.add_argument("--factors", default="haar", choices=["haar", "reflector"],
                     help="singular-vector factors of the n=16384 workload: Haar (torch QR) or products of two "
                          "random reflections (GEMMs only, for counter runs where the QR library is unavailable)")
@@ -423,6 +448,8 @@ def main():
             out["small_pds"] = small_pds(torch, zl, args.small_count, args.seed)
         elif args.part == "eig":
             out["eig_below"] = eig_below(torch, zl, args.eig, args.seed)
+        elif args.part == "normest":
+            out["normest"] = normest_rate(torch, zl, args.n, args.seed)
         else:
             out["zolo_svd"] = svd_tail(torch, zl, args.svd, args.seed)
         print(json.dumps(out), flush=True)
